@@ -1,0 +1,16 @@
+"""ampsolve: MI355X-native supernodal Cholesky / LDL^T factor and solves behind the reference's solver API."""
+from .sparse import (  # noqa: F401
+    DEFAULT_REAL_FACTOR,
+    NumericFactor,
+    SparseCholesky,
+    SupernodalLDLT,
+    Symbolic,
+    amdOrder,
+    analyze,
+    checkSymbolic,
+    factorNumeric,
+    patternFingerprint,
+    setDefaultRealFactor,
+    solveNumeric,
+    solveNumericMulti,
+)

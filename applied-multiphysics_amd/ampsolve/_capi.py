@@ -1,0 +1,107 @@
+"""ctypes binding of libampsolve.so (include/ampsolve.h). Loading fails loudly when the library is missing."""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libampsolve.so")
+
+AMP_OK = 0
+AMP_ERR_ARG = -1
+AMP_ERR_HIP = -2
+AMP_ERR_STATE = -3
+AMP_ERR_PATTERN = -4
+AMP_ERR_INTERNAL = -5
+AMP_CHOLESKY = 0
+AMP_LDLT = 1
+AMP_F64 = 0
+AMP_F32 = 1
+
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class SymInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nsuper", C.c_int32), ("maxFront", C.c_int32), ("nlevels", C.c_int32),
+                ("nnzL", C.c_int64), ("nnzA", C.c_int64), ("nnzLStored", C.c_int64), ("nnzPattern", C.c_int64),
+                ("flops", C.c_double), ("fill", C.c_double), ("gemmFlops", C.c_double),
+                ("patternKey", C.c_char * 64)]
+
+
+class SymArrays(C.Structure):
+    _fields_ = [(name, C.POINTER(C.c_int32)) for name in
+                ("perm", "pinv", "parent", "colCount", "lowerP", "lowerI", "lowerSrc", "superStart", "colSuper",
+                 "superParent", "childHead", "childNext", "Sp", "Si")] + [("Lxp", C.POINTER(C.c_int64))]
+
+
+class SymDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nsuper", C.c_int32), ("nnzPattern", C.c_int64)] + \
+               [(name, C.c_void_p) for name in ("perm", "parent", "colCount", "superStart", "Sp", "Si", "lowerP",
+                                                 "lowerI", "lowerSrc")] + [("patternKey", C.c_char_p)]
+
+
+class FactorInfo(C.Structure):
+    _fields_ = [("ok", C.c_int32), ("nNeg", C.c_int32), ("nPerturbed", C.c_int32), ("failColumn", C.c_int32),
+                ("ms", C.c_double)]
+
+
+# every entry point of include/ampsolve.h with its ctypes signature
+SIGNATURES = {
+    "amp_amd_order": ([C.c_int32, i32p, i32p, C.c_double, i32p], C.c_int),
+    "amp_analyze": ([C.c_int32, i32p, i32p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
+    "amp_sym_import": ([C.POINTER(SymDesc), C.POINTER(C.c_void_p)], C.c_int),
+    "amp_pattern_fingerprint": ([C.c_int32, i32p, i32p, C.c_char_p], C.c_int),
+    "amp_sym_get_info": ([C.c_void_p, C.POINTER(SymInfo)], C.c_int),
+    "amp_sym_get_arrays": ([C.c_void_p, C.POINTER(SymArrays)], C.c_int),
+    "amp_sym_free": ([C.c_void_p], None),
+    "amp_num_create": ([C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
+    "amp_factor": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.POINTER(FactorInfo)], C.c_int),
+    "amp_factor_device": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.POINTER(FactorInfo)], C.c_int),
+    "amp_solve": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_solve_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_download_factor": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "amp_num_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
+    "amp_num_get_kernel_ms": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
+    "amp_num_free": ([C.c_void_p], None),
+    "amp_last_error": ([], C.c_char_p),
+}
+
+_lib = None
+
+
+class AmpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    """The loaded library; raises when libampsolve.so has not been built (no silent fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libampsolve.so not built at {LIB_PATH}: run `make -C applied-multiphysics_amd` "
+                               "or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != AMP_OK:
+        raise AmpError(rc, lib().amp_last_error().decode())
+    return rc
+
+
+def ptr(a):
+    """Address of a numpy array or a torch tensor (device pointers pass through unchanged)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)

@@ -1,0 +1,178 @@
+"""Synthetic inputs of the benchmark and test configurations (host side, numpy + scipy).
+
+The meshes follow the structured fixtures of packages/physics/src/common/mesh.ts: rectTriMesh (:69-121, alternating
+diagonals) and boxMesh (:123-190, 6 Kuhn tets per hex sharing the 0-6 diagonal, orientation fixed by the volume sign).
+Matrices are P1 stiffness matrices sum_e k_e |T_e| grad(phi_a) . grad(phi_b) on the full symmetric CSR pattern of
+buildElementPattern (sparse.mjs:86-110: every row sorted, diagonal present), restricted to the free DOFs as restrict()
+does (scalar.ts:210-233). Random draws use the reference's LCG s <- (1664525 s + 1013904223) mod 2^32, u = s / 2^32
+(la-real-solve.test.mjs:33-34, eigen.ts:174).
+"""
+import numpy as np
+import scipy.sparse as sp
+
+LCG_A = 1664525
+LCG_C = 1013904223
+M32 = 1 << 32
+
+
+def lcg_u32(seed, count):
+    """The first `count` LCG states after `seed` (uint64 array holding 32-bit values), vectorised by block jumps."""
+    count = int(count)
+    out = np.empty(count, dtype=np.uint64)
+    if count == 0:
+        return out
+    blk = min(count, 4096)
+    s = int(seed) % M32
+    for i in range(blk):
+        s = (LCG_A * s + LCG_C) % M32
+        out[i] = s
+    ja, jc = 1, 0  # affine map of `blk` steps
+    for _ in range(blk):
+        ja, jc = (LCG_A * ja) % M32, (LCG_A * jc + LCG_C) % M32
+    ja64, jc64, m64 = np.uint64(ja), np.uint64(jc), np.uint64(M32)
+    i = blk
+    while i < count:
+        k = min(blk, count - i)
+        out[i:i + k] = ((ja64 * out[i - blk:i - blk + k]) % m64 + jc64) % m64
+        i += k
+    return out
+
+
+def lcg_uniform(seed, count):
+    """u = s / 2^32 in [0, 1)."""
+    return lcg_u32(seed, count).astype(np.float64) / float(M32)
+
+
+def lcg_normal(seed, count):
+    """Box-Muller on consecutive LCG pairs (u1 shifted off zero)."""
+    m = (count + 1) // 2
+    u = lcg_uniform(seed, 2 * m)
+    u1 = 1.0 - u[0::2]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.empty(2 * m)
+    z[0::2] = r * np.cos(2 * np.pi * u2)
+    z[1::2] = r * np.sin(2 * np.pi * u2)
+    return z[:count]
+
+
+# ---- meshes -----------------------------------------------------------------------------------------------------
+
+def rect_tri_mesh(nx, ny):
+    """Unit square, nx x ny cells, 2 P1 triangles per cell with alternating diagonals (mesh.ts:69-91)."""
+    NX, NY = nx, ny
+    j, i = np.meshgrid(np.arange(NY + 1), np.arange(NX + 1), indexing="ij")
+    nodes = np.stack([i.ravel() / NX, j.ravel() / NY], axis=1).astype(np.float64)
+    nid = lambda a, b: b * (NX + 1) + a  # noqa: E731
+    cj, ci = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    ci, cj = ci.ravel(), cj.ravel()
+    flip = (ci + cj) % 2 == 1
+    c00, c10, c11, c01 = nid(ci, cj), nid(ci + 1, cj), nid(ci + 1, cj + 1), nid(ci, cj + 1)
+    t1 = np.where(flip[:, None], np.stack([c00, c10, c11], 1), np.stack([c00, c10, c01], 1))
+    t2 = np.where(flip[:, None], np.stack([c00, c11, c01], 1), np.stack([c10, c11, c01], 1))
+    tris = np.empty((2 * len(ci), 3), dtype=np.int64)
+    tris[0::2] = t1
+    tris[1::2] = t2
+    return nodes, tris
+
+
+_HEX = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+_KUHN = [[0, 1, 2, 6], [0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6], [0, 4, 5, 6], [0, 5, 1, 6]]
+
+
+def _kuhn_corners():
+    """Corner offsets of the 6 Kuhn tets, reordered to positive volume (mesh.ts:138-151)."""
+    out = []
+    for t in _KUHN:
+        cs = _HEX[t].astype(np.float64)
+        u, v, w = cs[1] - cs[0], cs[2] - cs[0], cs[3] - cs[0]
+        if np.dot(u, np.cross(v, w)) < 0:
+            cs = cs[[0, 2, 1, 3]]
+        out.append(cs.astype(np.int64))
+    return np.stack(out)  # (6, 4, 3)
+
+
+def box_tet_mesh(nx, ny, nz):
+    """Unit cube, nx x ny x nz hexes, 6 Kuhn tets each (mesh.ts:123-150). Tets ordered hex by hex (k, j, i)."""
+    NX, NY, NZ = nx, ny, nz
+    kk, jj, ii = np.meshgrid(np.arange(NZ + 1), np.arange(NY + 1), np.arange(NX + 1), indexing="ij")
+    nodes = np.stack([ii.ravel() / NX, jj.ravel() / NY, kk.ravel() / NZ], 1).astype(np.float64)
+    hk, hj, hi = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = np.stack([hi.ravel(), hj.ravel(), hk.ravel()], 1)  # (nh, 3)
+    kc = _kuhn_corners()  # (6, 4, 3)
+    pts = base[:, None, None, :] + kc[None]  # (nh, 6, 4, 3)
+    ids = (pts[..., 2] * (NY + 1) + pts[..., 1]) * (NX + 1) + pts[..., 0]
+    return nodes, ids.reshape(-1, 4)
+
+
+# ---- P1 assembly ---------------------------------------------------------------------------------------------
+
+def p1_element_stiffness(coords):
+    """Geometric P1 stiffness |T| G G^T for simplices given as (ne, d+1, d) coordinates."""
+    d = coords.shape[2]
+    J = np.transpose(coords[:, 1:, :] - coords[:, :1, :], (0, 2, 1))  # columns are edge vectors
+    det = np.linalg.det(J)
+    Jinv = np.linalg.inv(J)  # rows: gradients of barycentrics 1..d
+    G = np.concatenate([-Jinv.sum(axis=1, keepdims=True), Jinv], axis=1)  # (ne, d+1, d)
+    vol = np.abs(det) / (2.0 if d == 2 else 6.0)
+    return vol[:, None, None] * np.einsum("eai,ebi->eab", G, G)
+
+
+def assemble(n, conn, Ke):
+    """Full symmetric CSR (sorted rows, diagonal present) of element matrices Ke (ne, k, k) on conn (ne, k): the
+    pattern of buildElementPattern (sparse.mjs:86-110) with the element entries summed into their slots."""
+    k = conn.shape[1]
+    rows = np.repeat(conn, k, axis=1).ravel().astype(np.int64)
+    cols = np.tile(conn, (1, k)).ravel().astype(np.int64)
+    keys = np.concatenate([rows * n + cols, np.arange(n, dtype=np.int64) * (n + 1)])
+    ukeys = np.unique(keys)
+    slot = np.searchsorted(ukeys, rows * n + cols)
+    data = np.bincount(slot, weights=Ke.ravel(), minlength=len(ukeys))
+    r, c = ukeys // n, ukeys % n
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(r, minlength=n), out=indptr[1:])
+    return sp.csr_matrix((data, c.astype(np.int32), indptr), shape=(n, n))
+
+
+def restrict(A, free_mask):
+    """Rows / columns with free_mask, order kept (scalar.ts:210-233)."""
+    idx = np.nonzero(free_mask)[0]
+    R = A[idx][:, idx].tocsr()
+    R.sort_indices()
+    return R, idx
+
+
+def csr_dict(A):
+    A = A.tocsr()
+    A.sort_indices()
+    return {"n": A.shape[0], "rowptr": A.indptr.astype(np.int32), "colidx": A.indices.astype(np.int32),
+            "nnz": int(A.nnz)}
+
+
+def poisson2d(nx=64, seed=1):
+    """Config: 2D Poisson, P1 triangles on the unit square, nx x nx cells, Dirichlet on every edge, k = 1,
+    RHS U[-1, 1] from the LCG. Returns (pat, values, b)."""
+    nodes, tris = rect_tri_mesh(nx, nx)
+    Ke = p1_element_stiffness(nodes[tris])
+    A = assemble(len(nodes), tris, Ke)
+    x, y = nodes[:, 0], nodes[:, 1]
+    eps = 1e-12
+    free = (x > eps) & (x < 1 - eps) & (y > eps) & (y < 1 - eps)
+    R, _ = restrict(A, free)
+    b = 2.0 * lcg_uniform(seed, R.shape[0]) - 1.0
+    return csr_dict(R), R.data.astype(np.float64), b
+
+
+def heat3d(ncell=60, seed=1, rhs_seed=2):
+    """Config: 3D steady heat, P1 tets on the unit cube, ncell^3 hexes x 6 Kuhn tets, per-element conductivity
+    10^(2u) (log-uniform in [1, 100]) from the LCG, Dirichlet on z = 0. Returns (pat, values, b)."""
+    nodes, tets = box_tet_mesh(ncell, ncell, ncell)
+    kc = _kuhn_corners().astype(np.float64) / ncell
+    Kref = p1_element_stiffness(kc)  # the 6 Kuhn shapes: every hex has the same geometry
+    kel = 10.0 ** (2.0 * lcg_uniform(seed, len(tets)))
+    Ke = Kref[np.arange(len(tets)) % 6] * kel[:, None, None]
+    A = assemble(len(nodes), tets, Ke)
+    free = nodes[:, 2] > 1e-12
+    R, _ = restrict(A, free)
+    b = 2.0 * lcg_uniform(rhs_seed, R.shape[0]) - 1.0
+    return csr_dict(R), R.data.astype(np.float64), b

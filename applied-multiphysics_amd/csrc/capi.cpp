@@ -1,0 +1,538 @@
+// C ABI of the library (include/ampsolve.h): handles, HBM storage, plan upload and the factor / solve drivers.
+#include "../../include/ampsolve.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "factor_kernels.h"
+#include "plan.h"
+#include "symbolic.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                                    \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return fail(AMP_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_));     \
+  } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    return hipMalloc(&p, std::max<size_t>(b, 256));
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <typename T>
+hipError_t upload(DevBuf& d, const std::vector<T>& v) {
+  hipError_t e = d.alloc(v.size() * sizeof(T));
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+struct amp_sym {
+  amp::Symbolic S;
+  amp::FactorPlan F;
+  amp::SolvePlan SP;
+  std::vector<int32_t> diagSrc;  // CSR slot of each diagonal entry (new column order), -1 when absent
+  bool onDevice = false;
+  // device copies of the structure and of the task arrays
+  DevBuf superStart, Sp, Si, Lxp, Uoff, Woff, childHead, childNext, rel, lowerP, lowerSrc, amap, adst, perm, dDiagSrc;
+  DevBuf zeros, smallFronts, ea, diag, gemm, sFronts, sBlocks;
+  amp::DeviceFactorPlan dfp;
+  amp::DeviceSolvePlan dsp;
+};
+
+struct amp_num {
+  amp_sym* sym = nullptr;
+  int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv;
+  bool ok = false;
+  bool profiling = false;
+  amp::ProfileSink prof;
+  std::map<std::string, std::pair<double, int32_t>> kernelMs;
+  ~amp_num() {
+    for (hipEvent_t e : prof.events) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void finishSym(amp_sym* h) {
+  const amp::Symbolic& S = h->S;
+  h->F = amp::buildFactorPlan(S);
+  h->SP = amp::buildSolvePlan(S, h->F);
+  h->diagSrc.assign(S.n, -1);
+  for (int32_t j = 0; j < S.n; j++)
+    for (int32_t q = S.lowerP[j]; q < S.lowerP[j + 1]; q++)
+      if (S.lowerI[q] == j) h->diagSrc[j] = S.lowerSrc[q];
+}
+
+int symToDevice(amp_sym* h) {
+  if (h->onDevice) return AMP_OK;
+  const amp::Symbolic& S = h->S;
+  const amp::FactorPlan& F = h->F;
+  HIP_TRY(upload(h->superStart, S.superStart));
+  HIP_TRY(upload(h->Sp, S.Sp));
+  HIP_TRY(upload(h->Si, S.Si));
+  HIP_TRY(upload(h->Lxp, S.Lxp));
+  HIP_TRY(upload(h->Uoff, F.Uoff));
+  HIP_TRY(upload(h->Woff, F.Woff));
+  HIP_TRY(upload(h->childHead, S.childHead));
+  HIP_TRY(upload(h->childNext, S.childNext));
+  HIP_TRY(upload(h->rel, F.rel));
+  HIP_TRY(upload(h->lowerP, S.lowerP));
+  HIP_TRY(upload(h->lowerSrc, S.lowerSrc));
+  HIP_TRY(upload(h->amap, F.amap));
+  HIP_TRY(upload(h->adst, F.adst));
+  HIP_TRY(upload(h->perm, S.perm));
+  HIP_TRY(upload(h->dDiagSrc, h->diagSrc));
+  HIP_TRY(upload(h->zeros, F.zeros));
+  HIP_TRY(upload(h->smallFronts, F.smallFronts));
+  HIP_TRY(upload(h->ea, F.ea));
+  HIP_TRY(upload(h->diag, F.diag));
+  HIP_TRY(upload(h->gemm, F.gemm));
+  HIP_TRY(upload(h->sFronts, h->SP.fronts));
+  HIP_TRY(upload(h->sBlocks, h->SP.blocks));
+  h->dfp.zeros = h->zeros.as<amp::ZeroTask>();
+  h->dfp.adst = h->adst.as<int64_t>();
+  h->dfp.smallFronts = h->smallFronts.as<int32_t>();
+  h->dfp.ea = h->ea.as<amp::EaTask>();
+  h->dfp.diag = h->diag.as<amp::DiagTask>();
+  h->dfp.gemm = h->gemm.p;
+  h->dfp.launches = F.launches;
+  h->dsp.fronts = h->sFronts.as<int32_t>();
+  h->dsp.blocks = h->sBlocks.as<amp::BlockTask>();
+  h->dsp.launches = h->SP.launches;
+  h->dsp.fvEntries = h->SP.fvEntries;
+  h->onDevice = true;
+  return AMP_OK;
+}
+
+template <typename T>
+amp::FactorArgs<T> factorArgs(amp_num* num) {
+  amp_sym* h = num->sym;
+  amp::FactorArgs<T> a{};
+  a.Lx = num->Lx.as<T>();
+  a.U = num->U.as<T>();
+  a.W = num->W.as<T>();
+  a.D = num->D.as<T>();
+  a.pert = num->pert.as<uint8_t>();
+  a.failCol = num->failCol.as<int32_t>();
+  a.superStart = h->superStart.as<int32_t>();
+  a.Sp = h->Sp.as<int32_t>();
+  a.Lxp = h->Lxp.as<int64_t>();
+  a.Uoff = h->Uoff.as<int64_t>();
+  a.Woff = h->Woff.as<int64_t>();
+  a.childHead = h->childHead.as<int32_t>();
+  a.childNext = h->childNext.as<int32_t>();
+  a.rel = h->rel.as<int32_t>();
+  a.Si = h->Si.as<int32_t>();
+  a.lowerP = h->lowerP.as<int32_t>();
+  a.lowerSrc = h->lowerSrc.as<int32_t>();
+  a.amap = h->amap.as<int32_t>();
+  a.ldlt = num->mode == AMP_LDLT;
+  a.tau = 0;
+  return a;
+}
+
+template <typename T>
+amp::SolveArgs<T> solveArgs(amp_num* num) {
+  amp_sym* h = num->sym;
+  amp::SolveArgs<T> a{};
+  a.Lx = num->Lx.as<T>();
+  a.W = num->W.as<T>();
+  a.D = num->mode == AMP_LDLT ? num->D.as<T>() : nullptr;
+  a.superStart = h->superStart.as<int32_t>();
+  a.Sp = h->Sp.as<int32_t>();
+  a.Si = h->Si.as<int32_t>();
+  a.Lxp = h->Lxp.as<int64_t>();
+  a.Woff = h->Woff.as<int64_t>();
+  a.childHead = h->childHead.as<int32_t>();
+  a.childNext = h->childNext.as<int32_t>();
+  a.rel = h->rel.as<int32_t>();
+  a.perm = h->perm.as<int32_t>();
+  a.n = h->S.n;
+  a.ldlt = num->mode == AMP_LDLT;
+  return a;
+}
+
+// max |a_jj| over the diagonal (diagScale, ldlt.mjs:419-424), computed on the host from a copy of the diagonal
+__global__ void k_gather_diag(const double* __restrict__ values, const int32_t* __restrict__ diagSrc,
+                              double* __restrict__ out, int32_t n) {
+  for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    out[j] = diagSrc[j] >= 0 ? values[diagSrc[j]] : 0.0;
+}
+
+template <typename T>
+int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_info* info) {
+  amp_sym* h = num->sym;
+  const int32_t n = h->S.n;
+  amp::FactorArgs<T> a = factorArgs<T>(num);
+  if (num->mode == AMP_LDLT && n > 0) {
+    // tau = pivotTol * max |a_jj| needs the diagonal on the host before the launch sequence
+    std::vector<double> diag(n);
+    hipLaunchKernelGGL(k_gather_diag, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, num->stream, d_values,
+                       h->dDiagSrc.as<int32_t>(), num->y.as<double>(), n);
+    HIP_TRY(hipMemcpyAsync(diag.data(), num->y.p, size_t(n) * 8, hipMemcpyDeviceToHost, num->stream));
+    HIP_TRY(hipStreamSynchronize(num->stream));
+    double amax = 0;
+    for (double d : diag) amax = std::max(amax, std::fabs(d));
+    a.tau = pivotTol * (amax > 0 ? amax : 1);
+  }
+  HIP_TRY(hipEventRecord(num->ev0, num->stream));
+  if (n > 0) amp::launchFactorPlan<T>(h->dfp, a, d_values, num->stream, num->profiling ? &num->prof : nullptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(num->ev1, num->stream));
+  int32_t counts[2] = {0, 0}, failCol = INT32_MAX;
+  if (n > 0) {
+    if (num->mode == AMP_LDLT) amp::launchLdltCounts<T>(a, n, num->counts.as<int32_t>(), num->stream);
+    HIP_TRY(hipMemcpyAsync(&failCol, num->failCol.p, 4, hipMemcpyDeviceToHost, num->stream));
+    HIP_TRY(hipMemcpyAsync(counts, num->counts.p, 8, hipMemcpyDeviceToHost, num->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
+  num->kernelMs.clear();
+  if (num->profiling) {
+    static const char* names[] = {"zero", "assemble", "small_front", "extend_add", "diag_block", "gemm_tile"};
+    for (size_t i = 0; i < num->prof.used; i++) {
+      float k = 0;
+      HIP_TRY(hipEventElapsedTime(&k, num->prof.events[2 * i], num->prof.events[2 * i + 1]));
+      auto& slot = num->kernelMs[names[num->prof.kinds[i]]];
+      slot.first += k;
+      slot.second += 1;
+    }
+  }
+  num->ok = failCol == INT32_MAX;
+  if (info) {
+    info->ok = num->ok ? 1 : 0;
+    info->failColumn = num->ok ? -1 : failCol;
+    info->nNeg = num->mode == AMP_LDLT ? counts[0] : 0;
+    info->nPerturbed = num->mode == AMP_LDLT ? counts[1] : 0;
+    info->ms = ms;
+  }
+  return AMP_OK;
+}
+
+int ensureVecs(amp_num* num, int32_t nrhs) {
+  const amp_sym* h = num->sym;
+  const size_t n = size_t(std::max(h->S.n, 1)) * nrhs;
+  const size_t es = num->prec == AMP_F64 ? 8 : 4;
+  if (num->b.bytes < n * 8) {
+    HIP_TRY(num->b.alloc(n * 8));
+    HIP_TRY(num->x.alloc(n * 8));
+  }
+  if (num->y.bytes < std::max(n * es, size_t(h->S.n) * 8)) HIP_TRY(num->y.alloc(std::max(n * es, size_t(h->S.n) * 8)));
+  const size_t fvn = size_t(std::max<int64_t>(h->SP.fvEntries, 1)) * nrhs * es;
+  if (num->fv.bytes < fvn) HIP_TRY(num->fv.alloc(fvn));
+  return AMP_OK;
+}
+
+int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
+  if (!num->ok) return fail(AMP_ERR_STATE, "solve: no successful numeric factorization");
+  int rc = ensureVecs(num, nrhs);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(num->ev0, num->stream));
+  if (num->prec == AMP_F64)
+    amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs, num->y.as<double>(),
+                                         num->fv.as<double>(), num->stream);
+  else
+    amp::launchSolvePlan<float, float>(num->sym->dsp, solveArgs<float>(num), d_b, d_x, nrhs, num->y.as<float>(),
+                                       num->fv.as<float>(), num->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(num->ev1, num->stream));
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  return AMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* amp_last_error(void) { return g_error.c_str(); }
+
+int amp_amd_order(int32_t n, const int32_t* rowptr, const int32_t* colidx, double denseFactor, int32_t* perm) {
+  if (n < 0 || (n > 0 && (!rowptr || !colidx || !perm))) return fail(AMP_ERR_ARG, "amp_amd_order: bad arguments");
+  if (!amp::amdOrder(n, rowptr, colidx, denseFactor, perm)) return fail(AMP_ERR_INTERNAL, "amdOrder: internal error");
+  return AMP_OK;
+}
+
+int amp_pattern_fingerprint(int32_t n, const int32_t* rowptr, const int32_t* colidx, char* out) {
+  if (n < 0 || !rowptr || (n > 0 && !colidx) || !out) return fail(AMP_ERR_ARG, "amp_pattern_fingerprint: bad arguments");
+  const std::string k = amp::patternFingerprint(n, rowptr, colidx);
+  std::snprintf(out, 48, "%s", k.c_str());
+  return AMP_OK;
+}
+
+int amp_analyze(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax,
+                amp_sym** out) {
+  if (!out || n < 0 || !rowptr || (n > 0 && !colidx)) return fail(AMP_ERR_ARG, "amp_analyze: bad arguments");
+  *out = nullptr;
+  std::unique_ptr<amp_sym> h(new amp_sym());
+  if (!amp::analyzePattern(n, rowptr, colidx, perm, relax != 0, h->S))
+    return fail(AMP_ERR_INTERNAL, "analyze: amdOrder internal error");
+  finishSym(h.get());
+  *out = h.release();
+  return AMP_OK;
+}
+
+int amp_sym_import(const amp_sym_desc* d, amp_sym** out) {
+  if (!d || !out || d->n < 0 || d->nsuper < 0) return fail(AMP_ERR_ARG, "amp_sym_import: bad arguments");
+  *out = nullptr;
+  const int32_t n = d->n, ns = d->nsuper;
+  if (n > 0 && (!d->perm || !d->parent || !d->superStart || !d->Sp || !d->Si || !d->lowerP || !d->lowerI || !d->lowerSrc))
+    return fail(AMP_ERR_ARG, "amp_sym_import: missing array");
+  std::unique_ptr<amp_sym> h(new amp_sym());
+  amp::Symbolic& S = h->S;
+  S.n = n;
+  S.nsuper = ns;
+  S.nnzPattern = d->nnzPattern;
+  S.patternKey = d->patternKey ? d->patternKey : "";
+  S.perm.assign(d->perm, d->perm + n);
+  S.pinv.resize(n);
+  for (int32_t i = 0; i < n; i++) {
+    if (S.perm[i] < 0 || S.perm[i] >= n) return fail(AMP_ERR_ARG, "amp_sym_import: perm out of range");
+    S.pinv[S.perm[i]] = i;
+  }
+  S.parent.assign(d->parent, d->parent + n);
+  S.superStart.assign(d->superStart, d->superStart + ns + 1);
+  S.Sp.assign(d->Sp, d->Sp + ns + 1);
+  S.Si.assign(d->Si, d->Si + (ns ? d->Sp[ns] : 0));
+  S.lowerP.assign(d->lowerP, d->lowerP + n + 1);
+  S.nnzA = n ? d->lowerP[n] : 0;
+  S.lowerI.assign(d->lowerI, d->lowerI + S.nnzA);
+  S.lowerSrc.assign(d->lowerSrc, d->lowerSrc + S.nnzA);
+  if (ns > 0 && (S.superStart[ns] != n || S.superStart[0] != 0)) return fail(AMP_ERR_ARG, "amp_sym_import: superStart");
+  S.colSuper.resize(n);
+  for (int32_t s = 0; s < ns; s++)
+    for (int32_t j = S.superStart[s]; j < S.superStart[s + 1]; j++) S.colSuper[j] = s;
+  S.superParent.resize(ns);
+  for (int32_t s = 0; s < ns; s++) {
+    const int32_t pl = S.parent[S.superStart[s + 1] - 1];
+    S.superParent[s] = pl < 0 ? -1 : S.colSuper[pl];
+    if (S.superParent[s] != -1 && S.superParent[s] <= s) return fail(AMP_ERR_ARG, "amp_sym_import: tree not postordered");
+  }
+  S.childHead.assign(ns, -1);
+  S.childNext.assign(ns, -1);
+  for (int32_t s = 0; s < ns; s++) {
+    const int32_t P = S.superParent[s];
+    if (P == -1) continue;
+    S.childNext[s] = S.childHead[P];
+    S.childHead[P] = s;
+  }
+  S.Lxp.assign(ns + 1, 0);
+  S.maxFront = 0;
+  for (int32_t s = 0; s < ns; s++) {
+    const int32_t m = S.Sp[s + 1] - S.Sp[s], nc = S.superStart[s + 1] - S.superStart[s];
+    if (m < nc) return fail(AMP_ERR_ARG, "amp_sym_import: Sp shorter than the supernode");
+    S.Lxp[s + 1] = S.Lxp[s] + int64_t(m) * nc;
+    S.maxFront = std::max(S.maxFront, m);
+  }
+  S.nnzLStored = S.Lxp[ns];
+  S.colCount.assign(n, 0);
+  S.nnzL = 0;
+  S.flops = 0;
+  for (int32_t s = 0; s < ns; s++) {
+    const int32_t m = S.Sp[s + 1] - S.Sp[s];
+    for (int32_t j = S.superStart[s]; j < S.superStart[s + 1]; j++) {
+      const int32_t cc = d->colCount ? d->colCount[j] : m - (j - S.superStart[s]);
+      S.colCount[j] = cc;
+      S.nnzL += cc;
+      S.flops += double(cc) * cc;
+    }
+  }
+  S.fill = S.nnzA ? double(S.nnzL) / double(S.nnzA) : 1;
+  finishSym(h.get());
+  *out = h.release();
+  return AMP_OK;
+}
+
+int amp_sym_get_info(const amp_sym* h, amp_sym_info* info) {
+  if (!h || !info) return fail(AMP_ERR_ARG, "amp_sym_get_info: bad arguments");
+  const amp::Symbolic& S = h->S;
+  info->n = S.n;
+  info->nsuper = S.nsuper;
+  info->maxFront = S.maxFront;
+  info->nlevels = h->F.nlevels;
+  info->nnzL = S.nnzL;
+  info->nnzA = S.nnzA;
+  info->nnzLStored = S.nnzLStored;
+  info->nnzPattern = S.nnzPattern;
+  info->flops = S.flops;
+  info->fill = S.fill;
+  info->gemmFlops = h->F.gemmFlops;
+  std::snprintf(info->patternKey, sizeof info->patternKey, "%s", S.patternKey.c_str());
+  return AMP_OK;
+}
+
+int amp_sym_get_arrays(const amp_sym* h, amp_sym_arrays* a) {
+  if (!h || !a) return fail(AMP_ERR_ARG, "amp_sym_get_arrays: bad arguments");
+  const amp::Symbolic& S = h->S;
+  a->perm = S.perm.data();
+  a->pinv = S.pinv.data();
+  a->parent = S.parent.data();
+  a->colCount = S.colCount.data();
+  a->lowerP = S.lowerP.data();
+  a->lowerI = S.lowerI.data();
+  a->lowerSrc = S.lowerSrc.data();
+  a->superStart = S.superStart.data();
+  a->colSuper = S.colSuper.data();
+  a->superParent = S.superParent.data();
+  a->childHead = S.childHead.data();
+  a->childNext = S.childNext.data();
+  a->Sp = S.Sp.data();
+  a->Si = S.Si.data();
+  a->Lxp = S.Lxp.data();
+  return AMP_OK;
+}
+
+void amp_sym_free(amp_sym* h) { delete h; }
+
+int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
+  if (!h || !out || (mode != AMP_CHOLESKY && mode != AMP_LDLT) || (precision != AMP_F64 && precision != AMP_F32))
+    return fail(AMP_ERR_ARG, "amp_num_create: bad arguments");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(AMP_ERR_HIP, "amp_num_create: no HIP device");
+  int rc = symToDevice(h);
+  if (rc) return rc;
+  std::unique_ptr<amp_num> num(new amp_num());
+  num->sym = h;
+  num->mode = mode;
+  num->prec = precision;
+  const size_t es = precision == AMP_F64 ? 8 : 4;
+  const amp::FactorPlan& F = h->F;
+  HIP_TRY(hipStreamCreateWithFlags(&num->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&num->ev0));
+  HIP_TRY(hipEventCreate(&num->ev1));
+  HIP_TRY(num->Lx.alloc(size_t(F.nnzLStored) * es));
+  HIP_TRY(num->U.alloc(size_t(F.uEntries) * es));
+  HIP_TRY(num->W.alloc(size_t(F.wEntries) * es));
+  HIP_TRY(num->D.alloc(size_t(std::max(h->S.n, 1)) * es));
+  HIP_TRY(num->pert.alloc(size_t(std::max(h->S.n, 1))));
+  HIP_TRY(num->failCol.alloc(4));
+  HIP_TRY(num->counts.alloc(8));
+  HIP_TRY(num->values.alloc(size_t(std::max<int64_t>(h->S.nnzPattern, 1)) * 8));
+  rc = ensureVecs(num.get(), 1);
+  if (rc) return rc;
+  *out = num.release();
+  return AMP_OK;
+}
+
+int amp_factor_device(amp_num* num, const double* d_values, int64_t nvalues, double pivotTol, amp_factor_info* info) {
+  if (!num || (nvalues > 0 && !d_values)) return fail(AMP_ERR_ARG, "amp_factor: bad arguments");
+  if (nvalues != num->sym->S.nnzPattern)
+    return fail(AMP_ERR_PATTERN, "factorNumeric: " + std::to_string(nvalues) + " values for a pattern with " +
+                                     std::to_string(num->sym->S.nnzPattern) + " entries");
+  num->ok = false;
+  return num->prec == AMP_F64 ? runFactor<double>(num, d_values, pivotTol, info)
+                              : runFactor<float>(num, d_values, pivotTol, info);
+}
+
+int amp_factor(amp_num* num, const double* values, int64_t nvalues, double pivotTol, amp_factor_info* info) {
+  if (!num || (nvalues > 0 && !values)) return fail(AMP_ERR_ARG, "amp_factor: bad arguments");
+  if (nvalues != num->sym->S.nnzPattern)
+    return fail(AMP_ERR_PATTERN, "factorNumeric: " + std::to_string(nvalues) + " values for a pattern with " +
+                                     std::to_string(num->sym->S.nnzPattern) + " entries");
+  if (nvalues > 0)
+    HIP_TRY(hipMemcpyAsync(num->values.p, values, size_t(nvalues) * 8, hipMemcpyHostToDevice, num->stream));
+  return amp_factor_device(num, num->values.as<double>(), nvalues, pivotTol, info);
+}
+
+int amp_solve_device(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
+  if (!num || nrhs < 0 || (nrhs > 0 && num->sym->S.n > 0 && (!d_b || !d_x))) return fail(AMP_ERR_ARG, "amp_solve: bad arguments");
+  return runSolve(num, d_b, d_x, nrhs);
+}
+
+int amp_solve(amp_num* num, const double* b, double* x, int32_t nrhs) {
+  if (!num || nrhs < 0 || (nrhs > 0 && num->sym->S.n > 0 && (!b || !x))) return fail(AMP_ERR_ARG, "amp_solve: bad arguments");
+  if (!num->ok) return fail(AMP_ERR_STATE, "solve: no successful numeric factorization");
+  const int32_t n = num->sym->S.n;
+  if (n == 0 || nrhs == 0) return AMP_OK;
+  int rc = ensureVecs(num, nrhs);
+  if (rc) return rc;
+  const size_t bytes = size_t(n) * nrhs * 8;
+  HIP_TRY(hipMemcpyAsync(num->b.p, b, bytes, hipMemcpyHostToDevice, num->stream));
+  rc = runSolve(num, num->b.as<double>(), num->x.as<double>(), nrhs);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(x, num->x.p, bytes, hipMemcpyDeviceToHost, num->stream));
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  return AMP_OK;
+}
+
+int amp_download_factor(amp_num* num, double* Lx, double* D) {
+  if (!num || !Lx) return fail(AMP_ERR_ARG, "amp_download_factor: bad arguments");
+  const amp::Symbolic& S = num->sym->S;
+  if (num->prec == AMP_F64) {
+    HIP_TRY(hipMemcpy(Lx, num->Lx.p, size_t(S.nnzLStored) * 8, hipMemcpyDeviceToHost));
+    if (D && num->mode == AMP_LDLT) HIP_TRY(hipMemcpy(D, num->D.p, size_t(S.n) * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(size_t(std::max<int64_t>(S.nnzLStored, int64_t(S.n))));
+    HIP_TRY(hipMemcpy(tmp.data(), num->Lx.p, size_t(S.nnzLStored) * 4, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < S.nnzLStored; i++) Lx[i] = tmp[i];
+    if (D && num->mode == AMP_LDLT) {
+      HIP_TRY(hipMemcpy(tmp.data(), num->D.p, size_t(S.n) * 4, hipMemcpyDeviceToHost));
+      for (int32_t i = 0; i < S.n; i++) D[i] = tmp[i];
+    }
+  }
+  return AMP_OK;
+}
+
+int amp_num_set_profiling(amp_num* num, int32_t enable) {
+  if (!num) return fail(AMP_ERR_ARG, "amp_num_set_profiling: bad arguments");
+  num->profiling = enable != 0;
+  if (num->profiling && num->prof.events.empty()) {
+    const size_t nl = num->sym->dfp.launches.size();
+    num->prof.events.resize(2 * nl + 2, nullptr);
+    num->prof.kinds.resize(nl + 1, 0);
+    for (hipEvent_t& e : num->prof.events) HIP_TRY(hipEventCreate(&e));
+  }
+  return AMP_OK;
+}
+
+int amp_num_get_kernel_ms(amp_num* num, const char* kernel, double* total_ms, int32_t* launches) {
+  if (!num || !kernel || !total_ms || !launches) return fail(AMP_ERR_ARG, "amp_num_get_kernel_ms: bad arguments");
+  auto it = num->kernelMs.find(kernel);
+  *total_ms = it == num->kernelMs.end() ? 0.0 : it->second.first;
+  *launches = it == num->kernelMs.end() ? 0 : it->second.second;
+  return AMP_OK;
+}
+
+void amp_num_free(amp_num* num) { delete num; }
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Launch interface of the factor and solve kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "kernels_common.h"
+#include "plan.h"
+
+namespace amp {
+
+// Device copies of a FactorPlan's task arrays (launch records stay on the host).
+struct DeviceFactorPlan {
+  const ZeroTask* zeros = nullptr;
+  const int64_t* adst = nullptr;
+  const int32_t* smallFronts = nullptr;
+  const EaTask* ea = nullptr;
+  const DiagTask* diag = nullptr;
+  const void* gemm = nullptr;
+  std::vector<Launch> launches;
+};
+
+// Optional per-launch timing: events[2 i], events[2 i + 1] bracket launch i; kinds[i] is its LaunchKind.
+struct ProfileSink {
+  std::vector<hipEvent_t> events;
+  std::vector<int32_t> kinds;
+  size_t used = 0;
+};
+
+template <typename T>
+void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t st,
+                      ProfileSink* prof);
+template <typename T>
+void launchLdltCounts(const FactorArgs<T>& a, int32_t n, int32_t* out, hipStream_t st);
+
+// ---- solves ----
+struct DeviceSolvePlan {
+  const int32_t* fronts = nullptr;
+  const BlockTask* blocks = nullptr;
+  std::vector<SolveLaunch> launches;
+  int64_t fvEntries = 0;
+};
+
+template <typename T>
+struct SolveArgs {
+  const T* Lx;
+  const T* W;
+  const T* D;  // LDL^T pivots, nullptr for Cholesky
+  const int32_t* superStart;
+  const int32_t* Sp;
+  const int32_t* Si;
+  const int64_t* Lxp;
+  const int64_t* Woff;
+  const int32_t* childHead;
+  const int32_t* childNext;
+  const int32_t* rel;
+  const int32_t* perm;
+  int32_t n;
+  int32_t ldlt;
+};
+
+// Solve q systems in place on the device: b (q vectors of n, original order) -> x. y (n q) and fv (fvEntries q) are
+// scratch of the working precision S; Lx / W / D are the factor's precision T.
+template <typename T, typename S>
+void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q, S* y,
+                     S* fv, hipStream_t st);
+
+}  // namespace amp

@@ -1,0 +1,384 @@
+// Device kernels of the multifrontal factorization (Cholesky and LDL^T with static pivoting), gfx950.
+//
+// Every kernel executes one launch of the host plan (plan.h): a list of independent tasks, one workgroup per task.
+// The arithmetic per front is the reference's dense partial factorization (packages/core/src/la/sparse.mjs:992-1098,
+// ldlt.mjs:492-535, pivots as wasm/front-real.mjs:367-389), reorganised as a blocked right-looking factorization
+// whose trailing products run on the matrix cores.
+#include "factor_kernels.h"
+
+namespace amp {
+
+namespace {
+
+// Pivot of column k (value d already updated by every earlier column). Cholesky: d -> sqrt(d), failure unless d > 0.
+// LDL^T: |d| < tau is replaced by +-tau (counted), a non-finite small pivot fails; the column keeps a unit diagonal.
+// Writes the scaling factor to *inv and the weight multiplier (1 or d) to *mult. Failures only record the column:
+// the factorization runs to the end with the IEEE result and the host reports the first failing column.
+template <typename T>
+__device__ inline T pivotValue(T d, int32_t col, const FactorArgs<T>& a, T* inv, T* mult) {
+  if (!a.ldlt) {
+    if (!(d > T(0))) atomicMin(a.failCol, col);
+    const T sq = sqrt(d);
+    *inv = T(1) / sq;
+    *mult = T(1);
+    return sq;
+  }
+  const T tau = T(a.tau);
+  uint8_t perturbed = 0;
+  if (!(fabs(d) >= tau)) {
+    if (!isfinite(d))
+      atomicMin(a.failCol, col);
+    else {
+      d = d < T(0) ? -tau : tau;
+      perturbed = 1;
+    }
+  }
+  a.pert[col] = perturbed;
+  a.D[col] = d;
+  *inv = T(1) / d;
+  *mult = d;
+  return T(1);
+}
+
+}  // namespace
+
+template <typename T>
+__global__ void k_zero(const ZeroTask* __restrict__ tasks, T* __restrict__ Lx) {
+  const ZeroTask t = tasks[blockIdx.x];
+  T* p = Lx + t.off;
+  for (int64_t i = threadIdx.x; i < t.len; i += blockDim.x) p[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_assemble(const int64_t* __restrict__ adst, const int32_t* __restrict__ lowerSrc,
+                           const double* __restrict__ values, T* __restrict__ Lx, int64_t count) {
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < count; q += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t d = adst[q];
+    if (d >= 0) Lx[d] = T(0.0 + values[lowerSrc[q]]);
+  }
+}
+
+// Whole front in LDS: assemble A's columns, extend-add the children (in the reference's child order), factor the nc
+// pivot columns right-looking, store the m x nc block and the mu x mu update matrix.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ fronts, FactorArgs<T> a,
+                                                    const double* __restrict__ values, int32_t maxM) {
+  extern __shared__ unsigned char smem[];
+  T* F = reinterpret_cast<T*>(smem);
+  T* wcol = F + maxM * maxM;
+  __shared__ T s_inv, s_mult;
+  const int tid = threadIdx.x;
+  const int32_t s = fronts[blockIdx.x];
+  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
+  for (int idx = tid; idx < m * m; idx += NT) F[idx] = T(0);
+  __syncthreads();
+  for (int32_t q = a.lowerP[f] + tid; q < a.lowerP[f + nc]; q += NT) F[a.amap[q]] = T(0.0 + values[a.lowerSrc[q]]);
+  __syncthreads();
+  for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t* rel = a.rel + a.Sp[c] + ncc;
+    const T* Uc = a.U + a.Uoff[c];
+    for (int idx = tid; idx < muc * muc; idx += NT) {
+      const int r = idx % muc, b = idx / muc;
+      if (r >= b) F[rel[r] + rel[b] * m] += Uc[idx];
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k < nc; k++) {
+    if (tid == 0) {
+      T inv, mult;
+      F[k + k * m] = pivotValue<T>(F[k + k * m], f + k, a, &inv, &mult);
+      s_inv = inv;
+      s_mult = mult;
+    }
+    __syncthreads();
+    const T inv = s_inv, mult = s_mult;
+    for (int i = k + 1 + tid; i < m; i += NT) {
+      const T l = F[i + k * m] * inv;
+      F[i + k * m] = l;
+      wcol[i] = l * mult;
+    }
+    __syncthreads();
+    const int w = m - k - 1;
+    for (int idx = tid; idx < w * w; idx += NT) {
+      const int ii = idx % w, jj = idx / w;
+      if (ii >= jj) {
+        const int i = k + 1 + ii, j = k + 1 + jj;
+        F[i + j * m] -= F[i + k * m] * wcol[j];
+      }
+    }
+    __syncthreads();
+  }
+  T* L = a.Lx + a.Lxp[s];
+  for (int idx = tid; idx < m * nc; idx += NT) L[idx] = F[idx];
+  if (mu > 0) {
+    T* Us = a.U + a.Uoff[s];
+    for (int idx = tid; idx < mu * mu; idx += NT) {
+      const int r = idx % mu, b = idx / mu;
+      Us[idx] = F[(nc + r) + (nc + b) * m];
+    }
+  }
+}
+
+// Extend-add into parent columns [j0, j1) of a tiled front. Each child column maps to one parent column through the
+// increasing relative map, so the child columns of the range are contiguous; children are added one after another
+// (reference order), each child's entries land on distinct parent entries.
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks, FactorArgs<T> a) {
+  const EaTask t = tasks[blockIdx.x];
+  const int32_t s = t.s;
+  const int32_t nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
+  T* L = a.Lx + a.Lxp[s];
+  T* Us = mu > 0 ? a.U + a.Uoff[s] : nullptr;
+  for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t* rel = a.rel + a.Sp[c] + ncc;
+    const T* Uc = a.U + a.Uoff[c];
+    int32_t lo = 0, hi = muc;  // first b with rel[b] >= j0
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (rel[mid] < t.j0) lo = mid + 1; else hi = mid;
+    }
+    const int32_t b0 = lo;
+    hi = muc;
+    while (lo < hi) {  // first b with rel[b] >= j1
+      const int32_t mid = (lo + hi) >> 1;
+      if (rel[mid] < t.j1) lo = mid + 1; else hi = mid;
+    }
+    const int32_t nb = lo - b0;
+    const int64_t total = int64_t(nb) * muc;
+    for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int32_t r = int32_t(idx % muc), b = b0 + int32_t(idx / muc);
+      if (r < b) continue;
+      const int32_t i = rel[r], j = rel[b];
+      const T v = Uc[r + int64_t(b) * muc];
+      if (j < nc)
+        L[i + int64_t(j) * m] += v;
+      else
+        Us[(i - nc) + int64_t(j - nc) * mu] += v;
+    }
+    __syncthreads();
+  }
+}
+
+// Diagonal block of a tiled front: right-looking factorization in LDS, then the inverse of its lower factor
+// (one column per thread, forward substitution) for the in-place panel solve and for the solves.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
+  constexpr int E = kTileEdge, P = kTileEdge + 1;
+  __shared__ T Fs[E * P];
+  __shared__ T Ws[E * P];
+  __shared__ T wcol[E];
+  __shared__ T s_inv, s_mult;
+  const DiagTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, nb = t.nb;
+  T* F = a.Lx + t.f_off;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int i = idx % nb, j = idx / nb;
+    Fs[i + j * P] = i >= j ? F[i + int64_t(j) * t.ld] : T(0);
+  }
+  __syncthreads();
+  for (int k = 0; k < nb; k++) {
+    if (tid == 0) {
+      T inv, mult;
+      Fs[k + k * P] = pivotValue<T>(Fs[k + k * P], t.col0 + k, a, &inv, &mult);
+      s_inv = inv;
+      s_mult = mult;
+    }
+    __syncthreads();
+    const T inv = s_inv, mult = s_mult;
+    for (int i = k + 1 + tid; i < nb; i += 256) {
+      const T l = Fs[i + k * P] * inv;
+      Fs[i + k * P] = l;
+      wcol[i] = l * mult;
+    }
+    __syncthreads();
+    const int w = nb - k - 1;
+    for (int idx = tid; idx < w * w; idx += 256) {
+      const int ii = idx % w, jj = idx / w;
+      if (ii >= jj) Fs[(k + 1 + ii) + (k + 1 + jj) * P] -= Fs[(k + 1 + ii) + k * P] * wcol[k + 1 + jj];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int i = idx % nb, j = idx / nb;
+    if (i >= j) F[i + int64_t(j) * t.ld] = Fs[i + j * P];
+  }
+  if (tid < nb) {
+    const int j = tid;
+    for (int i = 0; i < j; i++) Ws[i + j * P] = T(0);
+    for (int i = j; i < nb; i++) {
+      T sum = i == j ? T(1) : T(0);
+      for (int k = j; k < i; k++) sum -= Fs[i + k * P] * Ws[k + j * P];
+      Ws[i + j * P] = a.ldlt ? sum : sum / Fs[i + i * P];
+    }
+  }
+  __syncthreads();
+  T* W = a.W + t.w_off;
+  for (int idx = tid; idx < E * E; idx += 256) {
+    const int i = idx % E, j = idx / E;
+    W[idx] = (i < nb && j < nb) ? Ws[i + j * P] : T(0);
+  }
+}
+
+// 64 x 64 output tile of C (+)= -A diag(d) B^T (modes 0, 1) or C = A B^T diag(d)^-1 (mode 2) on the matrix cores.
+// Four waves, each owning a 32 x 32 quarter (2 x 2 MFMA blocks of 16 x 16); depth staged through LDS in chunks of 16
+// (A and B stored depth-major so a lane's operand is one LDS read); the tile is written back through LDS so every
+// global access is along columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U, T* W,
+                                                  const T* __restrict__ D, int32_t ldlt) {
+  using V4 = typename Vec4<T>::type;
+  constexpr int E = kTileEdge, KC = 16, PA = kTileEdge + 4, PC = kTileEdge + 1;
+  __shared__ T As[KC * PA];
+  __shared__ T Bs[KC * PA];
+  __shared__ T Cs[E * PC];
+  const GemmTask t = tasks[blockIdx.x];
+  T* const bases[3] = {Lx, U, W};
+  const T* A = bases[t.a_buf] + t.a_off;
+  const T* B = bases[t.b_buf] + t.b_off;
+  T* C = bases[t.c_buf] + t.c_off;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+  const bool scaleDepth = ldlt && t.mode != 2;
+  V4 acc[2][2];
+  for (int bi = 0; bi < 2; bi++)
+    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = V4{0, 0, 0, 0};
+  const int li = tid & 63, lk = tid >> 6;  // loader: row li, depth lk + 4 r
+  for (int k0 = 0; k0 < t.k; k0 += KC) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int kk = lk + 4 * r, k = k0 + kk;
+      const bool kin = k < t.k;
+      As[kk * PA + li] = (kin && li < t.rows) ? A[li + int64_t(k) * t.lda] : T(0);
+      T bv = (kin && li < t.cols) ? B[li + int64_t(k) * t.ldb] : T(0);
+      if (scaleDepth && kin) bv *= D[t.d_off + k];
+      Bs[kk * PA + li] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      const int kk = k4 + (lane >> 4);
+      T av[2], bv[2];
+      for (int bi = 0; bi < 2; bi++) av[bi] = As[kk * PA + wr + bi * 16 + (lane & 15)];
+      for (int bj = 0; bj < 2; bj++) bv[bj] = Bs[kk * PA + wc + bj * 16 + (lane & 15)];
+      for (int bi = 0; bi < 2; bi++)
+        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = mfma16x16x4(av[bi], bv[bj], acc[bi][bj]);
+    }
+    __syncthreads();
+  }
+  for (int bi = 0; bi < 2; bi++)
+    for (int bj = 0; bj < 2; bj++)
+      for (int r = 0; r < 4; r++) {
+        const int row = wr + bi * 16 + 4 * (lane >> 4) + r, col = wc + bj * 16 + (lane & 15);
+        Cs[row + col * PC] = acc[bi][bj][r];
+      }
+  __syncthreads();
+  for (int idx = tid; idx < E * E; idx += 256) {
+    const int i = idx & 63, j = idx >> 6;
+    if (i >= t.rows || j >= t.cols) continue;
+    T v = Cs[i + j * PC];
+    T& dst = C[i + int64_t(j) * t.ldc];
+    if (t.mode == 0)
+      dst = dst - v;
+    else if (t.mode == 1)
+      dst = -v;
+    else {
+      if (ldlt) v = v / D[t.d_off + j];
+      dst = v;
+    }
+  }
+}
+
+// nNeg / nPerturbed over the columns before the first failure (the counts the reference returns, ldlt.mjs:505-519).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ldlt_counts(const T* __restrict__ D, const uint8_t* __restrict__ pert,
+                                                     const int32_t* __restrict__ failCol, int32_t n,
+                                                     int32_t* __restrict__ out) {
+  __shared__ int32_t sn[256], sp[256];
+  const int32_t lim = min(n, *failCol);
+  int32_t cn = 0, cp = 0;
+  for (int32_t j = threadIdx.x; j < lim; j += 256) {
+    cn += D[j] < T(0);
+    cp += pert[j];
+  }
+  sn[threadIdx.x] = cn;
+  sp[threadIdx.x] = cp;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sn[threadIdx.x] += sn[threadIdx.x + w];
+      sp[threadIdx.x] += sp[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sn[0];
+    out[1] = sp[0];
+  }
+}
+
+__global__ void k_set_int(int32_t* p, int32_t v) { *p = v; }
+
+// ---- launchers ------------------------------------------------------------------------------------------------
+template <typename T>
+void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t st,
+                      ProfileSink* prof) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, a.failCol, INT32_MAX);
+  if (prof) prof->used = 0;
+  for (const Launch& L : dp.launches) {
+    const unsigned grid = unsigned(L.count);
+    if (prof && 2 * prof->used + 1 < prof->events.size()) {
+      (void)hipEventRecord(prof->events[2 * prof->used], st);
+      prof->kinds[prof->used] = L.kind;
+    }
+    switch (L.kind) {
+      case kZero:
+        hipLaunchKernelGGL(k_zero<T>, dim3(grid), dim3(256), 0, st, dp.zeros + L.first, a.Lx);
+        break;
+      case kAssemble: {
+        const unsigned g = unsigned(std::min<int64_t>((L.count + 255) / 256, 65536));
+        hipLaunchKernelGGL(k_assemble<T>, dim3(g), dim3(256), 0, st, dp.adst, a.lowerSrc, values, a.Lx, L.count);
+        break;
+      }
+      case kSmall: {
+        const size_t lds = (size_t(L.maxM) * L.maxM + L.maxM) * sizeof(T);
+        if (L.threads == 64)
+          hipLaunchKernelGGL((k_small_front<T, 64>), dim3(grid), dim3(64), lds, st, dp.smallFronts + L.first, a, values,
+                             L.maxM);
+        else
+          hipLaunchKernelGGL((k_small_front<T, 256>), dim3(grid), dim3(256), lds, st, dp.smallFronts + L.first, a,
+                             values, L.maxM);
+        break;
+      }
+      case kExtendAdd:
+        hipLaunchKernelGGL(k_extend_add<T>, dim3(grid), dim3(256), 0, st, dp.ea + L.first, a);
+        break;
+      case kDiag:
+        hipLaunchKernelGGL(k_diag_block<T>, dim3(grid), dim3(256), 0, st, dp.diag + L.first, a);
+        break;
+      case kGemm:
+        hipLaunchKernelGGL(k_gemm_tile<T>, dim3(grid), dim3(256), 0, st,
+                           reinterpret_cast<const GemmTask*>(dp.gemm) + L.first, a.Lx, a.U, a.W, a.D, a.ldlt);
+        break;
+    }
+    if (prof && 2 * prof->used + 1 < prof->events.size()) {
+      (void)hipEventRecord(prof->events[2 * prof->used + 1], st);
+      prof->used++;
+    }
+  }
+}
+
+template <typename T>
+void launchLdltCounts(const FactorArgs<T>& a, int32_t n, int32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ldlt_counts<T>, dim3(1), dim3(256), 0, st, a.D, a.pert, a.failCol, n, out);
+}
+
+template void launchFactorPlan<double>(const DeviceFactorPlan&, const FactorArgs<double>&, const double*, hipStream_t,
+                                       ProfileSink*);
+template void launchFactorPlan<float>(const DeviceFactorPlan&, const FactorArgs<float>&, const double*, hipStream_t,
+                                      ProfileSink*);
+template void launchLdltCounts<double>(const FactorArgs<double>&, int32_t, int32_t*, hipStream_t);
+template void launchLdltCounts<float>(const FactorArgs<float>&, int32_t, int32_t*, hipStream_t);
+
+}  // namespace amp

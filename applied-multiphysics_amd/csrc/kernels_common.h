@@ -1,0 +1,59 @@
+// Device helpers shared by the factor and solve kernels (gfx950, wave64, MFMA).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.h"
+
+namespace amp {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = v4d;
+};
+template <>
+struct Vec4<float> {
+  using type = v4f;
+};
+
+// 16x16x4 matrix-core step D = A B + C. Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; it receives
+// D[4 (l / 16) + r][l % 16] in component r.
+__device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline v4f mfma16x16x4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Kernel-side view of the factor storage and of the plan's per-supernode arrays.
+template <typename T>
+struct FactorArgs {
+  T* Lx;
+  T* U;
+  T* W;
+  T* D;
+  uint8_t* pert;
+  int32_t* failCol;
+  const int32_t* superStart;
+  const int32_t* Sp;
+  const int64_t* Lxp;
+  const int64_t* Uoff;
+  const int64_t* Woff;
+  const int32_t* childHead;
+  const int32_t* childNext;
+  const int32_t* rel;
+  const int32_t* Si;
+  const int32_t* lowerP;
+  const int32_t* lowerSrc;
+  const int32_t* amap;
+  int32_t ldlt;
+  double tau;
+};
+
+}  // namespace amp

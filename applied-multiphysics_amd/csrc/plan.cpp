@@ -1,0 +1,336 @@
+// Host construction of the factor and solve plans (see plan.h).
+#include "plan.h"
+
+#include <algorithm>
+
+namespace amp {
+
+namespace {
+
+inline int32_t frontRows(const Symbolic& S, int32_t s) { return S.Sp[s + 1] - S.Sp[s]; }
+inline int32_t frontCols(const Symbolic& S, int32_t s) { return S.superStart[s + 1] - S.superStart[s]; }
+
+}  // namespace
+
+FactorPlan buildFactorPlan(const Symbolic& S) {
+  FactorPlan P;
+  const int32_t n = S.n, ns = S.nsuper;
+  P.n = n;
+  P.nsuper = ns;
+  P.nnzLStored = S.nnzLStored;
+  P.nnzA = S.nnzA;
+  P.nnzPattern = S.nnzPattern;
+  P.flops = S.flops;
+  P.level.assign(ns, 0);
+  P.isBig.assign(ns, 0);
+  P.Uoff.assign(ns, -1);
+  P.Woff.assign(ns, -1);
+  if (ns == 0) return P;
+
+  // levels: children carry smaller indices than their parent (postorder)
+  for (int32_t s = 0; s < ns; s++) {
+    const int32_t p = S.superParent[s];
+    if (p >= 0) P.level[p] = std::max(P.level[p], P.level[s] + 1);
+  }
+  P.nlevels = 1 + *std::max_element(P.level.begin(), P.level.end());
+
+  int64_t uTotal = 0, wTotal = 0;
+  for (int32_t s = 0; s < ns; s++) {
+    const int32_t m = frontRows(S, s), nc = frontCols(S, s), mu = m - nc;
+    P.isBig[s] = m > kSmallMax;
+    if (mu > 0) {
+      P.Uoff[s] = uTotal;
+      uTotal += int64_t(mu) * mu;
+    }
+    if (P.isBig[s]) {
+      P.Woff[s] = wTotal;
+      wTotal += int64_t((nc + kTileEdge - 1) / kTileEdge) * kTileEdge * kTileEdge;
+    }
+  }
+  P.uEntries = uTotal;
+  P.wEntries = wTotal;
+
+  // relative maps: position of each child below row in the parent's row list
+  P.rel.assign(S.Si.size(), -1);
+  {
+    std::vector<int32_t> pos(n, -1);
+    for (int32_t p = 0; p < ns; p++) {
+      if (S.childHead[p] == -1) continue;
+      for (int32_t i = S.Sp[p]; i < S.Sp[p + 1]; i++) pos[S.Si[i]] = i - S.Sp[p];
+      for (int32_t c = S.childHead[p]; c != -1; c = S.childNext[c])
+        for (int32_t r = S.Sp[c] + frontCols(S, c); r < S.Sp[c + 1]; r++) P.rel[r] = pos[S.Si[r]];
+    }
+  }
+  // assembly maps of A's lower entries
+  P.amap.assign(S.nnzA, 0);
+  P.adst.assign(S.nnzA, -1);
+  {
+    std::vector<int32_t> pos(n, -1);
+    for (int32_t s = 0; s < ns; s++) {
+      const int32_t f = S.superStart[s], nc = frontCols(S, s), m = frontRows(S, s);
+      for (int32_t i = S.Sp[s]; i < S.Sp[s + 1]; i++) pos[S.Si[i]] = i - S.Sp[s];
+      for (int32_t c = 0; c < nc; c++)
+        for (int32_t q = S.lowerP[f + c]; q < S.lowerP[f + c + 1]; q++) {
+          const int32_t off = pos[S.lowerI[q]] + c * m;
+          P.amap[q] = off;
+          if (P.isBig[s]) P.adst[q] = S.Lxp[s] + off;
+        }
+    }
+  }
+
+  auto push = [&P](LaunchKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0,
+                   int32_t threads = 0) {
+    if (count <= 0) return;
+    P.launches.push_back(Launch{kind, level, first, count, maxM, threads});
+  };
+
+  // zero the L blocks of tiled fronts, then scatter A into them
+  for (int32_t s = 0; s < ns; s++) {
+    if (!P.isBig[s]) continue;
+    const int64_t len = S.Lxp[s + 1] - S.Lxp[s];
+    for (int64_t o = 0; o < len; o += kZeroChunk) P.zeros.push_back(ZeroTask{S.Lxp[s] + o, std::min<int64_t>(kZeroChunk, len - o)});
+  }
+  push(kZero, -1, 0, int64_t(P.zeros.size()));
+  push(kAssemble, -1, 0, S.nnzA);
+
+  std::vector<std::vector<int32_t>> byLevel(P.nlevels);
+  for (int32_t s = 0; s < ns; s++) byLevel[P.level[s]].push_back(s);
+
+  for (int32_t L = 0; L < P.nlevels; L++) {
+    const std::vector<int32_t>& fronts = byLevel[L];
+    // fronts factored in LDS, two size classes (workgroup of 64 threads up to 32 rows, else 256)
+    for (int cls = 0; cls < 2; cls++) {
+      const int64_t first = int64_t(P.smallFronts.size());
+      int32_t maxM = 0;
+      for (int32_t s : fronts) {
+        const int32_t m = frontRows(S, s);
+        if (P.isBig[s] || (cls == 0) != (m <= 32)) continue;
+        P.smallFronts.push_back(s);
+        maxM = std::max(maxM, m);
+      }
+      push(kSmall, L, first, int64_t(P.smallFronts.size()) - first, maxM, cls == 0 ? 64 : 256);
+    }
+    std::vector<int32_t> big;
+    for (int32_t s : fronts)
+      if (P.isBig[s]) big.push_back(s);
+    if (big.empty()) continue;
+
+    // children's contributions to the factor columns
+    {
+      const int64_t first = int64_t(P.ea.size());
+      for (int32_t s : big) {
+        if (S.childHead[s] == -1) continue;
+        const int32_t nc = frontCols(S, s);
+        for (int32_t j = 0; j < nc; j += kEaWidth) P.ea.push_back(EaTask{s, j, std::min(nc, j + kEaWidth), 0});
+      }
+      push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
+    }
+    // panel factorization in 64-column steps, all tiled fronts of the level in lockstep
+    int32_t maxNc = 0;
+    for (int32_t s : big) maxNc = std::max(maxNc, frontCols(S, s));
+    for (int32_t k0 = 0; k0 < maxNc; k0 += kTileEdge) {
+      const int64_t dFirst = int64_t(P.diag.size());
+      for (int32_t s : big) {
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+        if (nc <= k0) continue;
+        const int32_t nb = std::min(kTileEdge, nc - k0);
+        P.diag.push_back(DiagTask{S.Lxp[s] + k0 + int64_t(k0) * m, P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge, m,
+                                  nb, S.superStart[s] + k0, 0});
+      }
+      push(kDiag, L, dFirst, int64_t(P.diag.size()) - dFirst);
+      // L21 rows of this panel: A21 L11^-T (Cholesky) or A21 L11^-T D^-1 (LDL^T), in place
+      const int64_t tFirst = int64_t(P.gemm.size());
+      for (int32_t s : big) {
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
+        if (nc <= k0) continue;
+        const int32_t nb = std::min(kTileEdge, nc - k0);
+        const int64_t base = S.Lxp[s];
+        const int64_t woff = P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge;
+        for (int32_t r0 = k0 + nb; r0 < m; r0 += kTileEdge) {
+          GemmTask t{};
+          t.a_off = base + r0 + int64_t(k0) * m;
+          t.b_off = woff;
+          t.c_off = t.a_off;
+          t.lda = m;
+          t.ldb = kTileEdge;
+          t.ldc = m;
+          t.rows = std::min(kTileEdge, m - r0);
+          t.cols = nb;
+          t.k = nb;
+          t.d_off = f + k0;
+          t.a_buf = kBufLx;
+          t.b_buf = kBufW;
+          t.c_buf = kBufLx;
+          t.mode = 2;
+          P.gemm.push_back(t);
+        }
+      }
+      push(kGemm, L, tFirst, int64_t(P.gemm.size()) - tFirst);
+      // trailing update of the remaining panel columns of the L block
+      const int64_t uFirst = int64_t(P.gemm.size());
+      for (int32_t s : big) {
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
+        if (nc <= k0) continue;
+        const int32_t nb = std::min(kTileEdge, nc - k0);
+        const int64_t base = S.Lxp[s];
+        for (int32_t j0 = k0 + nb; j0 < nc; j0 += kTileEdge)
+          for (int32_t i0 = j0; i0 < m; i0 += kTileEdge) {
+            GemmTask t{};
+            t.a_off = base + i0 + int64_t(k0) * m;
+            t.b_off = base + j0 + int64_t(k0) * m;
+            t.c_off = base + i0 + int64_t(j0) * m;
+            t.lda = t.ldb = t.ldc = m;
+            t.rows = std::min(kTileEdge, m - i0);
+            t.cols = std::min(kTileEdge, nc - j0);
+            t.k = nb;
+            t.d_off = f + k0;
+            t.a_buf = t.b_buf = t.c_buf = kBufLx;
+            t.mode = 0;
+            P.gemm.push_back(t);
+          }
+      }
+      push(kGemm, L, uFirst, int64_t(P.gemm.size()) - uFirst);
+    }
+    // Schur complement U = -L21 D L21^T into the update matrix (write only)
+    {
+      const int64_t first = int64_t(P.gemm.size());
+      for (int32_t s : big) {
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s), mu = m - nc, f = S.superStart[s];
+        if (mu == 0) continue;
+        const int64_t base = S.Lxp[s];
+        for (int32_t j0 = 0; j0 < mu; j0 += kTileEdge)
+          for (int32_t i0 = j0; i0 < mu; i0 += kTileEdge) {
+            GemmTask t{};
+            t.a_off = base + nc + i0;
+            t.b_off = base + nc + j0;
+            t.c_off = P.Uoff[s] + i0 + int64_t(j0) * mu;
+            t.lda = t.ldb = m;
+            t.ldc = mu;
+            t.rows = std::min(kTileEdge, mu - i0);
+            t.cols = std::min(kTileEdge, mu - j0);
+            t.k = nc;
+            t.d_off = f;
+            t.a_buf = t.b_buf = kBufLx;
+            t.c_buf = kBufU;
+            t.mode = 1;
+            P.gemm.push_back(t);
+          }
+      }
+      push(kGemm, L, first, int64_t(P.gemm.size()) - first);
+    }
+    // children's contributions to the update matrix
+    {
+      const int64_t first = int64_t(P.ea.size());
+      for (int32_t s : big) {
+        if (S.childHead[s] == -1) continue;
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+        for (int32_t j = nc; j < m; j += kEaWidth) P.ea.push_back(EaTask{s, j, std::min(m, j + kEaWidth), 0});
+      }
+      push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
+    }
+  }
+  for (const GemmTask& t : P.gemm) {
+    if (t.mode == 2) {
+      P.gemmFlops += double(t.rows) * t.cols * t.k;  // triangular solve: rows x nb^2
+      continue;
+    }
+    // symmetric outputs: a tile whose corner sits on the diagonal only needs its lower part
+    const bool onDiag = (t.mode == 1 && t.a_off == t.b_off) || (t.mode == 0 && t.a_off == t.b_off);
+    double entries = double(t.rows) * t.cols;
+    if (onDiag) {
+      entries = 0;
+      for (int32_t j = 0; j < t.cols; j++) entries += std::max(0, t.rows - j);
+    }
+    P.gemmFlops += 2.0 * entries * t.k;
+  }
+  return P;
+}
+
+SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
+  SolvePlan P;
+  const int32_t ns = S.nsuper;
+  P.fvEntries = ns ? S.Sp[ns] : 0;
+  if (ns == 0) return P;
+  std::vector<std::vector<int32_t>> byLevel(F.nlevels);
+  for (int32_t s = 0; s < ns; s++) byLevel[F.level[s]].push_back(s);
+  auto push = [&P](SolveKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0) {
+    if (count <= 0) return;
+    P.launches.push_back(SolveLaunch{kind, level, first, count, maxM});
+  };
+  // forward, leaves first
+  for (int32_t L = 0; L < F.nlevels; L++) {
+    int64_t first = int64_t(P.fronts.size());
+    int32_t maxM = 0;
+    for (int32_t s : byLevel[L])
+      if (!F.isBig[s]) {
+        P.fronts.push_back(s);
+        maxM = std::max(maxM, frontRows(S, s));
+      }
+    push(kSolveSmallFwd, L, first, int64_t(P.fronts.size()) - first, maxM);
+    std::vector<int32_t> big;
+    for (int32_t s : byLevel[L])
+      if (F.isBig[s]) big.push_back(s);
+    if (big.empty()) continue;
+    first = int64_t(P.fronts.size());
+    P.fronts.insert(P.fronts.end(), big.begin(), big.end());
+    push(kSolveGather, L, first, int64_t(big.size()));
+    int32_t maxBlk = 0;
+    for (int32_t s : big) maxBlk = std::max(maxBlk, (frontCols(S, s) + kTileEdge - 1) / kTileEdge);
+    for (int32_t b = 0; b < maxBlk; b++) {
+      const int64_t bf = int64_t(P.blocks.size());
+      for (int32_t s : big) {
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+        if (b * kTileEdge >= nc) continue;
+        const int32_t rs = std::min(nc, (b + 1) * kTileEdge);  // rows below the block
+        if (rs >= m) {
+          P.blocks.push_back(BlockTask{s, b, m, m, 1, 0});  // nothing below: still solves the block
+          continue;
+        }
+        for (int32_t r = rs; r < m; r += kSolveRows) P.blocks.push_back(BlockTask{s, b, r, std::min(m, r + kSolveRows), r == rs, 0});
+      }
+      push(kSolveFwdBlock, L, bf, int64_t(P.blocks.size()) - bf);
+    }
+  }
+  // backward, root first
+  for (int32_t L = F.nlevels - 1; L >= 0; L--) {
+    std::vector<int32_t> big;
+    int64_t first = int64_t(P.fronts.size());
+    int32_t maxM = 0;
+    for (int32_t s : byLevel[L]) {
+      if (F.isBig[s]) {
+        big.push_back(s);
+        continue;
+      }
+      P.fronts.push_back(s);
+      maxM = std::max(maxM, frontRows(S, s));
+    }
+    push(kSolveSmallBwd, L, first, int64_t(P.fronts.size()) - first, maxM);
+    if (big.empty()) continue;
+    // z_own = y_own - L21^T y[below], columns in chunks
+    int64_t bf = int64_t(P.blocks.size());
+    for (int32_t s : big) {
+      const int32_t nc = frontCols(S, s);
+      for (int32_t c = 0; c < nc; c += kSolveCols) P.blocks.push_back(BlockTask{s, -1, c, std::min(nc, c + kSolveCols), 0, 0});
+    }
+    push(kSolveBwdInit, L, bf, int64_t(P.blocks.size()) - bf);
+    int32_t maxBlk = 0;
+    for (int32_t s : big) maxBlk = std::max(maxBlk, (frontCols(S, s) + kTileEdge - 1) / kTileEdge);
+    for (int32_t b = maxBlk - 1; b >= 0; b--) {
+      bf = int64_t(P.blocks.size());
+      for (int32_t s : big) {
+        const int32_t nc = frontCols(S, s);
+        if (b * kTileEdge >= nc) continue;
+        const int32_t c0 = b * kTileEdge;  // columns [0, c0) are updated by this block
+        if (c0 == 0) {
+          P.blocks.push_back(BlockTask{s, b, 0, 0, 1, 0});
+          continue;
+        }
+        for (int32_t c = 0; c < c0; c += kSolveCols) P.blocks.push_back(BlockTask{s, b, c, std::min(c0, c + kSolveCols), c == 0, 0});
+      }
+      push(kSolveBwdBlock, L, bf, int64_t(P.blocks.size()) - bf);
+    }
+  }
+  return P;
+}
+
+}  // namespace amp

@@ -1,0 +1,130 @@
+// Execution plan of the device multifrontal factorization and of the supernodal solves.
+//
+// Built once per symbolic structure on the host. It fixes the HBM layout (L blocks at the reference's Lxp offsets,
+// one update matrix per supernode, the inverted diagonal blocks of tiled supernodes) and a flat sequence of launches,
+// each a list of independent tasks. Levels of the supernodal elimination tree (level = height above the leaves) run
+// in order, so every child is complete before its parent starts. Task offsets are relative to buffer bases, so the
+// same plan is replayed by any executor.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "symbolic.h"
+
+namespace amp {
+
+constexpr int kTileEdge = 64;   // panel width / update-tile edge of tiled fronts; block size of their solves
+constexpr int kEaWidth = 16;    // parent columns per extend-add work item
+constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
+constexpr int kZeroChunk = 1 << 18;
+constexpr int kSolveRows = 256;  // rows per work item of the blocked forward solve
+constexpr int kSolveCols = 64;   // columns per work item of the blocked backward solve
+
+enum Buf : int8_t { kBufLx = 0, kBufU = 1, kBufW = 2 };
+
+// One output tile (rows x cols <= 64 x 64) of
+//   mode 0: C = C - A diag(d) B^T      mode 1: C = -A diag(d) B^T      mode 2: C = A B^T diag(d)^-1
+// column-major operands; d indexes the depth (modes 0, 1) or the output column (mode 2). Only LDL^T uses d.
+struct GemmTask {
+  int64_t a_off, b_off, c_off;
+  int32_t lda, ldb, ldc;
+  int32_t rows, cols, k;
+  int32_t d_off;
+  int8_t a_buf, b_buf, c_buf, mode;
+};
+
+// Factor the nb x nb diagonal block at Lx[f_off] (leading dimension ld) in place and write the inverse of its lower
+// factor (unit lower for LDL^T) to W[w_off] (leading dimension kTileEdge). col0: global index of its first column.
+struct DiagTask {
+  int64_t f_off, w_off;
+  int32_t ld, nb, col0, pad;
+};
+
+// Add every child's update matrix into columns [j0, j1) (front coordinates) of front s.
+struct EaTask {
+  int32_t s, j0, j1, pad;
+};
+
+struct ZeroTask {
+  int64_t off, len;  // Lx entries
+};
+
+enum LaunchKind : int32_t {
+  kZero = 0,
+  kAssemble = 1,
+  kSmall = 2,
+  kExtendAdd = 3,
+  kDiag = 4,
+  kGemm = 5,
+};
+
+struct Launch {
+  LaunchKind kind;
+  int32_t level;
+  int64_t first, count;  // range in the task array of this kind (kSmall: in smallFronts; kAssemble: lower entries)
+  int32_t maxM;          // kSmall: largest front of the launch (LDS sizing)
+  int32_t threads;       // kSmall: workgroup size
+};
+
+struct FactorPlan {
+  int32_t n = 0, nsuper = 0, nlevels = 0;
+  int64_t nnzLStored = 0, uEntries = 0, wEntries = 0, nnzA = 0, nnzPattern = 0;
+  double flops = 0;
+  double gemmFlops = 0;  // algorithmic flops of the gemm tasks: lower-triangle outputs, triangular solves as such
+  // per supernode
+  std::vector<int32_t> level, isBig;
+  std::vector<int64_t> Uoff;  // offset of the mu x mu update matrix (leading dimension mu), -1 when mu == 0
+  std::vector<int64_t> Woff;  // tiled supernodes: offset of their ceil(nc/64) inverted 64 x 64 diagonal blocks
+  std::vector<int32_t> rel;   // aligned with Si: position of each below row of a child inside its parent's front
+  // per lower entry of A (aligned with lowerP / lowerI / lowerSrc)
+  std::vector<int32_t> amap;  // offset inside its supernode's m x nc block (row + c m)
+  std::vector<int64_t> adst;  // tiled supernodes: absolute Lx index; -1 for supernodes factored in LDS
+  // task arrays
+  std::vector<ZeroTask> zeros;
+  std::vector<int32_t> smallFronts;
+  std::vector<EaTask> ea;
+  std::vector<DiagTask> diag;
+  std::vector<GemmTask> gemm;
+  std::vector<Launch> launches;
+};
+
+FactorPlan buildFactorPlan(const Symbolic& S);
+
+// ---- solves ------------------------------------------------------------------------------------------------------
+// Every supernode s owns a front vector v_s of length m (at Sp[s] in a buffer of Sp[nsuper] entries per right-hand
+// side). Forward: v_s = [y_own; 0] + sum of the children's below parts (through rel); x_own = L11^-1 v_own is written
+// to y; v_below -= L21 x_own and stays in v_s for the parent. Backward: z = y_own - L21^T y[below];
+// x_own = L11^-T z. Supernodes with m <= kSmallMax are solved whole by one workgroup; tiled supernodes go in 64-column
+// blocks through their inverted diagonal blocks, one launch per block step over all tiled supernodes of the level.
+struct BlockTask {
+  int32_t s, blk;        // supernode, 64-column block index (-1: the backward initialisation)
+  int32_t r0, r1;        // forward: rows [r0, r1) of the front updated; backward: columns [r0, r1) updated
+  int32_t writer, pad;   // 1: this item also stores the block's solution into y
+};
+
+enum SolveKind : int32_t {
+  kSolveSmallFwd = 0,
+  kSolveGather = 1,
+  kSolveFwdBlock = 2,
+  kSolveSmallBwd = 3,
+  kSolveBwdInit = 4,
+  kSolveBwdBlock = 5,
+};
+
+struct SolveLaunch {
+  SolveKind kind;
+  int32_t level;
+  int64_t first, count;  // small / gather: range in `fronts`; block kinds: range in `blocks`; init: in `blocks`
+  int32_t maxM;
+};
+
+struct SolvePlan {
+  int64_t fvEntries = 0;  // Sp[nsuper]
+  std::vector<int32_t> fronts;
+  std::vector<BlockTask> blocks;
+  std::vector<SolveLaunch> launches;
+};
+
+SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F);
+
+}  // namespace amp

@@ -1,0 +1,137 @@
+/*
+ * ampsolve: supernodal multifrontal Cholesky / LDL^T numeric factorization and supernodal triangular solves on one
+ * AMD Instinct MI355X (gfx950), behind a plain C ABI.
+ *
+ * This is the boundary the reference's JavaScript host would bind through a thin Node-API addon.
+ * Each entry point names the reference interface it replaces (jwt625/applied-multiphysics, packages/core/src/la/).
+ * All pointers are plain host pointers unless the name ends in _device, in which case they are HIP device pointers
+ * (HBM). The library copies what it needs during a call and never keeps a caller's host pointer.
+ * Every function returns AMP_OK (0) or a negative AMP_ERR_* code; amp_last_error() describes the last failure of the
+ * calling thread. A non-positive Cholesky pivot or a non-finite LDL^T pivot is NOT an error: the factor call returns
+ * AMP_OK with info.ok = 0, as the reference's factor returns false / {ok: false}.
+ * Handles are not reentrant: calls that share an amp_num must not overlap.
+ */
+#ifndef AMPSOLVE_H
+#define AMPSOLVE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMP_OK 0
+#define AMP_ERR_ARG -1         /* invalid argument (null pointer, negative size, unknown mode) */
+#define AMP_ERR_HIP -2         /* a HIP runtime call failed (no device, out of device memory, launch failure) */
+#define AMP_ERR_STATE -3       /* e.g. solve without a successful numeric factorization */
+#define AMP_ERR_PATTERN -4     /* value count or sparsity pattern does not match the symbolic analysis */
+#define AMP_ERR_INTERNAL -5    /* symbolic analysis failed internally */
+
+#define AMP_CHOLESKY 0
+#define AMP_LDLT 1
+
+#define AMP_F64 0 /* factor, update matrices and solves in FP64 (MFMA f64) */
+#define AMP_F32 1 /* factor and update matrices in FP32 (MFMA f32); solves in FP32 */
+
+typedef struct amp_sym amp_sym; /* symbolic analysis + execution plan (host), device copy of the plan */
+typedef struct amp_num amp_num; /* numeric factor resident in HBM */
+
+typedef struct {
+  int32_t n, nsuper, maxFront, nlevels;
+  int64_t nnzL, nnzA, nnzLStored, nnzPattern;
+  double flops, fill;
+  double gemmFlops;   /* algorithmic flops of the matrix-core tile kernel per factorization (lower parts only) */
+  char patternKey[64];
+} amp_sym_info;
+
+/* Read-only views of the analysis arrays (valid while the amp_sym lives); names and meaning as analyze() returns
+ * them (sparse.mjs:873-901). Lxp holds int64 offsets (the reference keeps them in a Float64Array). */
+typedef struct {
+  const int32_t *perm, *pinv, *parent, *colCount;
+  const int32_t *lowerP, *lowerI, *lowerSrc;
+  const int32_t *superStart, *colSuper, *superParent, *childHead, *childNext;
+  const int32_t *Sp, *Si;
+  const int64_t *Lxp;
+} amp_sym_arrays;
+
+/* A symbolic analysis made by the caller's own symbolic phase (the reference's analyze(), sparse.mjs:666-902),
+ * handed over as the arrays it returned. Everything else (superParent, children, Lxp, maxFront) is derived. */
+typedef struct {
+  int32_t n, nsuper;
+  int64_t nnzPattern;           /* number of values of the CSR pattern the analysis was made from */
+  const int32_t *perm;          /* n, new -> old */
+  const int32_t *parent;        /* n, elimination tree in new indices (-1 at roots) */
+  const int32_t *colCount;      /* n, column counts of L (may be NULL: supernodal counts are used) */
+  const int32_t *superStart;    /* nsuper + 1 */
+  const int32_t *Sp;            /* nsuper + 1 */
+  const int32_t *Si;            /* Sp[nsuper] */
+  const int32_t *lowerP;        /* n + 1 */
+  const int32_t *lowerI;        /* lowerP[n] */
+  const int32_t *lowerSrc;      /* lowerP[n] */
+  const char *patternKey;       /* the analysis' patternKey (may be NULL) */
+} amp_sym_desc;
+
+typedef struct {
+  int32_t ok;          /* 1: factor complete; 0: a pivot failed (see failColumn) */
+  int32_t nNeg;        /* LDL^T: negative pivots before the first failure (Sylvester inertia when ok) */
+  int32_t nPerturbed;  /* LDL^T: pivots replaced by +-tau before the first failure */
+  int32_t failColumn;  /* first failing column (new index), -1 when ok */
+  double ms;           /* device time of the factorization (HIP events on the handle's stream) */
+} amp_factor_info;
+
+/* ---- symbolic phase (host) ---------------------------------------------------------------------------------- */
+
+/* amdOrder (sparse.mjs:297-592): approximate minimum degree, perm new -> old. denseFactor: 10 in the reference. */
+int amp_amd_order(int32_t n, const int32_t *rowptr, const int32_t *colidx, double denseFactor, int32_t *perm);
+
+/* analyze (sparse.mjs:666-902): ordering (AMD when perm is NULL), etree, postorder, column counts, relaxed supernodes
+ * (relax != 0), row structures and block offsets, plus the device execution plan. */
+int amp_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const int32_t *perm, int32_t relax,
+                amp_sym **out);
+
+/* Take over an analysis from the caller's symbolic phase (the drop-in path for the reference's own analyze()). */
+int amp_sym_import(const amp_sym_desc *desc, amp_sym **out);
+
+/* patternFingerprint (sparse.mjs:908-925) into out (>= 48 bytes); checkSymbolic compares it (:927-930). */
+int amp_pattern_fingerprint(int32_t n, const int32_t *rowptr, const int32_t *colidx, char *out);
+
+int amp_sym_get_info(const amp_sym *sym, amp_sym_info *info);
+int amp_sym_get_arrays(const amp_sym *sym, amp_sym_arrays *arrays);
+void amp_sym_free(amp_sym *sym);
+
+/* ---- numeric phase (device) --------------------------------------------------------------------------------- */
+
+/* Storage of one numeric factor in HBM (the NumericFactor / SupernodalLDLT storage of sparse.mjs:946-949 and
+ * ldlt.mjs:428-448): mode AMP_CHOLESKY or AMP_LDLT, precision AMP_F64 or AMP_F32. Builds the device plan on first
+ * use of the analysis. */
+int amp_num_create(amp_sym *sym, int32_t mode, int32_t precision, amp_num **out);
+
+/* factorNumeric (sparse.mjs:941-1125) / SupernodalLDLT.factor (ldlt.mjs:450-562): values aligned with the CSR
+ * pattern (nvalues must equal the pattern's entry count, else AMP_ERR_PATTERN). pivotTol (LDL^T only, default 1e-13
+ * in the reference) scales max |a_jj|. */
+int amp_factor(amp_num *num, const double *values, int64_t nvalues, double pivotTol, amp_factor_info *info);
+/* Same with the values already in HBM. */
+int amp_factor_device(amp_num *num, const double *d_values, int64_t nvalues, double pivotTol, amp_factor_info *info);
+
+/* solveNumeric / solveNumericMulti (sparse.mjs:1261-1310), SupernodalLDLT.solve / solveMany (ldlt.mjs:565-643):
+ * nrhs vectors stored one after another, original ordering. AMP_ERR_STATE without a successful factor. */
+int amp_solve(amp_num *num, const double *b, double *x, int32_t nrhs);
+int amp_solve_device(amp_num *num, const double *d_b, double *d_x, int32_t nrhs);
+
+/* The factor in the reference's layout: Lx (nnzLStored, m x nc column-major blocks at Lxp; the upper part of each
+ * diagonal block is unspecified, as in the reference) and D (n, LDL^T pivots; ignored for Cholesky; may be NULL). */
+int amp_download_factor(amp_num *num, double *Lx, double *D);
+
+/* Device time (ms) of the last factor and of the last solve, and the total of each kernel class of the last
+ * factor when profiling is enabled (amp_num_set_profiling). */
+int amp_num_set_profiling(amp_num *num, int32_t enable);
+int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, int32_t *launches);
+
+void amp_num_free(amp_num *num);
+
+/* Message of the last failure on the calling thread ("" when none). */
+const char *amp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

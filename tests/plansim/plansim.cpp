@@ -1,0 +1,355 @@
+// Test-only CPU executor of the device factor / solve plans (applied-multiphysics_amd/csrc/plan.h).
+//
+// It replays the exact launch and task sequence the GPU executes, with plain loops for every task kind, so that the
+// plan (layout, offsets, task coverage, ordering) is checked against the oracle on the CPU. It is never linked into
+// the shipped library.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../applied-multiphysics_amd/csrc/plan.h"
+#include "../../applied-multiphysics_amd/csrc/symbolic.h"
+
+using namespace amp;
+
+namespace {
+
+struct Ctx {
+  const Symbolic* S;
+  const FactorPlan* P;
+  std::vector<double> Lx, U, W, D;
+  std::vector<uint8_t> pert;
+  int ldlt = 0;
+  double tau = 0;
+  int32_t failCol = INT32_MAX;
+};
+
+double* buf(Ctx& c, int8_t b) { return b == kBufLx ? c.Lx.data() : b == kBufU ? c.U.data() : c.W.data(); }
+
+// pivot + scaling + rank-1 update of an nb-column right-looking factorization of the m x m array F (ld)
+bool pivotColumn(Ctx& c, double* F, int32_t ld, int32_t k, int32_t rows, int32_t col, double* dOut) {
+  double d = F[k + int64_t(k) * ld];
+  if (!c.ldlt) {
+    if (!(d > 0)) c.failCol = std::min(c.failCol, col);
+    const double sq = std::sqrt(d);
+    F[k + int64_t(k) * ld] = sq;
+    const double inv = 1 / sq;
+    for (int32_t i = k + 1; i < rows; i++) F[i + int64_t(k) * ld] *= inv;
+    return true;
+  }
+  if (!(std::fabs(d) >= c.tau)) {
+    if (!std::isfinite(d))
+      c.failCol = std::min(c.failCol, col);
+    else {
+      d = d < 0 ? -c.tau : c.tau;
+      c.pert[col] = 1;
+    }
+  }
+  *dOut = d;
+  F[k + int64_t(k) * ld] = 1;
+  const double inv = 1 / d;
+  for (int32_t i = k + 1; i < rows; i++) F[i + int64_t(k) * ld] *= inv;
+  return true;
+}
+
+// right-looking partial factorization of nc columns of a rows x rows lower array
+void partialFactor(Ctx& c, double* F, int32_t ld, int32_t rows, int32_t nc, int32_t col0) {
+  for (int32_t k = 0; k < nc; k++) {
+    double d = 1;
+    pivotColumn(c, F, ld, k, rows, col0 + k, &d);
+    if (c.ldlt) c.D[col0 + k] = d;
+    const double* Fk = F + int64_t(k) * ld;
+    for (int32_t j = k + 1; j < rows; j++) {
+      const double wj = c.ldlt ? Fk[j] * d : Fk[j];
+      double* Fj = F + int64_t(j) * ld;
+      for (int32_t i = j; i < rows; i++) Fj[i] -= Fk[i] * wj;
+    }
+  }
+}
+
+void runSmall(Ctx& c, int32_t s, const double* values) {
+  const Symbolic& S = *c.S;
+  const FactorPlan& P = *c.P;
+  const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s], mu = m - nc;
+  std::vector<double> F(size_t(m) * m, 0.0);
+  for (int32_t q = S.lowerP[f]; q < S.lowerP[f + nc]; q++) F[P.amap[q]] = 0.0 + values[S.lowerSrc[q]];
+  for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+    const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
+    const int32_t* rel = P.rel.data() + S.Sp[ch] + ncc;
+    const double* Uc = c.U.data() + P.Uoff[ch];
+    for (int32_t b = 0; b < muc; b++)
+      for (int32_t a = b; a < muc; a++) F[rel[a] + int64_t(rel[b]) * m] += Uc[a + int64_t(b) * muc];
+  }
+  partialFactor(c, F.data(), m, m, nc, f);
+  std::memcpy(c.Lx.data() + S.Lxp[s], F.data(), size_t(m) * nc * 8);
+  if (mu > 0)
+    for (int32_t b = 0; b < mu; b++)
+      for (int32_t a = 0; a < mu; a++) c.U[P.Uoff[s] + a + int64_t(b) * mu] = F[nc + a + int64_t(nc + b) * m];
+}
+
+void runEa(Ctx& c, const EaTask& t) {
+  const Symbolic& S = *c.S;
+  const FactorPlan& P = *c.P;
+  const int32_t s = t.s, nc = S.superStart[s + 1] - S.superStart[s], m = S.Sp[s + 1] - S.Sp[s], mu = m - nc;
+  for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+    const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
+    const int32_t* rel = P.rel.data() + S.Sp[ch] + ncc;
+    const double* Uc = c.U.data() + P.Uoff[ch];
+    for (int32_t b = 0; b < muc; b++) {
+      const int32_t j = rel[b];
+      if (j < t.j0 || j >= t.j1) continue;
+      for (int32_t a = b; a < muc; a++) {
+        const int32_t i = rel[a];
+        const double v = Uc[a + int64_t(b) * muc];
+        if (j < nc)
+          c.Lx[S.Lxp[s] + i + int64_t(j) * m] += v;
+        else
+          c.U[P.Uoff[s] + (i - nc) + int64_t(j - nc) * mu] += v;
+      }
+    }
+  }
+}
+
+void runDiag(Ctx& c, const DiagTask& t) {
+  double* F = c.Lx.data() + t.f_off;
+  partialFactor(c, F, t.ld, t.nb, t.nb, t.col0);
+  double* W = c.W.data() + t.w_off;
+  const int32_t nb = t.nb;
+  for (int32_t j = 0; j < nb; j++) {  // column j of the inverse of the lower factor
+    for (int32_t i = 0; i < nb; i++) W[i + int64_t(j) * kTileEdge] = 0;
+    for (int32_t i = j; i < nb; i++) {
+      double s = i == j ? 1.0 : 0.0;
+      for (int32_t k = j; k < i; k++) s -= F[i + int64_t(k) * t.ld] * W[k + int64_t(j) * kTileEdge];
+      W[i + int64_t(j) * kTileEdge] = c.ldlt ? s : s / F[i + int64_t(i) * t.ld];
+    }
+  }
+}
+
+void runGemm(Ctx& c, const GemmTask& t) {
+  const double* A = buf(c, t.a_buf) + t.a_off;
+  const double* B = buf(c, t.b_buf) + t.b_off;
+  double* C = buf(c, t.c_buf) + t.c_off;
+  std::vector<double> acc(size_t(t.rows) * t.cols, 0.0);
+  for (int32_t j = 0; j < t.cols; j++)
+    for (int32_t i = 0; i < t.rows; i++) {
+      double s = 0;
+      for (int32_t k = 0; k < t.k; k++) {
+        double b = B[j + int64_t(k) * t.ldb];
+        if (c.ldlt && t.mode != 2) b *= c.D[t.d_off + k];
+        s += A[i + int64_t(k) * t.lda] * b;
+      }
+      if (c.ldlt && t.mode == 2) s /= c.D[t.d_off + j];
+      acc[i + size_t(j) * t.rows] = s;
+    }
+  for (int32_t j = 0; j < t.cols; j++)
+    for (int32_t i = 0; i < t.rows; i++) {
+      double& dst = C[i + int64_t(j) * t.ldc];
+      const double v = acc[i + size_t(j) * t.rows];
+      dst = t.mode == 0 ? dst - v : t.mode == 1 ? -v : v;
+    }
+}
+
+struct Handle {
+  Symbolic S;
+  FactorPlan F;
+  SolvePlan SP;
+  Ctx c;
+};
+
+}  // namespace
+
+extern "C" {
+
+void* ps_create(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax) {
+  Handle* h = new Handle();
+  if (!analyzePattern(n, rowptr, colidx, perm, relax != 0, h->S)) {
+    delete h;
+    return nullptr;
+  }
+  h->F = buildFactorPlan(h->S);
+  h->SP = buildSolvePlan(h->S, h->F);
+  return h;
+}
+
+void ps_free(void* p) { delete static_cast<Handle*>(p); }
+
+// plan statistics: launches, gemm tasks, U entries, W entries
+void ps_stats(void* p, int64_t* out) {
+  Handle* h = static_cast<Handle*>(p);
+  out[0] = int64_t(h->F.launches.size());
+  out[1] = int64_t(h->F.gemm.size());
+  out[2] = h->F.uEntries;
+  out[3] = h->F.wEntries;
+  out[4] = int64_t(h->SP.launches.size());
+  out[5] = h->F.nlevels;
+}
+
+// factor: returns failCol (INT32_MAX when ok); Lx (nnzLStored), D (n)
+int32_t ps_factor(void* p, const double* values, int32_t ldlt, double pivotTol, double* LxOut, double* DOut,
+                  uint8_t* pertOut) {
+  Handle* h = static_cast<Handle*>(p);
+  const Symbolic& S = h->S;
+  const FactorPlan& P = h->F;
+  Ctx& c = h->c;
+  c.S = &S;
+  c.P = &P;
+  c.Lx.assign(size_t(std::max<int64_t>(P.nnzLStored, 1)), std::nan(""));  // poison: every entry must be written
+  c.U.assign(size_t(std::max<int64_t>(P.uEntries, 1)), std::nan(""));
+  c.W.assign(size_t(std::max<int64_t>(P.wEntries, 1)), 0.0);
+  c.D.assign(size_t(std::max(S.n, 1)), 0.0);
+  c.pert.assign(size_t(std::max(S.n, 1)), 0);
+  c.ldlt = ldlt;
+  c.failCol = INT32_MAX;
+  if (ldlt) {
+    double amax = 0;
+    for (int32_t j = 0; j < S.n; j++)
+      for (int32_t q = S.lowerP[j]; q < S.lowerP[j + 1]; q++)
+        if (S.lowerI[q] == j) amax = std::max(amax, std::fabs(values[S.lowerSrc[q]]));
+    c.tau = pivotTol * (amax > 0 ? amax : 1);
+  }
+  for (const Launch& L : P.launches) {
+    for (int64_t t = L.first; t < L.first + L.count; t++) {
+      switch (L.kind) {
+        case kZero:
+          std::fill(c.Lx.begin() + P.zeros[t].off, c.Lx.begin() + P.zeros[t].off + P.zeros[t].len, 0.0);
+          break;
+        case kAssemble:
+          if (P.adst[t] >= 0) c.Lx[P.adst[t]] = 0.0 + values[S.lowerSrc[t]];
+          break;
+        case kSmall:
+          runSmall(c, P.smallFronts[t], values);
+          break;
+        case kExtendAdd:
+          runEa(c, P.ea[t]);
+          break;
+        case kDiag:
+          runDiag(c, P.diag[t]);
+          break;
+        case kGemm:
+          runGemm(c, P.gemm[t]);
+          break;
+      }
+    }
+  }
+  std::memcpy(LxOut, c.Lx.data(), size_t(P.nnzLStored) * 8);
+  if (DOut) std::memcpy(DOut, c.D.data(), size_t(S.n) * 8);
+  if (pertOut) std::memcpy(pertOut, c.pert.data(), size_t(S.n));
+  return c.failCol;
+}
+
+// solve q right-hand sides (one after another, original ordering) with the last factor
+void ps_solve(void* p, const double* b, double* x, int32_t q) {
+  Handle* h = static_cast<Handle*>(p);
+  const Symbolic& S = h->S;
+  const FactorPlan& F = h->F;
+  const SolvePlan& P = h->SP;
+  Ctx& c = h->c;
+  const int32_t n = S.n;
+  const double* Lx = c.Lx.data();
+  std::vector<double> y(size_t(n) * q), fv(size_t(std::max<int64_t>(P.fvEntries, 1)) * q, std::nan(""));
+  for (int32_t k = 0; k < q; k++)
+    for (int32_t i = 0; i < n; i++) y[size_t(k) * n + i] = b[size_t(k) * n + S.perm[i]];
+  bool scaled = false;
+  for (const SolveLaunch& L : P.launches) {
+    if (c.ldlt && !scaled && (L.kind == kSolveSmallBwd || L.kind == kSolveBwdInit)) {
+      for (int32_t k = 0; k < q; k++)
+        for (int32_t j = 0; j < n; j++) y[size_t(k) * n + j] /= c.D[j];
+      scaled = true;
+    }
+    for (int64_t t = L.first; t < L.first + L.count; t++) {
+      for (int32_t k = 0; k < q; k++) {
+        double* yk = y.data() + size_t(k) * n;
+        double* fk = fv.data() + size_t(k) * P.fvEntries;
+        if (L.kind == kSolveSmallFwd || L.kind == kSolveGather) {
+          const int32_t s = P.fronts[t];
+          const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
+          std::vector<double> v(m);
+          for (int32_t i = 0; i < m; i++) v[i] = i < nc ? yk[f + i] : 0.0;
+          for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+            const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
+            for (int32_t r = 0; r < muc; r++) v[F.rel[S.Sp[ch] + ncc + r]] += fk[S.Sp[ch] + ncc + r];
+          }
+          if (L.kind == kSolveSmallFwd) {
+            const double* Ls = Lx + S.Lxp[s];
+            for (int32_t col = 0; col < nc; col++) {
+              const double xc = c.ldlt ? v[col] : v[col] / Ls[col + int64_t(col) * m];
+              v[col] = xc;
+              for (int32_t r = col + 1; r < m; r++) v[r] -= Ls[r + int64_t(col) * m] * xc;
+            }
+            for (int32_t i = 0; i < nc; i++) yk[f + i] = v[i];
+          }
+          for (int32_t i = 0; i < m; i++) fk[S.Sp[s] + i] = v[i];
+        } else if (L.kind == kSolveFwdBlock) {
+          const BlockTask& bt = P.blocks[t];
+          const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
+          const int32_t c0 = bt.blk * kTileEdge, nb = std::min(kTileEdge, nc - c0);
+          const double* W = c.W.data() + F.Woff[s] + int64_t(bt.blk) * kTileEdge * kTileEdge;
+          const double* Ls = Lx + S.Lxp[s];
+          double xb[kTileEdge];
+          for (int32_t j = 0; j < nb; j++) {
+            double sum = 0;
+            for (int32_t kk = 0; kk <= j; kk++) sum += W[j + int64_t(kk) * kTileEdge] * fk[S.Sp[s] + c0 + kk];
+            xb[j] = sum;
+          }
+          for (int32_t r = bt.r0; r < bt.r1; r++) {
+            double sum = 0;
+            for (int32_t j = 0; j < nb; j++) sum += Ls[r + int64_t(c0 + j) * m] * xb[j];
+            fk[S.Sp[s] + r] -= sum;
+          }
+          if (bt.writer)
+            for (int32_t j = 0; j < nb; j++) yk[f + c0 + j] = xb[j];
+        } else if (L.kind == kSolveSmallBwd) {
+          const int32_t s = P.fronts[t];
+          const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
+          const double* Ls = Lx + S.Lxp[s];
+          std::vector<double> z(nc);
+          for (int32_t i = 0; i < nc; i++) {
+            double sum = yk[f + i];
+            for (int32_t r = nc; r < m; r++) sum -= Ls[r + int64_t(i) * m] * yk[S.Si[S.Sp[s] + r]];
+            z[i] = sum;
+          }
+          for (int32_t col = nc - 1; col >= 0; col--) {
+            const double xc = c.ldlt ? z[col] : z[col] / Ls[col + int64_t(col) * m];
+            z[col] = xc;
+            for (int32_t i = 0; i < col; i++) z[i] -= Ls[col + int64_t(i) * m] * xc;
+          }
+          for (int32_t i = 0; i < nc; i++) yk[f + i] = z[i];
+        } else if (L.kind == kSolveBwdInit) {
+          const BlockTask& bt = P.blocks[t];
+          const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
+          const double* Ls = Lx + S.Lxp[s];
+          for (int32_t i = bt.r0; i < bt.r1; i++) {
+            double sum = yk[f + i];
+            for (int32_t r = nc; r < m; r++) sum -= Ls[r + int64_t(i) * m] * yk[S.Si[S.Sp[s] + r]];
+            fk[S.Sp[s] + i] = sum;
+          }
+        } else if (L.kind == kSolveBwdBlock) {
+          const BlockTask& bt = P.blocks[t];
+          const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
+          const int32_t c0 = bt.blk * kTileEdge, nb = std::min(kTileEdge, nc - c0);
+          const double* W = c.W.data() + F.Woff[s] + int64_t(bt.blk) * kTileEdge * kTileEdge;
+          const double* Ls = Lx + S.Lxp[s];
+          double xb[kTileEdge];
+          for (int32_t j = 0; j < nb; j++) {
+            double sum = 0;
+            for (int32_t kk = j; kk < nb; kk++) sum += W[kk + int64_t(j) * kTileEdge] * fk[S.Sp[s] + c0 + kk];
+            xb[j] = sum;
+          }
+          for (int32_t i = bt.r0; i < bt.r1; i++) {
+            double sum = 0;
+            for (int32_t j = 0; j < nb; j++) sum += Ls[(c0 + j) + int64_t(i) * m] * xb[j];
+            fk[S.Sp[s] + i] -= sum;
+          }
+          if (bt.writer)
+            for (int32_t j = 0; j < nb; j++) yk[f + c0 + j] = xb[j];
+        }
+      }
+    }
+  }
+  for (int32_t k = 0; k < q; k++)
+    for (int32_t i = 0; i < n; i++) x[size_t(k) * n + S.perm[i]] = y[size_t(k) * n + i];
+}
+
+}  // extern "C"

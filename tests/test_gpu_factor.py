@@ -1,0 +1,204 @@
+"""GPU parity of the device Cholesky / LDL^T factor and solves against the oracle (the C restatement of the reference
+algorithm in oracle/). Tolerances: the device factor accumulates with fused multiply-adds on the matrix cores, so it
+is not bitwise equal to the reference's unfused arithmetic; the solution must agree with the oracle's to 1e-12
+relative (max norm) and reach the same residual level (< 1e-12) on these well-conditioned cases."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from ampsolve import workloads as W
+import ampsolve as amp
+from fixtures import chain, grid27, hex_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def lower_mask(S):
+    mask = np.zeros(S.nnzLStored, dtype=bool)
+    for s in range(S.nsuper):
+        nc = S.superStart[s + 1] - S.superStart[s]
+        m = S.Sp[s + 1] - S.Sp[s]
+        blk = np.tril(np.ones((m, nc), dtype=bool))
+        mask[S.Lxp[s]:S.Lxp[s + 1]] = blk.T.ravel()
+    return mask
+
+
+def relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def residual(pat, v, x, b):
+    A = sp.csr_matrix((v, pat["colidx"], pat["rowptr"]), shape=(pat["n"], pat["n"]))
+    return np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+
+
+CASES = {
+    "poisson2d_64": lambda: W.poisson2d(64)[:2],
+    "grid27_11": lambda: grid27(11, 11, 11)[:2],
+    "grid27_23x9x5": lambda: grid27(23, 9, 5)[:2],
+    "grid27_10_zeros": lambda: grid27(10, 10, 10, zeroEvery=3)[:2],
+    "heat3d_14": lambda: W.heat3d(14)[:2],
+    "hex_matrix": lambda: hex_matrix(),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_cholesky_matches_oracle(name):
+    print(f"case {name}", flush=True)
+    pat, v = CASES[name]()
+    n = pat["n"]
+    b = 2 * W.lcg_uniform(7, n) - 1
+    sym = amp.analyze(pat)
+    So = O.Sym(pat)
+    assert np.array_equal(sym.perm, So.perm) and np.array_equal(sym.Si, So.Si)  # identical structure
+    ok_o, Lo = O.chol_factor(So, v)
+    xo = O.chol_solve(So, Lo, b)
+    num = amp.factorNumeric(sym, v)
+    assert ok_o and num.ok
+    Lg, _ = num.download()
+    m = lower_mask(So)
+    assert relmax(Lg[m], Lo[m]) < 1e-11
+    xg = amp.solveNumeric(num, b)
+    assert relmax(xg, xo) < 1e-12
+    assert residual(pat, v, xg, b) < 1e-12
+
+
+def test_tridiagonal_closed_form():
+    """tridiag(-1, 2, -1) in natural order: L_kk = sqrt((k+2)/(k+1)), L_k+1,k = -sqrt((k+1)/(k+2)) (0-based)."""
+    n = 300
+    pat, v = chain(n)
+    sym = amp.analyze(pat, {"perm": np.arange(n, dtype=np.int32), "relax": False})
+    num = amp.factorNumeric(sym, v)
+    assert num.ok
+    Lx, _ = num.download()
+    k = np.arange(n, dtype=np.float64)
+    diag = np.empty(n)
+    sub = np.empty(n - 1)
+    for s in range(sym.nsuper):
+        f, nc = sym.superStart[s], sym.superStart[s + 1] - sym.superStart[s]
+        m = sym.Sp[s + 1] - sym.Sp[s]
+        blk = Lx[sym.Lxp[s]:sym.Lxp[s + 1]].reshape(nc, m).T
+        for c in range(nc):
+            j = f + c
+            diag[j] = blk[c, c]
+            if j + 1 < n:
+                sub[j] = blk[c + 1, c]
+    assert relmax(diag, np.sqrt((k + 2) / (k + 1))) < 1e-14
+    assert relmax(sub, -np.sqrt((k[:-1] + 1) / (k[:-1] + 2))) < 1e-14
+
+
+@pytest.mark.parametrize("dims,opt", [((11, 11, 11), {}), ((23, 9, 5), {}), ((10, 10, 10), {"zeroEvery": 3})])
+@pytest.mark.parametrize("pivot_tol", [1e-13, 0.05])
+def test_ldlt_indefinite_matches_oracle(dims, opt, pivot_tol):
+    print(f"ldlt {dims} {opt} {pivot_tol}", flush=True)
+    pat, v, n = grid27(*dims, shift=26, **opt)
+    b = np.sin(np.arange(n, dtype=np.float64))
+    sym = amp.analyze(pat)
+    So = O.Sym(pat)
+    ok_o, Lo, Do, nneg_o, npert_o = O.ldlt_factor(So, v, pivot_tol)
+    f = amp.SupernodalLDLT(sym)
+    r = f.factor(v, pivotTol=pivot_tol)
+    assert ok_o and r["ok"]
+    assert r["nNeg"] == nneg_o > 0
+    assert r["nPerturbed"] == npert_o
+    if pivot_tol > 1e-3:
+        assert npert_o > 0
+    Lg, Dg = f.num.download()
+    assert relmax(Dg, Do) < 1e-10
+    xo = O.ldlt_solve(So, Lo, Do, b)
+    xg = f.solve(b)
+    assert relmax(xg, xo) < (1e-12 if pivot_tol < 1e-3 else 1e-9)
+
+
+def test_cholesky_reports_indefinite_and_solve_refuses():
+    pat, v, n = grid27(11, 11, 11, shift=26)
+    num = amp.factorNumeric(amp.analyze(pat), v)
+    assert num.ok is False
+    with pytest.raises(RuntimeError, match="no successful numeric factorization"):
+        amp.solveNumeric(num, np.zeros(n))
+    ch = amp.SparseCholesky(pat)
+    assert ch.factor(v) is False
+    with pytest.raises(RuntimeError, match="without a successful factorization"):
+        ch.solve(np.zeros(n))
+
+
+def test_failure_in_root_front():
+    """Non-positive Cholesky pivot and non-finite LDL^T pivot in the last-eliminated column (root front)."""
+    pat, v, n = grid27(11, 11, 11)
+    sym = amp.analyze(pat)
+    last = sym.perm[n - 1]
+
+    def with_diag(x):
+        w = v.copy()
+        for p in range(pat["rowptr"][last], pat["rowptr"][last + 1]):
+            if pat["colidx"][p] == last:
+                w[p] = x
+        return w
+
+    assert amp.factorNumeric(sym, with_diag(-1e6)).ok is False
+    So = O.Sym(pat)
+    ok_o, _, _, nneg_o, npert_o = O.ldlt_factor(So, with_diag(np.nan))
+    r = amp.SupernodalLDLT(sym).factor(with_diag(np.nan))
+    assert r["ok"] is False and ok_o is False
+    assert r["nNeg"] == nneg_o and r["nPerturbed"] == npert_o
+
+
+@pytest.mark.parametrize("q", [1, 2, 3, 4, 11])
+def test_multi_rhs(q):
+    pat, v, n = grid27(12, 12, 12, symmetric=True)
+    So = O.Sym(pat)
+    _, Lo = O.chol_factor(So, v)
+    B = np.stack([2 * W.lcg_uniform(100 + k, n) - 1 for k in range(q)])
+    B[0, ::5] = 0.0
+    ch = amp.SparseCholesky(pat)
+    assert ch.factor(v)
+    X = ch.solveMany(list(B))
+    for k in range(q):
+        assert relmax(X[k], O.chol_solve(So, Lo, B[k])) < 1e-12
+    f = amp.SupernodalLDLT(ch.sym)
+    assert f.factor(v)["ok"]
+    Y = f.solveMany(B.reshape(-1), q).reshape(q, n)
+    for k in range(q):
+        assert relmax(Y[k], X[k]) < 1e-11
+
+
+def test_pattern_and_count_checks():
+    pat, v, n = grid27(6, 6, 6)
+    sym = amp.analyze(pat)
+    other = dict(pat, colidx=pat["colidx"].copy())
+    last = other["rowptr"][1] - 1
+    other["colidx"][last] = (other["colidx"][last] + 1) % n
+    with pytest.raises(ValueError, match="different sparsity pattern"):
+        amp.SparseCholesky(other, sym=sym)
+    with pytest.raises(ValueError, match="different sparsity pattern"):
+        amp.factorNumeric(sym, v, None, other)
+    with pytest.raises(ValueError, match="values for a pattern"):
+        amp.factorNumeric(sym, v[1:])
+
+
+def test_empty_pattern():
+    empty = {"n": 0, "rowptr": np.zeros(1, dtype=np.int32), "colidx": np.zeros(0, dtype=np.int32), "nnz": 0}
+    ch = amp.SparseCholesky(empty)
+    assert ch.factor(np.zeros(0))
+    assert len(ch.solve(np.zeros(0))) == 0
+
+
+def test_device_pointers_and_determinism(device):
+    import torch
+
+    pat, v, b = W.heat3d(10)
+    sym = amp.analyze(pat)
+    num = amp.NumericFactor(sym, 0, "hip")
+    dv = torch.from_numpy(v).to(device)
+    db = torch.from_numpy(b).to(device)
+    assert num.factor(dv)
+    L1, _ = num.download()
+    x1 = num.solve(db).cpu().numpy()
+    assert num.factor(dv)
+    L2, _ = num.download()
+    x2 = num.solve(db).cpu().numpy()
+    assert np.array_equal(L1, L2) and np.array_equal(x1, x2)  # run-to-run bitwise (no atomics in the arithmetic)
+    So = O.Sym(pat)
+    _, Lo = O.chol_factor(So, v)
+    assert relmax(x1, O.chol_solve(So, Lo, b)) < 1e-12
