@@ -41,6 +41,18 @@ class SymDesc(C.Structure):
                                                  "lowerI", "lowerSrc")] + [("patternKey", C.c_char_p)]
 
 
+class RefineOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("maxSteps", C.c_int32), ("restart", C.c_int32), ("tol", C.c_double)]
+
+
+class RefineInfo(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("converged", C.c_int32), ("relResidual", C.c_double), ("ms", C.c_double)]
+
+
+AMP_REFINE_IR = 0
+AMP_REFINE_GMRES = 1
+
+
 class FactorInfo(C.Structure):
     _fields_ = [("ok", C.c_int32), ("nNeg", C.c_int32), ("nPerturbed", C.c_int32), ("failColumn", C.c_int32),
                 ("ms", C.c_double)]
@@ -60,6 +72,11 @@ SIGNATURES = {
     "amp_factor_device": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.POINTER(FactorInfo)], C.c_int),
     "amp_solve": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "amp_solve_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_refine": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(RefineOpts), C.POINTER(RefineInfo)],
+                   C.c_int),
+    "amp_refine_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(RefineOpts),
+                           C.POINTER(RefineInfo)], C.c_int),
+    "amp_sym_set_pattern": ([C.c_void_p, i32p, i32p], C.c_int),
     "amp_download_factor": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amp_num_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "amp_num_get_kernel_ms": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),

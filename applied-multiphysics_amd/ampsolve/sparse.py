@@ -178,6 +178,26 @@ class NumericFactor:
         A.check(A.lib().amp_solve(self._h, A.ptr(b) if b.size else None, A.ptr(x) if x.size else None, int(q)))
         return x
 
+    def refine(self, values, b, method="ir", maxSteps=3, restart=40, tol=1e-14, out=None):
+        """Refine the factor's solution of one right-hand side on the device (FP64 residual and update):
+        method 'ir' (solveRefined, ldlt.mjs:253-275) or 'gmres' (gmres-real.mjs:31-104 preconditioned by the factor).
+        values: the unperturbed matrix values on the analysed pattern. Returns (x, info)."""
+        o = A.RefineOpts()
+        o.method = A.AMP_REFINE_IR if method == "ir" else A.AMP_REFINE_GMRES
+        o.maxSteps, o.restart, o.tol = int(maxSteps), int(restart), float(tol)
+        info = A.RefineInfo()
+        if hasattr(b, "data_ptr") and getattr(b, "is_cuda", False):
+            x = b.new_empty(b.shape) if out is None else out
+            A.check(A.lib().amp_refine_device(self._h, A.ptr(values), A.ptr(b), A.ptr(x), C.byref(o), C.byref(info)))
+        else:
+            v = np.ascontiguousarray(values, dtype=np.float64)
+            bb = np.ascontiguousarray(b, dtype=np.float64)
+            x = np.zeros_like(bb) if out is None else out
+            A.check(A.lib().amp_refine(self._h, A.ptr(v) if v.size else None, A.ptr(bb) if bb.size else None,
+                                       A.ptr(x) if x.size else None, C.byref(o), C.byref(info)))
+        return x, {"steps": info.steps, "converged": bool(info.converged), "relResidual": info.relResidual,
+                   "ms": info.ms}
+
     def download(self):
         """(Lx, D) in the reference's layout (D is None for Cholesky)."""
         Lx = np.zeros(max(self.sym.nnzLStored, 1))
@@ -333,7 +353,9 @@ class SupernodalLDLT:
         return self.num.solve(b, out, nrhs=q)
 
     def solveRefined(self, pat, values, b, maxSteps=3, tol=1e-14):
-        """Iterative refinement against the unperturbed values (ldlt.mjs:253-275)."""
-        from .refine import refine_host
+        """Iterative refinement against the unperturbed values (ldlt.mjs:253-275), on the device.
+        Returns {'x', 'relResidual', 'steps'}."""
         self._need()
-        return refine_host(self, pat, values, b, maxSteps, tol)
+        checkSymbolic(self.sym, pat)
+        x, info = self.num.refine(values, b, "ir", maxSteps=maxSteps, tol=tol)
+        return {"x": x, "relResidual": info["relResidual"], "steps": info["steps"]}

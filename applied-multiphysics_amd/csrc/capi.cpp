@@ -1,6 +1,4 @@
 // C ABI of the library (include/ampsolve.h): handles, HBM storage, plan upload and the factor / solve drivers.
-#include "../../include/ampsolve.h"
-
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,83 +10,21 @@
 #include <string>
 #include <vector>
 
-#include "factor_kernels.h"
-#include "plan.h"
-#include "symbolic.h"
+#include "handles.h"
 
+using amp::DevBuf;
+using amp::fail;
+using amp::upload;
+
+namespace amp {
 namespace {
-
 thread_local std::string g_error;
-
+}
 int fail(int code, const std::string& msg) {
   g_error = msg;
   return code;
 }
-
-#define HIP_TRY(call)                                                                                    \
-  do {                                                                                                   \
-    hipError_t e_ = (call);                                                                              \
-    if (e_ != hipSuccess) return fail(AMP_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_));     \
-  } while (0)
-
-// Owning device allocation.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t alloc(size_t b) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = b;
-    return hipMalloc(&p, std::max<size_t>(b, 256));
-  }
-  template <typename T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-};
-
-template <typename T>
-hipError_t upload(DevBuf& d, const std::vector<T>& v) {
-  hipError_t e = d.alloc(v.size() * sizeof(T));
-  if (e != hipSuccess || v.empty()) return e;
-  return hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-}
-
-}  // namespace
-
-struct amp_sym {
-  amp::Symbolic S;
-  amp::FactorPlan F;
-  amp::SolvePlan SP;
-  std::vector<int32_t> diagSrc;  // CSR slot of each diagonal entry (new column order), -1 when absent
-  bool onDevice = false;
-  // device copies of the structure and of the task arrays
-  DevBuf superStart, Sp, Si, Lxp, Uoff, Woff, childHead, childNext, rel, lowerP, lowerSrc, amap, adst, perm, dDiagSrc;
-  DevBuf zeros, smallFronts, ea, diag, gemm, sFronts, sBlocks;
-  amp::DeviceFactorPlan dfp;
-  amp::DeviceSolvePlan dsp;
-};
-
-struct amp_num {
-  amp_sym* sym = nullptr;
-  int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv;
-  bool ok = false;
-  bool profiling = false;
-  amp::ProfileSink prof;
-  std::map<std::string, std::pair<double, int32_t>> kernelMs;
-  ~amp_num() {
-    for (hipEvent_t e : prof.events) (void)hipEventDestroy(e);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
+}  // namespace amp
 
 namespace {
 
@@ -282,9 +218,13 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
 
 }  // namespace
 
+namespace amp {
+int solveOnDevice(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) { return runSolve(num, d_b, d_x, nrhs); }
+}  // namespace amp
+
 extern "C" {
 
-const char* amp_last_error(void) { return g_error.c_str(); }
+const char* amp_last_error(void) { return amp::g_error.c_str(); }
 
 int amp_amd_order(int32_t n, const int32_t* rowptr, const int32_t* colidx, double denseFactor, int32_t* perm) {
   if (n < 0 || (n > 0 && (!rowptr || !colidx || !perm))) return fail(AMP_ERR_ARG, "amp_amd_order: bad arguments");
@@ -306,6 +246,8 @@ int amp_analyze(int32_t n, const int32_t* rowptr, const int32_t* colidx, const i
   std::unique_ptr<amp_sym> h(new amp_sym());
   if (!amp::analyzePattern(n, rowptr, colidx, perm, relax != 0, h->S))
     return fail(AMP_ERR_INTERNAL, "analyze: amdOrder internal error");
+  h->rowptr.assign(rowptr, rowptr + n + 1);
+  h->colidx.assign(colidx, colidx + rowptr[n]);
   finishSym(h.get());
   *out = h.release();
   return AMP_OK;

@@ -1,0 +1,97 @@
+// Internal definitions of the C ABI handles (include/ampsolve.h) shared by the library's host translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ampsolve.h"
+#include "factor_kernels.h"
+#include "plan.h"
+#include "symbolic.h"
+
+namespace amp {
+
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(call)                                                                                    \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return ::amp::fail(AMP_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    return hipMalloc(&p, std::max<size_t>(b, 256));
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <typename T>
+hipError_t upload(DevBuf& d, const std::vector<T>& v) {
+  hipError_t e = d.alloc(v.size() * sizeof(T));
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int solveOnDevice(amp_num* num, const double* d_b, double* d_x, int32_t nrhs);
+
+}  // namespace amp
+
+struct amp_sym {
+  amp::Symbolic S;
+  amp::FactorPlan F;
+  amp::SolvePlan SP;
+  std::vector<int32_t> diagSrc;  // CSR slot of each diagonal entry (new column order), -1 when absent
+  std::vector<int32_t> rowptr, colidx;  // the analysed CSR pattern (for the refinement residual), empty if unknown
+  amp::DevBuf dRowptr, dColidx;
+  bool onDevice = false;
+  // device copies of the structure and of the task arrays
+  amp::DevBuf superStart, Sp, Si, Lxp, Uoff, Woff, childHead, childNext, rel, lowerP, lowerSrc, amap, adst, perm, dDiagSrc;
+  amp::DevBuf zeros, smallFronts, ea, diag, gemm, sFronts, sBlocks;
+  amp::DeviceFactorPlan dfp;
+  amp::DeviceSolvePlan dsp;
+};
+
+struct amp_num {
+  amp_sym* sym = nullptr;
+  int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv;
+  // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
+  amp::DevBuf r, dvec, w, V, Z, partial, small;
+  bool ok = false;
+  bool profiling = false;
+  amp::ProfileSink prof;
+  std::map<std::string, std::pair<double, int32_t>> kernelMs;
+  ~amp_num() {
+    for (hipEvent_t e : prof.events) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+

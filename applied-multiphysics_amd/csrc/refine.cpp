@@ -1,0 +1,255 @@
+// Device refinement drivers behind amp_refine / amp_refine_device (include/ampsolve.h).
+//
+// Iterative refinement restates solveRefined (packages/core/src/la/ldlt.mjs:253-275): x = solve(b), then up to
+// maxSteps times r = b - A x (CSR row order, FP64), stop when |r| / |b| < tol, x += solve(r).
+// GMRES-IR restates the restarted right-preconditioned GMRES of gmres-real.mjs:31-104 (Givens rotations on the
+// Hessenberg column, true residual at exit) with the factor's solve as the preconditioner and the factor's solution
+// as the initial guess; the Arnoldi step orthogonalizes with two passes of classical Gram-Schmidt (one reduction per
+// pass on the GPU instead of one per basis vector) in place of modified Gram-Schmidt.
+// Every vector lives in HBM in FP64; only the Hessenberg matrix and the scalars of the rotations are on the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "handles.h"
+#include "refine_kernels.h"
+
+using amp::DevBuf;
+using amp::fail;
+
+namespace {
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes < bytes) HIP_TRY(b.alloc(bytes));
+  return AMP_OK;
+}
+
+int patternToDevice(amp_sym* h) {
+  if (h->rowptr.empty()) return fail(AMP_ERR_STATE, "refine: the CSR pattern is unknown (amp_sym_set_pattern)");
+  if (!h->dRowptr.p) {
+    HIP_TRY(amp::upload(h->dRowptr, h->rowptr));
+    HIP_TRY(amp::upload(h->dColidx, h->colidx));
+  }
+  return AMP_OK;
+}
+
+struct Ctx {
+  amp_num* num;
+  int32_t n;
+  hipStream_t st;
+  const int32_t *rp, *ci;
+  const double* val;
+  double* small;  // device scratch for reduction results (kMaxRestart + 2)
+  double* partial;
+
+  // host copy of out[0..nv) = V_j . w
+  int dots(const double* V, int64_t ldv, int32_t nv, const double* w, double* host) {
+    amp::launchMultiDot(V, ldv, nv, w, n, partial, small, st);
+    HIP_TRY(hipMemcpyAsync(host, small, size_t(nv) * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMP_OK;
+  }
+  int norm(const double* w, double* out) {
+    int rc = dots(w, 0, 1, w, out);
+    *out = std::sqrt(*out);
+    return rc;
+  }
+  // r = b - A x, returns |r|
+  int residual(const double* b, const double* x, double* r, double* rn) {
+    amp::launchResidual(rp, ci, val, x, b, r, n, st);
+    return norm(r, rn);
+  }
+};
+
+int runIR(Ctx& c, const double* d_b, double* d_x, double bn, const amp_refine_opts* o, amp_refine_info* info) {
+  amp_num* num = c.num;
+  int rc = amp::solveOnDevice(num, d_b, d_x, 1);
+  if (rc) return rc;
+  double rel = INFINITY;
+  int32_t steps = 0;
+  for (; steps <= o->maxSteps; steps++) {
+    double rn = 0;
+    rc = c.residual(d_b, d_x, num->r.as<double>(), &rn);
+    if (rc) return rc;
+    rel = rn / bn;
+    if (rel < o->tol || steps == o->maxSteps) break;
+    rc = amp::solveOnDevice(num, num->r.as<double>(), num->dvec.as<double>(), 1);
+    if (rc) return rc;
+    amp::launchAxpby(1.0, num->dvec.as<double>(), 1.0, d_x, c.n, c.st);
+  }
+  info->steps = steps;
+  info->relResidual = rel;
+  info->converged = rel < o->tol;
+  return AMP_OK;
+}
+
+int runGmres(Ctx& c, const double* d_b, double* d_x, double bn, const amp_refine_opts* o, amp_refine_info* info) {
+  amp_num* num = c.num;
+  const int32_t n = c.n, m = o->restart;
+  const int64_t ld = n;
+  double* V = num->V.as<double>();
+  double* Z = num->Z.as<double>();
+  double* r = num->r.as<double>();
+  double* w = num->w.as<double>();
+  int rc = amp::solveOnDevice(num, d_b, d_x, 1);  // x0 = the factor's solution
+  if (rc) return rc;
+  std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+  double beta = 0;
+  rc = c.residual(d_b, d_x, r, &beta);
+  if (rc) return rc;
+  double relres = beta / bn;
+  int32_t iters = 0;
+  while (relres > o->tol && iters < o->maxSteps) {
+    amp::launchAxpby(1.0 / beta, r, 0.0, V, n, c.st);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int32_t k = 0;
+    for (; k < m && iters < o->maxSteps; k++) {
+      iters++;
+      double* Vk = V + k * ld;
+      double* Zk = Z + k * ld;
+      rc = amp::solveOnDevice(num, Vk, Zk, 1);
+      if (rc) return rc;
+      amp::launchResidual(c.rp, c.ci, c.val, Zk, nullptr, w, n, c.st);  // w = A z_k
+      // two passes of classical Gram-Schmidt against V_0 .. V_k
+      rc = c.dots(V, ld, k + 1, w, h.data());
+      if (rc) return rc;
+      HIP_TRY(hipMemcpyAsync(c.small, h.data(), size_t(k + 1) * 8, hipMemcpyHostToDevice, c.st));
+      amp::launchMultiAxpy(V, ld, k + 1, c.small, w, n, c.st);
+      rc = c.dots(V, ld, k + 1, w, h2.data());
+      if (rc) return rc;
+      HIP_TRY(hipMemcpyAsync(c.small, h2.data(), size_t(k + 1) * 8, hipMemcpyHostToDevice, c.st));
+      amp::launchMultiAxpy(V, ld, k + 1, c.small, w, n, c.st);
+      for (int32_t j = 0; j <= k; j++) H[size_t(j) * m + k] = h[j] + h2[j];
+      double hn = 0;
+      rc = c.norm(w, &hn);
+      if (rc) return rc;
+      H[size_t(k + 1) * m + k] = hn;
+      if (hn > 0) amp::launchAxpby(1.0 / hn, w, 0.0, V + (k + 1) * ld, n, c.st);
+      for (int32_t j = 0; j < k; j++) {
+        const double a = H[size_t(j) * m + k], b = H[size_t(j + 1) * m + k];
+        H[size_t(j) * m + k] = cs[j] * a + sn[j] * b;
+        H[size_t(j + 1) * m + k] = -sn[j] * a + cs[j] * b;
+      }
+      const double a = H[size_t(k) * m + k], b = H[size_t(k + 1) * m + k];
+      double d = std::hypot(a, b);
+      if (!(d != 0)) d = 1e-300;
+      cs[k] = a / d;
+      sn[k] = b / d;
+      H[size_t(k) * m + k] = d;
+      H[size_t(k + 1) * m + k] = 0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      relres = std::fabs(g[k + 1]) / bn;
+      if (relres <= o->tol || hn == 0) {
+        k++;
+        break;
+      }
+    }
+    for (int32_t i = k - 1; i >= 0; i--) {
+      double s = g[i];
+      for (int32_t j = i + 1; j < k; j++) s -= H[size_t(i) * m + j] * y[j];
+      y[i] = s / H[size_t(i) * m + i];
+    }
+    HIP_TRY(hipMemcpyAsync(c.small, y.data(), size_t(k) * 8, hipMemcpyHostToDevice, c.st));
+    amp::launchCombine(Z, ld, k, c.small, d_x, n, c.st);
+    if (relres <= o->tol || iters >= o->maxSteps) break;
+    rc = c.residual(d_b, d_x, r, &beta);
+    if (rc) return rc;
+    relres = beta / bn;
+  }
+  double tr = 0;
+  rc = c.residual(d_b, d_x, r, &tr);
+  if (rc) return rc;
+  info->steps = iters;
+  info->relResidual = tr / bn;
+  info->converged = info->relResidual <= std::max(o->tol * 10, 1e-14);
+  return AMP_OK;
+}
+
+int refineDevice(amp_num* num, const double* d_values, const double* d_b, double* d_x, const amp_refine_opts* o,
+                 amp_refine_info* info) {
+  amp_sym* h = num->sym;
+  const int32_t n = h->S.n;
+  if (!num->ok) return fail(AMP_ERR_STATE, "refine: no successful numeric factorization");
+  if (o->method != AMP_REFINE_IR && o->method != AMP_REFINE_GMRES) return fail(AMP_ERR_ARG, "refine: unknown method");
+  if (o->method == AMP_REFINE_GMRES && (o->restart < 1 || o->restart > amp::kMaxRestart))
+    return fail(AMP_ERR_ARG, "refine: restart must be in [1, 64]");
+  if (o->maxSteps < 0) return fail(AMP_ERR_ARG, "refine: negative step count");
+  int rc = patternToDevice(h);
+  if (rc) return rc;
+  const size_t vb = size_t(std::max(n, 1)) * 8;
+  if ((rc = ensure(num->r, vb)) || (rc = ensure(num->dvec, vb)) || (rc = ensure(num->w, vb))) return rc;
+  if ((rc = ensure(num->partial, size_t(amp::partialEntries(amp::kMaxRestart + 2)) * 8))) return rc;
+  if ((rc = ensure(num->small, size_t(amp::kMaxRestart + 2) * 8))) return rc;
+  if (o->method == AMP_REFINE_GMRES) {
+    if ((rc = ensure(num->V, vb * (o->restart + 1))) || (rc = ensure(num->Z, vb * o->restart))) return rc;
+  }
+  hipEvent_t e0 = num->ev0, e1 = num->ev1;
+  HIP_TRY(hipEventRecord(e0, num->stream));
+  Ctx c{num, n, num->stream, h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_values, num->small.as<double>(),
+        num->partial.as<double>()};
+  double bn = 0;
+  rc = c.norm(d_b, &bn);
+  if (rc) return rc;
+  if (!(bn != 0) || std::isnan(bn)) bn = 1;  // `Math.sqrt(bn) || 1`
+  rc = o->method == AMP_REFINE_IR ? runIR(c, d_b, d_x, bn, o, info) : runGmres(c, d_b, d_x, bn, o, info);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(e1, num->stream));
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  info->ms = ms;
+  return AMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amp_sym_set_pattern(amp_sym* h, const int32_t* rowptr, const int32_t* colidx) {
+  if (!h || !rowptr || (h->S.n > 0 && !colidx)) return fail(AMP_ERR_ARG, "amp_sym_set_pattern: bad arguments");
+  const int32_t n = h->S.n;
+  if (int64_t(rowptr[n]) != h->S.nnzPattern) return fail(AMP_ERR_PATTERN, "amp_sym_set_pattern: entry count differs");
+  h->rowptr.assign(rowptr, rowptr + n + 1);
+  h->colidx.assign(colidx, colidx + rowptr[n]);
+  h->dRowptr.reset();
+  h->dColidx.reset();
+  return AMP_OK;
+}
+
+int amp_refine_device(amp_num* num, const double* d_values, const double* d_b, double* d_x, const amp_refine_opts* o,
+                      amp_refine_info* info) {
+  if (!num || !o || !info || (num->sym->S.n > 0 && (!d_values || !d_b || !d_x)))
+    return fail(AMP_ERR_ARG, "amp_refine: bad arguments");
+  *info = amp_refine_info{};
+  if (num->sym->S.n == 0) {
+    info->converged = 1;
+    return AMP_OK;
+  }
+  return refineDevice(num, d_values, d_b, d_x, o, info);
+}
+
+int amp_refine(amp_num* num, const double* values, const double* b, double* x, const amp_refine_opts* o,
+               amp_refine_info* info) {
+  if (!num || !o || !info || (num->sym->S.n > 0 && (!values || !b || !x)))
+    return fail(AMP_ERR_ARG, "amp_refine: bad arguments");
+  const int32_t n = num->sym->S.n;
+  *info = amp_refine_info{};
+  if (n == 0) {
+    info->converged = 1;
+    return AMP_OK;
+  }
+  if (!num->ok) return fail(AMP_ERR_STATE, "refine: no successful numeric factorization");
+  const size_t nv = size_t(num->sym->S.nnzPattern);
+  HIP_TRY(hipMemcpyAsync(num->values.p, values, nv * 8, hipMemcpyHostToDevice, num->stream));
+  HIP_TRY(hipMemcpyAsync(num->b.p, b, size_t(n) * 8, hipMemcpyHostToDevice, num->stream));
+  int rc = refineDevice(num, num->values.as<double>(), num->b.as<double>(), num->x.as<double>(), o, info);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(x, num->x.p, size_t(n) * 8, hipMemcpyDeviceToHost, num->stream));
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  return AMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// FP64 vector kernels of the device refinement: CSR residual, dot products and updates (gfx950).
+//
+// Iterative refinement (solveRefined, ldlt.mjs:253-275) and right-preconditioned restarted GMRES (gmres-real.mjs:31-104)
+// keep the residual r = b - A x, the solution update and the Krylov basis in FP64 in HBM; the factor of working
+// precision only enters through the preconditioner solve. Reductions are two-stage (per-block partials, then one block)
+// so results are deterministic run to run.
+#include "refine_kernels.h"
+
+namespace amp {
+
+namespace {
+
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 512;
+
+__device__ inline double blockSum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x < blockDim.x / 64) t = sh[threadIdx.x];
+  if (wave == 0)
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  __syncthreads();
+  return t;
+}
+
+}  // namespace
+
+// r = b - A x (row order of the CSR pattern, one row per thread); b == nullptr gives r = A x
+__global__ __launch_bounds__(256) void k_csr_residual(const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ colidx,
+                                                      const double* __restrict__ val, const double* __restrict__ x,
+                                                      const double* __restrict__ b, double* __restrict__ r, int32_t n) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = b ? b[i] : 0.0;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; p++) s -= val[p] * x[colidx[p]];
+    r[i] = b ? s : -s;
+  }
+}
+
+// partial[j * gridDim.x + block] = sum over the block's rows of V_j . w, for j < nv (V_j = V + j ldv)
+__global__ __launch_bounds__(kRedThreads) void k_multi_dot(const double* __restrict__ V, int64_t ldv, int32_t nv,
+                                                            const double* __restrict__ w, int32_t n,
+                                                            double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads / 64];
+  for (int32_t j = 0; j < nv; j++) {
+    const double* vj = V + j * ldv;
+    double s = 0;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += vj[i] * w[i];
+    s = blockSum(s, sh);
+    if (threadIdx.x == 0) partial[int64_t(j) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce_partials(const double* __restrict__ partial, int32_t nparts,
+                                                                  int32_t nv, double* __restrict__ out) {
+  __shared__ double sh[kRedThreads / 64];
+  for (int32_t j = 0; j < nv; j++) {
+    double s = 0;
+    for (int32_t i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[int64_t(j) * nparts + i];
+    s = blockSum(s, sh);
+    if (threadIdx.x == 0) out[j] = s;
+  }
+}
+
+// w -= sum_j h_j V_j (h on the device)
+__global__ void k_multi_axpy(const double* __restrict__ V, int64_t ldv, int32_t nv, const double* __restrict__ h,
+                             double* __restrict__ w, int32_t n) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int32_t j = 0; j < nv; j++) s -= h[j] * V[j * ldv + i];
+    w[i] = s;
+  }
+}
+
+// y = a x + c y   (scalars from the host)
+__global__ void k_axpby(double a, const double* __restrict__ x, double c, double* __restrict__ y, int32_t n) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i] + c * y[i];
+}
+
+// x += sum_j coef_j Z_j (coefficients from the host, at most kMaxRestart)
+__global__ void k_combine(const double* __restrict__ Z, int64_t ldz, int32_t nz, const double* __restrict__ coef,
+                          double* __restrict__ x, int32_t n) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int32_t j = 0; j < nz; j++) s += coef[j] * Z[j * ldz + i];
+    x[i] += s;
+  }
+}
+
+static unsigned gridFor(int32_t n) { return unsigned(std::min<int64_t>((int64_t(n) + 255) / 256, 4096)); }
+
+void launchResidual(const int32_t* rowptr, const int32_t* colidx, const double* val, const double* x, const double* b,
+                    double* r, int32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_csr_residual, dim3(gridFor(n)), dim3(256), 0, st, rowptr, colidx, val, x, b, r, n);
+}
+
+void launchMultiDot(const double* V, int64_t ldv, int32_t nv, const double* w, int32_t n, double* partial, double* out,
+                    hipStream_t st) {
+  const unsigned g = unsigned(std::min<int64_t>((int64_t(n) + kRedThreads - 1) / kRedThreads, kRedBlocks));
+  hipLaunchKernelGGL(k_multi_dot, dim3(g), dim3(kRedThreads), 0, st, V, ldv, nv, w, n, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, st, partial, int32_t(g), nv, out);
+}
+
+void launchMultiAxpy(const double* V, int64_t ldv, int32_t nv, const double* h, double* w, int32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_multi_axpy, dim3(gridFor(n)), dim3(256), 0, st, V, ldv, nv, h, w, n);
+}
+
+void launchAxpby(double a, const double* x, double c, double* y, int32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpby, dim3(gridFor(n)), dim3(256), 0, st, a, x, c, y, n);
+}
+
+void launchCombine(const double* Z, int64_t ldz, int32_t nz, const double* coef, double* x, int32_t n,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_combine, dim3(gridFor(n)), dim3(256), 0, st, Z, ldz, nz, coef, x, n);
+}
+
+int64_t partialEntries(int32_t maxVectors) { return int64_t(kRedBlocks) * maxVectors; }
+
+}  // namespace amp

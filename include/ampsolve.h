@@ -117,6 +117,34 @@ int amp_factor_device(amp_num *num, const double *d_values, int64_t nvalues, dou
 int amp_solve(amp_num *num, const double *b, double *x, int32_t nrhs);
 int amp_solve_device(amp_num *num, const double *d_b, double *d_x, int32_t nrhs);
 
+/* ---- refinement (device, FP64 residual and update) ---------------------------------------------------------- */
+
+#define AMP_REFINE_IR 0    /* solveRefined (ldlt.mjs:253-275): x = solve(b); x += solve(b - A x) */
+#define AMP_REFINE_GMRES 1 /* gmres (gmres-real.mjs:31-104) right-preconditioned by the factor, x0 = solve(b) */
+
+typedef struct {
+  int32_t method;   /* AMP_REFINE_IR or AMP_REFINE_GMRES */
+  int32_t maxSteps; /* IR: refinement steps (3 in the reference); GMRES: maximum iterations (400) */
+  int32_t restart;  /* GMRES restart length, 1..64 (40 in the reference) */
+  double tol;       /* relative residual |b - A x| / |b| to stop at (IR 1e-14, GMRES 1e-10 in the reference) */
+} amp_refine_opts;
+
+typedef struct {
+  int32_t steps;      /* IR: refinement steps taken; GMRES: iterations */
+  int32_t converged;  /* IR: relResidual < tol; GMRES: true residual <= max(10 tol, 1e-14) */
+  double relResidual; /* |b - A x| / |b| of the returned x (FP64) */
+  double ms;          /* device time of the whole refinement */
+} amp_refine_info;
+
+/* Refine the factor's solution against the unperturbed values on the analysed CSR pattern (one right-hand side).
+ * The residual r = b - A x, the update and the Krylov vectors stay in FP64 in HBM whatever the factor precision. */
+int amp_refine(amp_num *num, const double *values, const double *b, double *x, const amp_refine_opts *opts,
+               amp_refine_info *info);
+int amp_refine_device(amp_num *num, const double *d_values, const double *d_b, double *d_x,
+                      const amp_refine_opts *opts, amp_refine_info *info);
+/* The CSR pattern (rowptr n + 1, colidx) for the residual of an analysis taken over with amp_sym_import. */
+int amp_sym_set_pattern(amp_sym *sym, const int32_t *rowptr, const int32_t *colidx);
+
 /* The factor in the reference's layout: Lx (nnzLStored, m x nc column-major blocks at Lxp; the upper part of each
  * diagonal block is unspecified, as in the reference) and D (n, LDL^T pivots; ignored for Cholesky; may be NULL). */
 int amp_download_factor(amp_num *num, double *Lx, double *D);
