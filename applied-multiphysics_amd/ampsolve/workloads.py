@@ -176,3 +176,50 @@ def heat3d(ncell=60, seed=1, rhs_seed=2):
     R, _ = restrict(A, free)
     b = 2.0 * lcg_uniform(rhs_seed, R.shape[0]) - 1.0
     return csr_dict(R), R.data.astype(np.float64), b
+
+
+_TET_EDGES = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]])
+
+
+def maxwell_edge(ncell=20, sigma=50.0, seed=3, rhs_seed=4, eps_choices=(1.0, 2.1, 11.7)):
+    """Config: shift-inverted 3D Maxwell, lowest-order Nedelec (Whitney) edge elements on the unit cube, ncell^3 hexes
+    x 6 Kuhn tets, PEC walls eliminated, per-element eps_r drawn uniformly from {1.0, 2.1, 11.7}, mu_r = 1; the matrix
+    is K - sigma M (symmetric indefinite). Element matrices: curl w_ij = 2 grad(l_i) x grad(l_j),
+    int w_a . w_b from int l_p l_q = |T| (1 + delta_pq) / 20 (as fem/nedelec1.mjs assembles them). Edges are oriented
+    from the lower to the higher node index. Returns (pat, values, b)."""
+    nodes, tets = box_tet_mesh(ncell, ncell, ncell)
+    X = nodes[tets]  # (ne, 4, 3)
+    J = np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))
+    vol = np.abs(np.linalg.det(J)) / 6.0
+    Jinv = np.linalg.inv(J)
+    G = np.concatenate([-Jinv.sum(axis=1, keepdims=True), Jinv], axis=1)  # (ne, 4, 3) grad(lambda)
+    ne = len(tets)
+    nch = len(eps_choices)
+    eps = np.array(eps_choices)[np.minimum((nch * lcg_uniform(seed, ne)).astype(np.int64), nch - 1)]
+    I, Jn = _TET_EDGES[:, 0], _TET_EDGES[:, 1]
+    curl = 2.0 * np.cross(G[:, I, :], G[:, Jn, :])  # (ne, 6, 3)
+    Ke = vol[:, None, None] * np.einsum("eai,ebi->eab", curl, curl)
+    GG = np.einsum("eai,ebi->eab", G, G)  # grad l_p . grad l_q
+    d = lambda p, q: (1.0 + (p[:, None] == q[None, :]))  # noqa: E731
+    i_, j_ = I, Jn
+    Me = (vol / 20.0)[:, None, None] * (
+        d(i_, i_) * GG[:, j_][:, :, j_] - d(i_, j_) * GG[:, j_][:, :, i_]
+        - d(j_, i_) * GG[:, i_][:, :, j_] + d(j_, j_) * GG[:, i_][:, :, i_])
+    Ae = Ke - sigma * eps[:, None, None] * Me
+    a, b = tets[:, I], tets[:, Jn]  # local edge endpoints (global node ids)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    sgn = np.where(a < b, 1.0, -1.0)
+    nn = len(nodes)
+    keys = lo * nn + hi
+    ukeys, eid = np.unique(keys, return_inverse=True)
+    eid = eid.reshape(ne, 6)
+    Ae = Ae * sgn[:, :, None] * sgn[:, None, :]
+    A = assemble(len(ukeys), eid, Ae)
+    e0, e1 = ukeys // nn, ukeys % nn
+    on = np.zeros(len(ukeys), dtype=bool)
+    for ax in range(3):
+        for v in (0.0, 1.0):
+            on |= (np.abs(nodes[e0, ax] - v) < 1e-12) & (np.abs(nodes[e1, ax] - v) < 1e-12)
+    R, _ = restrict(A, ~on)
+    b = lcg_normal(rhs_seed, R.shape[0])
+    return csr_dict(R), R.data.astype(np.float64), b

@@ -163,7 +163,8 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
   HIP_TRY(hipStreamSynchronize(num->stream));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
-  num->kernelMs.clear();
+  for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
+    it = it->first.rfind("solve_", 0) == 0 ? std::next(it) : num->kernelMs.erase(it);
   if (num->profiling) {
     static const char* names[] = {"zero", "assemble", "small_front", "extend_add", "diag_block", "gemm_tile"};
     for (size_t i = 0; i < num->prof.used; i++) {
@@ -203,16 +204,37 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   if (!num->ok) return fail(AMP_ERR_STATE, "solve: no successful numeric factorization");
   int rc = ensureVecs(num, nrhs);
   if (rc) return rc;
+  amp::ProfileSink* prof = num->profiling ? &num->solveProf : nullptr;
+  if (prof) prof->used = 0;
   HIP_TRY(hipEventRecord(num->ev0, num->stream));
   if (num->prec == AMP_F64)
     amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs, num->y.as<double>(),
-                                         num->fv.as<double>(), num->stream);
+                                         num->fv.as<double>(), num->stream, prof);
   else
     amp::launchSolvePlan<float, float>(num->sym->dsp, solveArgs<float>(num), d_b, d_x, nrhs, num->y.as<float>(),
-                                       num->fv.as<float>(), num->stream);
+                                       num->fv.as<float>(), num->stream, prof);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(num->ev1, num->stream));
   HIP_TRY(hipStreamSynchronize(num->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
+  num->solveMs = ms;
+  if (prof) {
+    static const char* names[] = {"solve_small_fwd", "solve_gather", "solve_fwd_block",
+                                  "solve_small_bwd", "solve_bwd_init", "solve_bwd_block"};
+    for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
+      it = it->first.rfind("solve_", 0) == 0 ? num->kernelMs.erase(it) : std::next(it);
+    for (size_t i = 0; i < prof->used; i++) {
+      float k = 0;
+      HIP_TRY(hipEventElapsedTime(&k, prof->events[2 * i], prof->events[2 * i + 1]));
+      auto& slot = num->kernelMs[names[prof->kinds[i] - 16]];
+      slot.first += k;
+      slot.second += 1;
+    }
+    auto& total = num->kernelMs["solve_total"];
+    total.first = ms;
+    total.second = 1;
+  }
   return AMP_OK;
 }
 
@@ -463,6 +485,10 @@ int amp_num_set_profiling(amp_num* num, int32_t enable) {
     num->prof.events.resize(2 * nl + 2, nullptr);
     num->prof.kinds.resize(nl + 1, 0);
     for (hipEvent_t& e : num->prof.events) HIP_TRY(hipEventCreate(&e));
+    const size_t ns = num->sym->dsp.launches.size();
+    num->solveProf.events.resize(2 * ns + 2, nullptr);
+    num->solveProf.kinds.resize(ns + 1, 0);
+    for (hipEvent_t& e : num->solveProf.events) HIP_TRY(hipEventCreate(&e));
   }
   return AMP_OK;
 }

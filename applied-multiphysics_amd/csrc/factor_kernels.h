@@ -63,6 +63,6 @@ struct SolveArgs {
 // scratch of the working precision S; Lx / W / D are the factor's precision T.
 template <typename T, typename S>
 void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q, S* y,
-                     S* fv, hipStream_t st);
+                     S* fv, hipStream_t st, ProfileSink* prof);
 
 }  // namespace amp

@@ -161,78 +161,78 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
   }
 }
 
-// Diagonal block of a tiled front: right-looking factorization in LDS, then the inverse of its lower factor
-// (one column per thread, forward substitution) for the in-place panel solve and for the solves.
+// Diagonal block of a tiled front, one wave: lane i keeps row i of the block in registers; at step k lane k takes the
+// pivot, every lane scales its entry of column k, the column goes through LDS and each lane applies the rank-1 update
+// to its row (right-looking, 64 steps without workgroup-wide barriers). Then lane j computes column j of the inverse
+// of the lower factor by forward substitution against the factor in LDS (broadcast reads), and the inverse is written
+// back through LDS so the stores run along columns.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
+__global__ __launch_bounds__(64) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
   constexpr int E = kTileEdge, P = kTileEdge + 1;
-  __shared__ T Fs[E * P];
+  __shared__ T col[E];
+  __shared__ T Ls[E * P];
   __shared__ T Ws[E * P];
-  __shared__ T wcol[E];
   __shared__ T s_inv, s_mult;
   const DiagTask t = tasks[blockIdx.x];
-  const int tid = threadIdx.x, nb = t.nb;
+  const int lane = threadIdx.x, nb = t.nb;
   T* F = a.Lx + t.f_off;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int i = idx % nb, j = idx / nb;
-    Fs[i + j * P] = i >= j ? F[i + int64_t(j) * t.ld] : T(0);
+  T r[E];
+#pragma unroll
+  for (int j = 0; j < E; j++) r[j] = (j < nb && lane < nb && lane >= j) ? F[lane + int64_t(j) * t.ld] : T(0);
+#pragma unroll
+  for (int k = 0; k < E; k++) {
+    if (k < nb) {
+      if (lane == k) {
+        T inv, mult;
+        r[k] = pivotValue<T>(r[k], t.col0 + k, a, &inv, &mult);
+        s_inv = inv;
+        s_mult = mult;
+      }
+      __syncthreads();
+      const T inv = s_inv, mult = s_mult;
+      if (lane > k) r[k] *= inv;
+      col[lane] = r[k] * mult;
+      __syncthreads();
+      const T lik = r[k];
+#pragma unroll
+      for (int j = k + 1; j < E; j++)
+        if (lane >= j) r[j] -= lik * col[j];
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = r[j];
+    Ls[lane * P + j] = r[j];  // row-major copy of the factor (zeros above the diagonal)
   }
   __syncthreads();
-  for (int k = 0; k < nb; k++) {
-    if (tid == 0) {
-      T inv, mult;
-      Fs[k + k * P] = pivotValue<T>(Fs[k + k * P], t.col0 + k, a, &inv, &mult);
-      s_inv = inv;
-      s_mult = mult;
-    }
-    __syncthreads();
-    const T inv = s_inv, mult = s_mult;
-    for (int i = k + 1 + tid; i < nb; i += 256) {
-      const T l = Fs[i + k * P] * inv;
-      Fs[i + k * P] = l;
-      wcol[i] = l * mult;
-    }
-    __syncthreads();
-    const int w = nb - k - 1;
-    for (int idx = tid; idx < w * w; idx += 256) {
-      const int ii = idx % w, jj = idx / w;
-      if (ii >= jj) Fs[(k + 1 + ii) + (k + 1 + jj) * P] -= Fs[(k + 1 + ii) + k * P] * wcol[k + 1 + jj];
-    }
-    __syncthreads();
+  // column `lane` of the inverse: L w = e_lane (unit diagonal for LDL^T), kept in Ws[i][lane]
+  for (int i = 0; i < nb; i++) {
+    T sum = i == lane ? T(1) : T(0);
+    for (int k = lane; k < i; k++) sum -= Ls[i * P + k] * Ws[k * P + lane];
+    Ws[i * P + lane] = i >= lane ? (a.ldlt ? sum : sum / Ls[i * P + i]) : T(0);
   }
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int i = idx % nb, j = idx / nb;
-    if (i >= j) F[i + int64_t(j) * t.ld] = Fs[i + j * P];
-  }
-  if (tid < nb) {
-    const int j = tid;
-    for (int i = 0; i < j; i++) Ws[i + j * P] = T(0);
-    for (int i = j; i < nb; i++) {
-      T sum = i == j ? T(1) : T(0);
-      for (int k = j; k < i; k++) sum -= Fs[i + k * P] * Ws[k + j * P];
-      Ws[i + j * P] = a.ldlt ? sum : sum / Fs[i + i * P];
-    }
-  }
+  for (int i = nb; i < E; i++) Ws[i * P + lane] = T(0);
   __syncthreads();
-  T* W = a.W + t.w_off;
-  for (int idx = tid; idx < E * E; idx += 256) {
-    const int i = idx % E, j = idx / E;
-    W[idx] = (i < nb && j < nb) ? Ws[i + j * P] : T(0);
-  }
+  T* Wt = a.W + t.w_off;
+  for (int j = 0; j < E; j++) Wt[lane + j * E] = lane < nb && j < nb ? Ws[lane * P + j] : T(0);
 }
 
 // 64 x 64 output tile of C (+)= -A diag(d) B^T (modes 0, 1) or C = A B^T diag(d)^-1 (mode 2) on the matrix cores.
-// Four waves, each owning a 32 x 32 quarter (2 x 2 MFMA blocks of 16 x 16); depth staged through LDS in chunks of 16
-// (A and B stored depth-major so a lane's operand is one LDS read); the tile is written back through LDS so every
-// global access is along columns.
+// Four waves, each owning a 32 x 32 quarter (2 x 2 MFMA blocks of 16 x 16). The depth is staged through LDS in chunks
+// of 32 (A and B depth-major, so a lane's operand is one LDS read); the next chunk is loaded into registers while the
+// current one feeds the matrix cores. The tile is written back through the same LDS so every global access runs
+// along columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U, T* W,
                                                   const T* __restrict__ D, int32_t ldlt) {
   using V4 = typename Vec4<T>::type;
-  constexpr int E = kTileEdge, KC = 16, PA = kTileEdge + 4, PC = kTileEdge + 1;
-  __shared__ T As[KC * PA];
-  __shared__ T Bs[KC * PA];
-  __shared__ T Cs[E * PC];
+  constexpr int E = kTileEdge, KC = 32, PA = kTileEdge + 4, PC = kTileEdge + 1, NL = KC / 4;
+  static_assert(2 * KC * PA >= E * PC, "epilogue staging must fit in the operand buffers");
+  __shared__ T smem[2 * KC * PA];
+  T* As = smem;
+  T* Bs = smem + KC * PA;
+  T* Cs = smem;
   const GemmTask t = tasks[blockIdx.x];
   T* const bases[3] = {Lx, U, W};
   const T* A = bases[t.a_buf] + t.a_off;
@@ -241,21 +241,32 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
   const bool scaleDepth = ldlt && t.mode != 2;
+  const int li = tid & 63, lk = tid >> 6;  // loader: row li, depths lk + 4 r
+  const bool rowA = li < t.rows, rowB = li < t.cols;
   V4 acc[2][2];
   for (int bi = 0; bi < 2; bi++)
     for (int bj = 0; bj < 2; bj++) acc[bi][bj] = V4{0, 0, 0, 0};
-  const int li = tid & 63, lk = tid >> 6;  // loader: row li, depth lk + 4 r
+  T pa[NL], pb[NL];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NL; r++) {
+      const int k = k0 + lk + 4 * r;
+      const bool kin = k < t.k;
+      pa[r] = (kin && rowA) ? A[li + int64_t(k) * t.lda] : T(0);
+      T bv = (kin && rowB) ? B[li + int64_t(k) * t.ldb] : T(0);
+      if (scaleDepth && kin) bv *= D[t.d_off + k];
+      pb[r] = bv;
+    }
+  };
+  load(0);
   for (int k0 = 0; k0 < t.k; k0 += KC) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int kk = lk + 4 * r, k = k0 + kk;
-      const bool kin = k < t.k;
-      As[kk * PA + li] = (kin && li < t.rows) ? A[li + int64_t(k) * t.lda] : T(0);
-      T bv = (kin && li < t.cols) ? B[li + int64_t(k) * t.ldb] : T(0);
-      if (scaleDepth && kin) bv *= D[t.d_off + k];
-      Bs[kk * PA + li] = bv;
+    for (int r = 0; r < NL; r++) {
+      As[(lk + 4 * r) * PA + li] = pa[r];
+      Bs[(lk + 4 * r) * PA + li] = pb[r];
     }
     __syncthreads();
+    if (k0 + KC < t.k) load(k0 + KC);
 #pragma unroll
     for (int k4 = 0; k4 < KC; k4 += 4) {
       const int kk = k4 + (lane >> 4);
@@ -270,7 +281,7 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
   for (int bi = 0; bi < 2; bi++)
     for (int bj = 0; bj < 2; bj++)
       for (int r = 0; r < 4; r++) {
-        const int row = wr + bi * 16 + 4 * (lane >> 4) + r, col = wc + bj * 16 + (lane & 15);
+        const int row = wr + bi * 16 + mfmaRow<T>(lane, r), col = wc + bj * 16 + (lane & 15);
         Cs[row + col * PC] = acc[bi][bj][r];
       }
   __syncthreads();
@@ -355,7 +366,7 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         hipLaunchKernelGGL(k_extend_add<T>, dim3(grid), dim3(256), 0, st, dp.ea + L.first, a);
         break;
       case kDiag:
-        hipLaunchKernelGGL(k_diag_block<T>, dim3(grid), dim3(256), 0, st, dp.diag + L.first, a);
+        hipLaunchKernelGGL(k_diag_block<T>, dim3(grid), dim3(64), 0, st, dp.diag + L.first, a);
         break;
       case kGemm:
         hipLaunchKernelGGL(k_gemm_tile<T>, dim3(grid), dim3(256), 0, st,

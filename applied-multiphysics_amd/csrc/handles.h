@@ -85,10 +85,12 @@ struct amp_num {
   amp::DevBuf r, dvec, w, V, Z, partial, small;
   bool ok = false;
   bool profiling = false;
-  amp::ProfileSink prof;
+  amp::ProfileSink prof, solveProf;
+  float solveMs = 0;
   std::map<std::string, std::pair<double, int32_t>> kernelMs;
   ~amp_num() {
     for (hipEvent_t e : prof.events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : solveProf.events) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);

@@ -22,13 +22,23 @@ struct Vec4<float> {
   using type = v4f;
 };
 
-// 16x16x4 matrix-core step D = A B + C. Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; it receives
-// D[4 (l / 16) + r][l % 16] in component r.
+// 16x16x4 matrix-core step D = A B + C. Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]. The result rows
+// differ by type: FP64 component r of lane l is D[4 r + l / 16][l % 16]; FP32 component r is D[4 (l / 16) + r][l % 16].
 __device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 __device__ inline v4f mfma16x16x4(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T>
+__device__ inline int mfmaRow(int lane, int r);
+template <>
+__device__ inline int mfmaRow<double>(int lane, int r) {
+  return 4 * r + (lane >> 4);
+}
+template <>
+__device__ inline int mfmaRow<float>(int lane, int r) {
+  return 4 * (lane >> 4) + r;
 }
 
 // Kernel-side view of the factor storage and of the plan's per-supernode arrays.

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_block(const BlockTask* __rest
 
 template <typename T, typename S>
 void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q, S* y,
-                     S* fv, hipStream_t st) {
+                     S* fv, hipStream_t st, ProfileSink* prof) {
   const int32_t n = a.n;
   if (n == 0 || q == 0) return;
   const unsigned g = unsigned(std::min<int64_t>((int64_t(n) * q + 255) / 256, 65536));
@@ -234,6 +234,11 @@ void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
       scaled = true;
     }
     const unsigned grid = unsigned(L.count);
+    const bool rec = prof && 2 * prof->used + 1 < prof->events.size();
+    if (rec) {
+      (void)hipEventRecord(prof->events[2 * prof->used], st);
+      prof->kinds[prof->used] = 16 + L.kind;
+    }
     switch (L.kind) {
       case kSolveSmallFwd:
         hipLaunchKernelGGL((k_solve_small_fwd<T, S>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
@@ -260,16 +265,20 @@ void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                            dp.fvEntries);
         break;
     }
+    if (rec) {
+      (void)hipEventRecord(prof->events[2 * prof->used + 1], st);
+      prof->used++;
+    }
   }
   if (a.ldlt && !scaled) hipLaunchKernelGGL((k_diag_scale<T, S>), dim3(g), dim3(256), 0, st, y, a.D, n, q);
   hipLaunchKernelGGL(k_perm_out<S>, dim3(g), dim3(256), 0, st, y, x, a.perm, n, q);
 }
 
 template void launchSolvePlan<double, double>(const DeviceSolvePlan&, const SolveArgs<double>&, const double*, double*,
-                                              int32_t, double*, double*, hipStream_t);
+                                              int32_t, double*, double*, hipStream_t, ProfileSink*);
 template void launchSolvePlan<float, float>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                            int32_t, float*, float*, hipStream_t);
+                                            int32_t, float*, float*, hipStream_t, ProfileSink*);
 template void launchSolvePlan<float, double>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                             int32_t, double*, double*, hipStream_t);
+                                             int32_t, double*, double*, hipStream_t, ProfileSink*);
 
 }  // namespace amp

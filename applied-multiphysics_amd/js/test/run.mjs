@@ -61,7 +61,7 @@ if (mode === '--gpu') {
   const bad = new SparseCholesky(pat);
   out.indefinite_ok = bad.factor(laplace(30, 6).val);
   try { bad.solve(b); } catch (e) { out.solve_error = e.message; }
-  const ind = laplace(30, 4);
+  const ind = laplace(30, 1.3);
   const f = new SupernodalLDLT(sym);
   const r = f.factor(ind.val);
   out.ldlt = { ok: r.ok, nNeg: r.nNeg, nPerturbed: r.nPerturbed };

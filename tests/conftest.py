@@ -11,6 +11,14 @@ for p in (os.path.join(ROOT, "applied-multiphysics_amd"), os.path.join(ROOT, "or
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    # PyTorch-ROCm carries its own HIP runtime: bring its device up before libampsolve initialises /opt/rocm's, so
+    # that tests which hand torch device buffers to the library see the device in both
+    try:
+        import torch
+
+        torch.cuda.is_available()
+    except Exception:
+        pass
 
 
 def gpu_available():

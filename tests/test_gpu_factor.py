@@ -33,20 +33,23 @@ def residual(pat, v, x, b):
     return np.linalg.norm(A @ x - b) / np.linalg.norm(b)
 
 
+# (pattern, values, symmetric values?) -- the grid() fixtures of la-real-front.test.mjs are deliberately
+# non-symmetric: the factor reads only the entries analyze() maps through lowerSrc, so no residual against the CSR
 CASES = {
-    "poisson2d_64": lambda: W.poisson2d(64)[:2],
-    "grid27_11": lambda: grid27(11, 11, 11)[:2],
-    "grid27_23x9x5": lambda: grid27(23, 9, 5)[:2],
-    "grid27_10_zeros": lambda: grid27(10, 10, 10, zeroEvery=3)[:2],
-    "heat3d_14": lambda: W.heat3d(14)[:2],
-    "hex_matrix": lambda: hex_matrix(),
+    "poisson2d_64": lambda: (*W.poisson2d(64)[:2], True),
+    "grid27_11": lambda: (*grid27(11, 11, 11)[:2], False),
+    "grid27_23x9x5": lambda: (*grid27(23, 9, 5)[:2], False),
+    "grid27_10_zeros": lambda: (*grid27(10, 10, 10, zeroEvery=3)[:2], False),
+    "grid27_12_sym": lambda: (*grid27(12, 12, 12, symmetric=True)[:2], True),
+    "heat3d_14": lambda: (*W.heat3d(14)[:2], True),
+    "hex_matrix": lambda: (*hex_matrix(), True),
 }
 
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_cholesky_matches_oracle(name):
     print(f"case {name}", flush=True)
-    pat, v = CASES[name]()
+    pat, v, symmetric = CASES[name]()
     n = pat["n"]
     b = 2 * W.lcg_uniform(7, n) - 1
     sym = amp.analyze(pat)
@@ -61,7 +64,8 @@ def test_cholesky_matches_oracle(name):
     assert relmax(Lg[m], Lo[m]) < 1e-11
     xg = amp.solveNumeric(num, b)
     assert relmax(xg, xo) < 1e-12
-    assert residual(pat, v, xg, b) < 1e-12
+    if symmetric:
+        assert residual(pat, v, xg, b) < 1e-12
 
 
 def test_tridiagonal_closed_form():
@@ -91,8 +95,10 @@ def test_tridiagonal_closed_form():
 @pytest.mark.parametrize("dims,opt", [((11, 11, 11), {}), ((23, 9, 5), {}), ((10, 10, 10), {"zeroEvery": 3})])
 @pytest.mark.parametrize("pivot_tol", [1e-13, 0.05])
 def test_ldlt_indefinite_matches_oracle(dims, opt, pivot_tol):
+    """The shifted grids are nearly singular (diagonal 26.5 - 26), so the two factors are compared through the inertia
+    (exact), the pivots and a residual of the same level as the oracle's, against the symmetric values."""
     print(f"ldlt {dims} {opt} {pivot_tol}", flush=True)
-    pat, v, n = grid27(*dims, shift=26, **opt)
+    pat, v, n = grid27(*dims, shift=26, symmetric=True, **opt)
     b = np.sin(np.arange(n, dtype=np.float64))
     sym = amp.analyze(pat)
     So = O.Sym(pat)
@@ -108,7 +114,10 @@ def test_ldlt_indefinite_matches_oracle(dims, opt, pivot_tol):
     assert relmax(Dg, Do) < 1e-10
     xo = O.ldlt_solve(So, Lo, Do, b)
     xg = f.solve(b)
-    assert relmax(xg, xo) < (1e-12 if pivot_tol < 1e-3 else 1e-9)
+    ro, rg = residual(pat, v, xo, b), residual(pat, v, xg, b)
+    assert rg <= 10 * ro + 1e-13, (rg, ro)
+    if pivot_tol < 1e-3:
+        assert relmax(xg, xo) < 1e-9
 
 
 def test_cholesky_reports_indefinite_and_solve_refuses():
