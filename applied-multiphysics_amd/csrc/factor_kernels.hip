@@ -5,42 +5,10 @@
 // ldlt.mjs:492-535, pivots as wasm/front-real.mjs:367-389), reorganised as a blocked right-looking factorization
 // whose trailing products run on the matrix cores.
 #include "factor_kernels.h"
+#include "pivot.h"
 
 namespace amp {
 
-namespace {
-
-// Pivot of column k (value d already updated by every earlier column). Cholesky: d -> sqrt(d), failure unless d > 0.
-// LDL^T: |d| < tau is replaced by +-tau (counted), a non-finite small pivot fails; the column keeps a unit diagonal.
-// Writes the scaling factor to *inv and the weight multiplier (1 or d) to *mult. Failures only record the column:
-// the factorization runs to the end with the IEEE result and the host reports the first failing column.
-template <typename T>
-__device__ inline T pivotValue(T d, int32_t col, const FactorArgs<T>& a, T* inv, T* mult) {
-  if (!a.ldlt) {
-    if (!(d > T(0))) atomicMin(a.failCol, col);
-    const T sq = sqrt(d);
-    *inv = T(1) / sq;
-    *mult = T(1);
-    return sq;
-  }
-  const T tau = T(a.tau);
-  uint8_t perturbed = 0;
-  if (!(fabs(d) >= tau)) {
-    if (!isfinite(d))
-      atomicMin(a.failCol, col);
-    else {
-      d = d < T(0) ? -tau : tau;
-      perturbed = 1;
-    }
-  }
-  a.pert[col] = perturbed;
-  a.D[col] = d;
-  *inv = T(1) / d;
-  *mult = d;
-  return T(1);
-}
-
-}  // namespace
 
 template <typename T>
 __global__ void k_zero(const ZeroTask* __restrict__ tasks, T* __restrict__ Lx) {
@@ -145,77 +113,20 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
       const int32_t mid = (lo + hi) >> 1;
       if (rel[mid] < t.j1) lo = mid + 1; else hi = mid;
     }
-    const int32_t nb = lo - b0;
-    const int64_t total = int64_t(nb) * muc;
-    for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int32_t r = int32_t(idx % muc), b = b0 + int32_t(idx / muc);
-      if (r < b) continue;
-      const int32_t i = rel[r], j = rel[b];
-      const T v = Uc[r + int64_t(b) * muc];
-      if (j < nc)
-        L[i + int64_t(j) * m] += v;
-      else
-        Us[(i - nc) + int64_t(j - nc) * mu] += v;
+    const int32_t b1 = lo;
+    for (int32_t b = b0; b < b1; b++) {  // child column b lands in parent column rel[b], rows rel[r] for r >= b
+      const int32_t j = rel[b];
+      const T* src = Uc + int64_t(b) * muc;
+      if (j < nc) {
+        T* dst = L + int64_t(j) * m;
+        for (int32_t r = b + threadIdx.x; r < muc; r += blockDim.x) dst[rel[r]] += src[r];
+      } else {
+        T* dst = Us + int64_t(j - nc) * mu;
+        for (int32_t r = b + threadIdx.x; r < muc; r += blockDim.x) dst[rel[r] - nc] += src[r];
+      }
     }
     __syncthreads();
   }
-}
-
-// Diagonal block of a tiled front, one wave: lane i keeps row i of the block in registers; at step k lane k takes the
-// pivot, every lane scales its entry of column k, the column goes through LDS and each lane applies the rank-1 update
-// to its row (right-looking, 64 steps without workgroup-wide barriers). Then lane j computes column j of the inverse
-// of the lower factor by forward substitution against the factor in LDS (broadcast reads), and the inverse is written
-// back through LDS so the stores run along columns.
-template <typename T>
-__global__ __launch_bounds__(64) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
-  constexpr int E = kTileEdge, P = kTileEdge + 1;
-  __shared__ T col[E];
-  __shared__ T Ls[E * P];
-  __shared__ T Ws[E * P];
-  __shared__ T s_inv, s_mult;
-  const DiagTask t = tasks[blockIdx.x];
-  const int lane = threadIdx.x, nb = t.nb;
-  T* F = a.Lx + t.f_off;
-  T r[E];
-#pragma unroll
-  for (int j = 0; j < E; j++) r[j] = (j < nb && lane < nb && lane >= j) ? F[lane + int64_t(j) * t.ld] : T(0);
-#pragma unroll
-  for (int k = 0; k < E; k++) {
-    if (k < nb) {
-      if (lane == k) {
-        T inv, mult;
-        r[k] = pivotValue<T>(r[k], t.col0 + k, a, &inv, &mult);
-        s_inv = inv;
-        s_mult = mult;
-      }
-      __syncthreads();
-      const T inv = s_inv, mult = s_mult;
-      if (lane > k) r[k] *= inv;
-      col[lane] = r[k] * mult;
-      __syncthreads();
-      const T lik = r[k];
-#pragma unroll
-      for (int j = k + 1; j < E; j++)
-        if (lane >= j) r[j] -= lik * col[j];
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < E; j++) {
-    if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = r[j];
-    Ls[lane * P + j] = r[j];  // row-major copy of the factor (zeros above the diagonal)
-  }
-  __syncthreads();
-  // column `lane` of the inverse: L w = e_lane (unit diagonal for LDL^T), kept in Ws[i][lane]
-  for (int i = 0; i < nb; i++) {
-    T sum = i == lane ? T(1) : T(0);
-    for (int k = lane; k < i; k++) sum -= Ls[i * P + k] * Ws[k * P + lane];
-    Ws[i * P + lane] = i >= lane ? (a.ldlt ? sum : sum / Ls[i * P + i]) : T(0);
-  }
-  for (int i = nb; i < E; i++) Ws[i * P + lane] = T(0);
-  __syncthreads();
-  T* Wt = a.W + t.w_off;
-  for (int j = 0; j < E; j++) Wt[lane + j * E] = lane < nb && j < nb ? Ws[lane * P + j] : T(0);
 }
 
 // 64 x 64 output tile of C (+)= -A diag(d) B^T (modes 0, 1) or C = A B^T diag(d)^-1 (mode 2) on the matrix cores.
@@ -366,7 +277,7 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         hipLaunchKernelGGL(k_extend_add<T>, dim3(grid), dim3(256), 0, st, dp.ea + L.first, a);
         break;
       case kDiag:
-        hipLaunchKernelGGL(k_diag_block<T>, dim3(grid), dim3(64), 0, st, dp.diag + L.first, a);
+        launchDiagBlocks<T>(dp.diag + L.first, grid, a, st);
         break;
       case kGemm:
         hipLaunchKernelGGL(k_gemm_tile<T>, dim3(grid), dim3(256), 0, st,

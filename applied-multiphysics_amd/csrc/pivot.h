@@ -1,0 +1,41 @@
+// Pivot rule shared by the factor kernels (device).
+#pragma once
+#include "kernels_common.h"
+
+namespace amp {
+
+// Pivot of column k (value d already updated by every earlier column). Cholesky: d -> sqrt(d), failure unless d > 0.
+// LDL^T: |d| < tau is replaced by +-tau (counted), a non-finite small pivot fails; the column keeps a unit diagonal.
+// Writes the scaling factor to *inv and the weight multiplier (1 or d) to *mult. Failures only record the column:
+// the factorization runs to the end with the IEEE result and the host reports the first failing column.
+template <typename T>
+__device__ inline T pivotValue(T d, int32_t col, const FactorArgs<T>& a, T* inv, T* mult) {
+  if (!a.ldlt) {
+    if (!(d > T(0))) atomicMin(a.failCol, col);
+    const T sq = sqrt(d);
+    *inv = T(1) / sq;
+    *mult = T(1);
+    return sq;
+  }
+  const T tau = T(a.tau);
+  uint8_t perturbed = 0;
+  if (!(fabs(d) >= tau)) {
+    if (!isfinite(d))
+      atomicMin(a.failCol, col);
+    else {
+      d = d < T(0) ? -tau : tau;
+      perturbed = 1;
+    }
+  }
+  a.pert[col] = perturbed;
+  a.D[col] = d;
+  *inv = T(1) / d;
+  *mult = d;
+  return T(1);
+}
+
+
+template <typename T>
+void launchDiagBlocks(const DiagTask* tasks, unsigned count, const FactorArgs<T>& a, hipStream_t st);
+
+}  // namespace amp

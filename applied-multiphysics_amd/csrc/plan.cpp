@@ -310,7 +310,8 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     int64_t bf = int64_t(P.blocks.size());
     for (int32_t s : big) {
       const int32_t nc = frontCols(S, s);
-      for (int32_t c = 0; c < nc; c += kSolveCols) P.blocks.push_back(BlockTask{s, -1, c, std::min(nc, c + kSolveCols), 0, 0});
+      for (int32_t c = 0; c < nc; c += kSolveInitCols)
+        P.blocks.push_back(BlockTask{s, -1, c, std::min(nc, c + kSolveInitCols), 0, 0});
     }
     push(kSolveBwdInit, L, bf, int64_t(P.blocks.size()) - bf);
     int32_t maxBlk = 0;

@@ -17,8 +17,9 @@ constexpr int kTileEdge = 64;   // panel width / update-tile edge of tiled front
 constexpr int kEaWidth = 16;    // parent columns per extend-add work item
 constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
 constexpr int kZeroChunk = 1 << 18;
-constexpr int kSolveRows = 256;  // rows per work item of the blocked forward solve
-constexpr int kSolveCols = 64;   // columns per work item of the blocked backward solve
+constexpr int kSolveRows = 128;     // rows per work item of the blocked forward solve
+constexpr int kSolveCols = 64;      // columns per work item of the blocked backward solve
+constexpr int kSolveInitCols = 16;  // columns per work item of the backward initialisation (z = y - L21^T y_below)
 
 enum Buf : int8_t { kBufLx = 0, kBufU = 1, kBufW = 2 };
 

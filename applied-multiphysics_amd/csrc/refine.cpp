@@ -134,7 +134,7 @@ int runGmres(Ctx& c, const double* d_b, double* d_x, double bn, const amp_refine
       }
       const double a = H[size_t(k) * m + k], b = H[size_t(k + 1) * m + k];
       double d = std::hypot(a, b);
-      if (!(d != 0)) d = 1e-300;
+      if (!(d != 0) || std::isnan(d)) d = 1e-300;  // `Math.hypot(a, c) || 1e-300`
       cs[k] = a / d;
       sn[k] = b / d;
       H[size_t(k) * m + k] = d;

@@ -75,9 +75,10 @@ __global__ void k_multi_axpy(const double* __restrict__ V, int64_t ldv, int32_t 
   }
 }
 
-// y = a x + c y   (scalars from the host)
+// y = a x + c y (scalars from the host); with c == 0 the old y is not read (fresh storage may hold NaN patterns)
 __global__ void k_axpby(double a, const double* __restrict__ x, double c, double* __restrict__ y, int32_t n) {
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i] + c * y[i];
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = c == 0.0 ? a * x[i] : a * x[i] + c * y[i];
 }
 
 // x += sum_j coef_j Z_j (coefficients from the host, at most kMaxRestart)

@@ -99,32 +99,53 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
   for (int32_t k = 0; k < q; k++) gatherFront<T, S>(a, s, y + int64_t(k) * a.n, fv + k * fvN, fv + k * fvN + a.Sp[s], nc, m);
 }
 
-// Forward step of one 64-column block of a tiled supernode: x = Winv v_blk (every work item recomputes it), then
-// v[r] -= L[r, blk] x for the item's rows; the writer item stores x into y.
+// The inverted 64 x 64 diagonal block of block `blk` of supernode s into LDS (row i, column j at Wsh[i + j * 65]).
+template <typename T, typename S>
+__device__ inline void loadInverse(const SolveArgs<T>& a, int32_t s, int32_t blk, S* Wsh) {
+  const T* W = a.W + a.Woff[s] + int64_t(blk) * kTileEdge * kTileEdge;
+  for (int idx = threadIdx.x; idx < kTileEdge * kTileEdge; idx += blockDim.x)
+    Wsh[(idx & 63) + (idx >> 6) * (kTileEdge + 1)] = S(W[idx]);
+}
+
+// Forward step of one 64-column block of a tiled supernode: x = Winv v_blk (every work item recomputes it from the
+// inverse staged in LDS), then v[r] -= L[r, blk] x for the item's rows, four threads per row over 16 columns each;
+// the writer item stores x into y.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_solve_fwd_block(const BlockTask* __restrict__ tasks, SolveArgs<T> a, S* y,
                                                          S* fv, int32_t q, int64_t fvN) {
-  __shared__ S vb[kTileEdge], xb[kTileEdge];
+  constexpr int E = kTileEdge, P = kTileEdge + 1;
+  __shared__ S Wsh[E * P];
+  __shared__ S vb[E], xb[E];
+  __shared__ S part[4][kSolveRows];
   const BlockTask t = tasks[blockIdx.x];
   const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t c0 = t.blk * kTileEdge, nb = min(kTileEdge, nc - c0);
-  const T* W = a.W + a.Woff[s] + int64_t(t.blk) * kTileEdge * kTileEdge;
+  const int32_t c0 = t.blk * E, nb = min(E, nc - c0);
   const T* L = a.Lx + a.Lxp[s];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
+  loadInverse<T, S>(a, s, t.blk, Wsh);
   for (int32_t k = 0; k < q; k++) {
     S* fvs = fv + k * fvN + a.Sp[s];
-    if (tid < nb) vb[tid] = fvs[c0 + tid];
+    if (tid < E) vb[tid] = tid < nb ? fvs[c0 + tid] : S(0);
     __syncthreads();
-    if (tid < nb) {
+    if (tid < E) {
       S sum = 0;
-      for (int kk = 0; kk <= tid; kk++) sum += S(W[tid + kk * kTileEdge]) * vb[kk];
+#pragma unroll 16
+      for (int kk = 0; kk < E; kk++) sum += Wsh[tid + kk * P] * vb[kk];
       xb[tid] = sum;
     }
     __syncthreads();
-    for (int r = t.r0 + tid; r < t.r1; r += blockDim.x) {
+    for (int32_t r0 = t.r0; r0 < t.r1; r0 += 64) {
+      const int32_t r = r0 + rr;
       S sum = 0;
-      for (int j = 0; j < nb; j++) sum += S(L[r + int64_t(c0 + j) * m]) * xb[j];
-      fvs[r] -= sum;
+      if (r < t.r1) {
+        const T* Lr = L + r + int64_t(c0 + 16 * g) * m;
+#pragma unroll
+        for (int j = 0; j < 16; j++) sum += S(Lr[int64_t(j) * m]) * xb[16 * g + j];
+      }
+      part[g][rr] = sum;
+      __syncthreads();
+      if (g == 0 && r < t.r1) fvs[r] -= (part[0][rr] + part[1][rr]) + (part[2][rr] + part[3][rr]);
+      __syncthreads();
     }
     if (t.writer && tid < nb) y[int64_t(k) * a.n + f + c0 + tid] = xb[tid];
     __syncthreads();
@@ -166,47 +187,67 @@ __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restri
   }
 }
 
-// z_i = y_own[i] - L21[:, i] . y[below] for the item's columns; one wave per column, lanes over the rows.
+// z_i = y_own[i] - L21[:, i] . y[below] for the item's (at most 16) columns: y[below] is staged through LDS in chunks,
+// each wave owns four columns and its lanes run over the rows.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a, S* y,
                                                         S* fv, int32_t q, int64_t fvN) {
+  constexpr int CH = 2048;
+  __shared__ S yb[CH];
   const BlockTask t = tasks[blockIdx.x];
   const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   const T* L = a.Lx + a.Lxp[s];
   const int32_t* rows = a.Si + a.Sp[s] + nc;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int32_t i0 = t.r0 + 4 * wave;
   for (int32_t k = 0; k < q; k++) {
     const S* yk = y + int64_t(k) * a.n;
+    S acc[4] = {0, 0, 0, 0};
+    for (int32_t base = 0; base < mu; base += CH) {
+      const int32_t len = min(CH, mu - base);
+      __syncthreads();
+      for (int r = threadIdx.x; r < len; r += blockDim.x) yb[r] = yk[rows[base + r]];
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        if (i0 + c >= t.r1) continue;
+        const T* Li = L + nc + base + int64_t(i0 + c) * m;
+        S sum = 0;
+        for (int r = lane; r < len; r += 64) sum += S(Li[r]) * yb[r];
+        acc[c] += sum;
+      }
+    }
     S* fvs = fv + k * fvN + a.Sp[s];
-    for (int32_t i = t.r0 + wave; i < t.r1; i += 4) {
-      const T* Li = L + nc + int64_t(i) * m;
-      S sum = 0;
-      for (int32_t r = lane; r < mu; r += 64) sum += S(Li[r]) * yk[rows[r]];
-      sum = waveSum(sum);
-      if (lane == 0) fvs[i] = yk[f + i] - sum;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const S v = waveSum(acc[c]);
+      if (lane == 0 && i0 + c < t.r1) fvs[i0 + c] = yk[f + i0 + c] - v;
     }
   }
 }
 
-// Backward step of one block: x = Winv^T z_blk (every item recomputes it), then z_i -= L[blk, i] . x for the item's
-// columns i (one wave per column); the writer item stores x into y.
+// Backward step of one block: x = Winv^T z_blk (every item recomputes it from the inverse staged in LDS), then
+// z_i -= L[blk, i] . x for the item's columns i (one wave per column); the writer item stores x into y.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_solve_bwd_block(const BlockTask* __restrict__ tasks, SolveArgs<T> a, S* y,
                                                          S* fv, int32_t q, int64_t fvN) {
-  __shared__ S zb[kTileEdge], xb[kTileEdge];
+  constexpr int E = kTileEdge, P = kTileEdge + 1;
+  __shared__ S Wsh[E * P];
+  __shared__ S zb[E], xb[E];
   const BlockTask t = tasks[blockIdx.x];
   const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t c0 = t.blk * kTileEdge, nb = min(kTileEdge, nc - c0);
-  const T* W = a.W + a.Woff[s] + int64_t(t.blk) * kTileEdge * kTileEdge;
+  const int32_t c0 = t.blk * E, nb = min(E, nc - c0);
   const T* L = a.Lx + a.Lxp[s];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  loadInverse<T, S>(a, s, t.blk, Wsh);
   for (int32_t k = 0; k < q; k++) {
     S* fvs = fv + k * fvN + a.Sp[s];
-    if (tid < nb) zb[tid] = fvs[c0 + tid];
+    if (tid < E) zb[tid] = tid < nb ? fvs[c0 + tid] : S(0);
     __syncthreads();
-    if (tid < nb) {
+    if (tid < E) {
       S sum = 0;
-      for (int kk = tid; kk < nb; kk++) sum += S(W[kk + tid * kTileEdge]) * zb[kk];
+#pragma unroll 16
+      for (int kk = 0; kk < E; kk++) sum += Wsh[kk + tid * P] * zb[kk];
       xb[tid] = sum;
     }
     __syncthreads();
