@@ -3,17 +3,30 @@
 
 namespace amp {
 
-// One wave per block. Lane i keeps row i of the block in registers (right-looking elimination: at step k lane k takes
-// the pivot, every lane scales its entry of column k, the column is broadcast through LDS and each lane updates its
-// row). The inverse W = L^-1 is built in the same steps by Gauss-Jordan elimination with lane c keeping column c of W:
-// W[k][c] /= l_kk, then W[i][c] -= l_ik W[k][c] for i > k, reading l_ik from the same broadcast column. All loops
-// over the 64 columns are unrolled, so every operand is a register or a broadcast LDS read.
+namespace {
+
+// Broadcast lane `l`'s value to the whole wave through a scalar register (l is a compile-time constant here).
+__device__ inline double laneValue(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float laneValue(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+}  // namespace
+
+// One wave per block, no barriers. Lane i keeps row i of the block in registers (right-looking elimination: at step k
+// every lane reads the pivot from lane k, lane k records it, every lane scales its entry of column k, and the column's
+// entries are broadcast lane by lane through scalar registers for the rank-1 update of each row). The inverse
+// W = L^-1 is built in the same steps by Gauss-Jordan elimination with lane c keeping column c of W:
+// W[k][c] /= l_kk, then W[i][c] -= l_ik W[k][c] for i > k, with the same broadcast column. All loops over the 64
+// columns are unrolled, so every operand is a register.
 template <typename T>
 __global__ __launch_bounds__(64) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
   constexpr int E = kTileEdge, P = kTileEdge + 1;
-  __shared__ T colL[E], colW[E];
   __shared__ T Ws[E * P];
-  __shared__ T s_inv, s_mult;
   const DiagTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x, nb = t.nb;
   T* F = a.Lx + t.f_off;
@@ -26,27 +39,22 @@ __global__ __launch_bounds__(64) void k_diag_block(const DiagTask* __restrict__ 
 #pragma unroll
   for (int k = 0; k < E; k++) {
     if (k < nb) {
-      if (lane == k) {
-        T inv, mult;
-        r[k] = pivotValue<T>(r[k], t.col0 + k, a, &inv, &mult);
-        s_inv = inv;
-        s_mult = mult;
-      }
-      __syncthreads();
-      const T inv = s_inv, mult = s_mult;
+      // every lane evaluates the pivot rule on the same value; only lane k records it (D, flags, failure)
+      T inv, mult;
+      const T dk = laneValue(r[k], k);
+      const T piv = pivotRule<T>(dk, t.col0 + k, a, &inv, &mult, lane == k);
+      if (lane == k) r[k] = piv;
       if (lane > k) r[k] *= inv;
-      colL[lane] = r[k];
-      colW[lane] = r[k] * mult;
-      __syncthreads();
       const T lik = r[k];
-#pragma unroll
-      for (int j = k + 1; j < E; j++)
-        if (lane >= j) r[j] -= lik * colW[j];
-      const T wk = a.ldlt ? w[k] : w[k] / colL[k];
+      const T lkk = laneValue(r[k], k);
+      const T wk = a.ldlt ? w[k] : w[k] / lkk;
       w[k] = wk;
 #pragma unroll
-      for (int i = k + 1; i < E; i++) w[i] -= colL[i] * wk;
-      __syncthreads();
+      for (int j = k + 1; j < E; j++) {
+        const T ljk = laneValue(lik, j);
+        if (lane >= j) r[j] -= lik * (ljk * mult);
+        w[j] -= ljk * wk;
+      }
     }
   }
 #pragma unroll

@@ -79,7 +79,7 @@ struct amp_num {
   amp_sym* sym = nullptr;
   int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evR0 = nullptr, evR1 = nullptr;
   amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv;
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
@@ -92,6 +92,8 @@ struct amp_num {
     for (hipEvent_t e : prof.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : solveProf.events) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
+    if (evR0) (void)hipEventDestroy(evR0);
+    if (evR1) (void)hipEventDestroy(evR1);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
   }

@@ -35,6 +35,36 @@ __device__ inline T pivotValue(T d, int32_t col, const FactorArgs<T>& a, T* inv,
 }
 
 
+// The same rule evaluated by every lane of a wave on a broadcast value; only the lane with `record` writes D, the
+// perturbation flag and the failure column (so the stores happen once).
+template <typename T>
+__device__ inline T pivotRule(T d, int32_t col, const FactorArgs<T>& a, T* inv, T* mult, bool record) {
+  if (!a.ldlt) {
+    if (record && !(d > T(0))) atomicMin(a.failCol, col);
+    const T sq = sqrt(d);
+    *inv = T(1) / sq;
+    *mult = T(1);
+    return sq;
+  }
+  const T tau = T(a.tau);
+  uint8_t perturbed = 0;
+  if (!(fabs(d) >= tau)) {
+    if (!isfinite(d)) {
+      if (record) atomicMin(a.failCol, col);
+    } else {
+      d = d < T(0) ? -tau : tau;
+      perturbed = 1;
+    }
+  }
+  if (record) {
+    a.pert[col] = perturbed;
+    a.D[col] = d;
+  }
+  *inv = T(1) / d;
+  *mult = d;
+  return T(1);
+}
+
 template <typename T>
 void launchDiagBlocks(const DiagTask* tasks, unsigned count, const FactorArgs<T>& a, hipStream_t st);
 

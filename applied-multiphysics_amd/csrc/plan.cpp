@@ -125,71 +125,102 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
       }
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }
-    // panel factorization in 64-column steps, all tiled fronts of the level in lockstep
+    // panel factorization, all tiled fronts of the level in lockstep: super-panels of kSuperPanel columns are first
+    // brought up to date left-looking (one product of depth = all earlier columns), then factored right-looking in
+    // 64-column steps whose trailing updates stay inside the super-panel
     int32_t maxNc = 0;
     for (int32_t s : big) maxNc = std::max(maxNc, frontCols(S, s));
-    for (int32_t k0 = 0; k0 < maxNc; k0 += kTileEdge) {
-      const int64_t dFirst = int64_t(P.diag.size());
-      for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
-        if (nc <= k0) continue;
-        const int32_t nb = std::min(kTileEdge, nc - k0);
-        P.diag.push_back(DiagTask{S.Lxp[s] + k0 + int64_t(k0) * m, P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge, m,
-                                  nb, S.superStart[s] + k0, 0});
-      }
-      push(kDiag, L, dFirst, int64_t(P.diag.size()) - dFirst);
-      // L21 rows of this panel: A21 L11^-T (Cholesky) or A21 L11^-T D^-1 (LDL^T), in place
-      const int64_t tFirst = int64_t(P.gemm.size());
-      for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
-        if (nc <= k0) continue;
-        const int32_t nb = std::min(kTileEdge, nc - k0);
-        const int64_t base = S.Lxp[s];
-        const int64_t woff = P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge;
-        for (int32_t r0 = k0 + nb; r0 < m; r0 += kTileEdge) {
-          GemmTask t{};
-          t.a_off = base + r0 + int64_t(k0) * m;
-          t.b_off = woff;
-          t.c_off = t.a_off;
-          t.lda = m;
-          t.ldb = kTileEdge;
-          t.ldc = m;
-          t.rows = std::min(kTileEdge, m - r0);
-          t.cols = nb;
-          t.k = nb;
-          t.d_off = f + k0;
-          t.a_buf = kBufLx;
-          t.b_buf = kBufW;
-          t.c_buf = kBufLx;
-          t.mode = 2;
-          P.gemm.push_back(t);
+    for (int32_t c0 = 0; c0 < maxNc; c0 += kSuperPanel) {
+      if (c0 > 0) {
+        const int64_t first = int64_t(P.gemm.size());
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
+          if (nc <= c0) continue;
+          const int32_t c1 = std::min(nc, c0 + kSuperPanel);
+          const int64_t base = S.Lxp[s];
+          for (int32_t j0 = c0; j0 < c1; j0 += kTileEdge)
+            for (int32_t i0 = j0; i0 < m; i0 += kTileEdge) {
+              GemmTask t{};
+              t.a_off = base + i0;
+              t.b_off = base + j0;
+              t.c_off = base + i0 + int64_t(j0) * m;
+              t.lda = t.ldb = t.ldc = m;
+              t.rows = std::min(kTileEdge, m - i0);
+              t.cols = std::min(kTileEdge, c1 - j0);
+              t.k = c0;
+              t.d_off = f;
+              t.a_buf = t.b_buf = t.c_buf = kBufLx;
+              t.mode = 0;
+              P.gemm.push_back(t);
+            }
         }
+        push(kGemm, L, first, int64_t(P.gemm.size()) - first);
       }
-      push(kGemm, L, tFirst, int64_t(P.gemm.size()) - tFirst);
-      // trailing update of the remaining panel columns of the L block
-      const int64_t uFirst = int64_t(P.gemm.size());
-      for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
-        if (nc <= k0) continue;
-        const int32_t nb = std::min(kTileEdge, nc - k0);
-        const int64_t base = S.Lxp[s];
-        for (int32_t j0 = k0 + nb; j0 < nc; j0 += kTileEdge)
-          for (int32_t i0 = j0; i0 < m; i0 += kTileEdge) {
+      for (int32_t k0 = c0; k0 < std::min(maxNc, c0 + kSuperPanel); k0 += kTileEdge) {
+        const int64_t dFirst = int64_t(P.diag.size());
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+          if (nc <= k0) continue;
+          const int32_t nb = std::min(kTileEdge, nc - k0);
+          P.diag.push_back(DiagTask{S.Lxp[s] + k0 + int64_t(k0) * m,
+                                    P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge, m, nb,
+                                    S.superStart[s] + k0, 0});
+        }
+        push(kDiag, L, dFirst, int64_t(P.diag.size()) - dFirst);
+        // L21 rows of this panel: A21 L11^-T (Cholesky) or A21 L11^-T D^-1 (LDL^T), in place
+        const int64_t tFirst = int64_t(P.gemm.size());
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
+          if (nc <= k0) continue;
+          const int32_t nb = std::min(kTileEdge, nc - k0);
+          const int64_t base = S.Lxp[s];
+          const int64_t woff = P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge;
+          for (int32_t r0 = k0 + nb; r0 < m; r0 += kTileEdge) {
             GemmTask t{};
-            t.a_off = base + i0 + int64_t(k0) * m;
-            t.b_off = base + j0 + int64_t(k0) * m;
-            t.c_off = base + i0 + int64_t(j0) * m;
-            t.lda = t.ldb = t.ldc = m;
-            t.rows = std::min(kTileEdge, m - i0);
-            t.cols = std::min(kTileEdge, nc - j0);
+            t.a_off = base + r0 + int64_t(k0) * m;
+            t.b_off = woff;
+            t.c_off = t.a_off;
+            t.lda = m;
+            t.ldb = kTileEdge;
+            t.ldc = m;
+            t.rows = std::min(kTileEdge, m - r0);
+            t.cols = nb;
             t.k = nb;
             t.d_off = f + k0;
-            t.a_buf = t.b_buf = t.c_buf = kBufLx;
-            t.mode = 0;
+            t.a_buf = kBufLx;
+            t.b_buf = kBufW;
+            t.c_buf = kBufLx;
+            t.mode = 2;
             P.gemm.push_back(t);
           }
+        }
+        push(kGemm, L, tFirst, int64_t(P.gemm.size()) - tFirst);
+        // trailing update of the remaining columns of this super-panel
+        const int64_t uFirst = int64_t(P.gemm.size());
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
+          if (nc <= k0) continue;
+          const int32_t nb = std::min(kTileEdge, nc - k0);
+          const int32_t c1 = std::min(nc, c0 + kSuperPanel);
+          const int64_t base = S.Lxp[s];
+          for (int32_t j0 = k0 + nb; j0 < c1; j0 += kTileEdge)
+            for (int32_t i0 = j0; i0 < m; i0 += kTileEdge) {
+              GemmTask t{};
+              t.a_off = base + i0 + int64_t(k0) * m;
+              t.b_off = base + j0 + int64_t(k0) * m;
+              t.c_off = base + i0 + int64_t(j0) * m;
+              t.lda = t.ldb = t.ldc = m;
+              t.rows = std::min(kTileEdge, m - i0);
+              t.cols = std::min(kTileEdge, c1 - j0);
+              t.k = nb;
+              t.d_off = f + k0;
+              t.a_buf = t.b_buf = t.c_buf = kBufLx;
+              t.mode = 0;
+              P.gemm.push_back(t);
+            }
+        }
+        push(kGemm, L, uFirst, int64_t(P.gemm.size()) - uFirst);
       }
-      push(kGemm, L, uFirst, int64_t(P.gemm.size()) - uFirst);
     }
     // Schur complement U = -L21 D L21^T into the update matrix (write only)
     {

@@ -15,6 +15,7 @@ namespace amp {
 
 constexpr int kTileEdge = 64;   // panel width / update-tile edge of tiled fronts; block size of their solves
 constexpr int kEaWidth = 16;    // parent columns per extend-add work item
+constexpr int kSuperPanel = 256;  // columns brought up to date together (left-looking) in a tiled front
 constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
 constexpr int kZeroChunk = 1 << 18;
 constexpr int kSolveRows = 128;     // rows per work item of the blocked forward solve

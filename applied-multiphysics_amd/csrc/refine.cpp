@@ -186,7 +186,11 @@ int refineDevice(amp_num* num, const double* d_values, const double* d_b, double
   if (o->method == AMP_REFINE_GMRES) {
     if ((rc = ensure(num->V, vb * (o->restart + 1))) || (rc = ensure(num->Z, vb * o->restart))) return rc;
   }
-  hipEvent_t e0 = num->ev0, e1 = num->ev1;
+  if (!num->evR0) {
+    HIP_TRY(hipEventCreate(&num->evR0));
+    HIP_TRY(hipEventCreate(&num->evR1));
+  }
+  hipEvent_t e0 = num->evR0, e1 = num->evR1;  // the inner solves use ev0 / ev1 themselves
   HIP_TRY(hipEventRecord(e0, num->stream));
   Ctx c{num, n, num->stream, h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_values, num->small.as<double>(),
         num->partial.as<double>()};

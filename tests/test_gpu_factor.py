@@ -42,6 +42,7 @@ CASES = {
     "grid27_10_zeros": lambda: (*grid27(10, 10, 10, zeroEvery=3)[:2], False),
     "grid27_12_sym": lambda: (*grid27(12, 12, 12, symmetric=True)[:2], True),
     "heat3d_14": lambda: (*W.heat3d(14)[:2], True),
+    "heat3d_20": lambda: (*W.heat3d(20)[:2], True),
     "hex_matrix": lambda: (*hex_matrix(), True),
 }
 
