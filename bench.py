@@ -49,14 +49,40 @@ def cpu_baseline(pat, values, target_flops):
     ok, fl, _ = O.chol_factor_subtree(S, values, root)
     dt = time.perf_counter() - t0
     m = int(S.Sp[root + 1] - S.Sp[root])
-    return {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+    model, ncpu = cpu_info()
+    return {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port", "cpu": model, "nproc": ncpu,
             "sample": f"oracle FP64 factor of the complete subtree of supernode {root} (root front m={m}, supernodes "
                       f"{first[root]}..{root}) of the same 61^3 elimination tree: {fl:.3e} flops in {dt:.1f} s; the "
                       f"full factor ({S.flops:.3e} flops) at that rate takes {S.flops / (fl / dt):.0f} s",
             "ok": bool(ok), "est_full_factor_s": S.flops / (fl / dt)}
 
 
-def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix):
+def gemm_traffic(path):
+    """HBM bytes per k_gemm_tile launch of the given path, from the committed counter summary (rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs of this benchmark, scaled by the same counters on a plain copy), or None."""
+    f = os.path.join(ROOT, "profiles", "gemm_tile_heat3d61_hbm_traffic.txt")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        k, _, v = line.partition(":")
+        if k.strip() == f"bytes_per_launch_{path}":
+            return float(v)
+    return None
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir"):
     import numpy as np
     import torch
 
@@ -72,7 +98,10 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix):
         if name == "fp64":
             num.solve(db, dx)
             return None
-        _, info = num.refine(dv, db, "gmres", maxSteps=200, restart=30, tol=tol, out=dx)
+        if method == "ir":
+            _, info = num.refine(dv, db, "ir", maxSteps=30, tol=tol, out=dx)
+        else:
+            _, info = num.refine(dv, db, "gmres", maxSteps=200, restart=30, tol=tol, out=dx)
         return info
 
     for i in range(warmup):
@@ -100,6 +129,7 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix):
            "gemm_launches_per_step": gemm_n // max(steps, 1),
            "kernel_ms_last_factor": {k: v[0] for k, v in kinds.items()}}
     if rinfo:
+        res["refine_method"] = method
         res["refine_iterations"] = [r["steps"] for r in rinfo]
         res["refine_ms"] = float(np.mean([r["ms"] for r in rinfo]))
     num.set_profiling(False)
@@ -115,6 +145,9 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-flops", type=float, default=6e10, help="work of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--refine", default="ir", choices=("ir", "gmres"), help="refinement of the mixed path")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run 3 plain device copies of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path is single-device by design",
@@ -131,6 +164,14 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     dev = torch.device("cuda:0")
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 copies of {args.copy_mb} MiB")
+        del src, dst
     t0 = time.perf_counter()
     pat, values, b = W.heat3d(args.ncell)
     log(f"workload heat3d ncell={args.ncell}: n={pat['n']} nnz={pat['nnz']} ({time.perf_counter() - t0:.1f} s)")
@@ -145,7 +186,7 @@ def main():
     paths = {}
     for name in args.paths.split(","):
         dx = torch.empty_like(db)
-        res, num = run_path(name, sym, dv, db, dx, args.steps, args.warmup, args.tol, name)
+        res, num = run_path(name, sym, dv, db, dx, args.steps, args.warmup, args.tol, name, args.refine)
         x = dx.cpu().numpy()
         res["relres"] = float(np.linalg.norm(A @ x - b) / np.linalg.norm(b))
         paths[name] = (res, x)
@@ -179,7 +220,7 @@ def main():
         "relres": r["relres"],
         "paths": {k: v[0] for k, v in paths.items()},
         "roofline": {"bound": "mfma", "kernel": "k_gemm_tile", "achieved": ach, "peak": PEAK[best],
-                     "unit": "TFLOP/s", "frac": ach / PEAK[best], "traffic": None,
+                     "unit": "TFLOP/s", "frac": ach / PEAK[best], "traffic": gemm_traffic(best),
                      "launches_per_step": r["gemm_launches_per_step"], "flops_per_step": sym.gemmFlops},
     }
     if args.cpu_flops > 0:
