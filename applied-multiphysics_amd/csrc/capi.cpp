@@ -173,6 +173,9 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
       auto& slot = num->kernelMs[names[num->prof.kinds[i]]];
       slot.first += k;
       slot.second += 1;
+      auto& lv = num->kernelMs[std::string(names[num->prof.kinds[i]]) + "@" + std::to_string(num->prof.levels[i])];
+      lv.first += k;
+      lv.second += 1;
     }
   }
   num->ok = failCol == INT32_MAX;
@@ -197,6 +200,8 @@ int ensureVecs(amp_num* num, int32_t nrhs) {
   if (num->y.bytes < std::max(n * es, size_t(h->S.n) * 8)) HIP_TRY(num->y.alloc(std::max(n * es, size_t(h->S.n) * 8)));
   const size_t fvn = size_t(std::max<int64_t>(h->SP.fvEntries, 1)) * nrhs * es;
   if (num->fv.bytes < fvn) HIP_TRY(num->fv.alloc(fvn));
+  const size_t pzn = size_t(std::max<int64_t>(h->SP.initSlots, 1)) * amp::kTileEdge * std::min(nrhs, amp::kFlowRhs) * es;
+  if (num->pz.bytes < pzn) HIP_TRY(num->pz.alloc(pzn));
   return AMP_OK;
 }
 
@@ -208,20 +213,25 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   if (prof) prof->used = 0;
   HIP_TRY(hipEventRecord(num->ev0, num->stream));
   if (num->prec == AMP_F64)
-    amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs, num->y.as<double>(),
-                                         num->fv.as<double>(), num->stream, prof);
+    HIP_TRY((amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs,
+                                                  num->y.as<double>(), num->fv.as<double>(), num->pz.as<double>(),
+                                                  num->fc, num->stream, prof)));
   else
-    amp::launchSolvePlan<float, float>(num->sym->dsp, solveArgs<float>(num), d_b, d_x, nrhs, num->y.as<float>(),
-                                       num->fv.as<float>(), num->stream, prof);
+    HIP_TRY((amp::launchSolvePlan<float, float>(num->sym->dsp, solveArgs<float>(num), d_b, d_x, nrhs,
+                                                num->y.as<float>(), num->fv.as<float>(), num->pz.as<float>(), num->fc,
+                                                num->stream, prof)));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(num->ev1, num->stream));
+  int32_t waitErr = 0;
+  HIP_TRY(hipMemcpyAsync(&waitErr, num->fc.err, 4, hipMemcpyDeviceToHost, num->stream));
   HIP_TRY(hipStreamSynchronize(num->stream));
+  if (waitErr) return fail(AMP_ERR_INTERNAL, "solve: a block dependency wait timed out");
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
   num->solveMs = ms;
   if (prof) {
-    static const char* names[] = {"solve_small_fwd", "solve_gather", "solve_fwd_block",
-                                  "solve_small_bwd", "solve_bwd_init", "solve_bwd_block"};
+    static const char* names[] = {"solve_small_fwd", "solve_gather", "solve_fwd_flow", "solve_small_bwd",
+                                  "solve_bwd_flow", "solve_bwd_gather", "solve_bwd_init"};
     for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
       it = it->first.rfind("solve_", 0) == 0 ? num->kernelMs.erase(it) : std::next(it);
     for (size_t i = 0; i < prof->used; i++) {
@@ -230,6 +240,9 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
       auto& slot = num->kernelMs[names[prof->kinds[i] - 16]];
       slot.first += k;
       slot.second += 1;
+      auto& lv = num->kernelMs[std::string(names[prof->kinds[i] - 16]) + "@" + std::to_string(prof->levels[i])];
+      lv.first += k;
+      lv.second += 1;
     }
     auto& total = num->kernelMs["solve_total"];
     total.first = ms;
@@ -412,6 +425,17 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   HIP_TRY(num->failCol.alloc(4));
   HIP_TRY(num->counts.alloc(8));
   HIP_TRY(num->values.alloc(size_t(std::max<int64_t>(h->S.nnzPattern, 1)) * 8));
+  {
+    const amp::SolvePlan& SP = h->SP;
+    const size_t bytes = size_t(2 * SP.flowFlags + 1) * 4;  // flags of both directions, error flag
+    HIP_TRY(num->flow.alloc(bytes));
+    HIP_TRY(hipMemset(num->flow.p, 0, bytes));
+    int32_t* w = num->flow.as<int32_t>();
+    num->fc.fwdFlag = w;
+    num->fc.bwdFlag = w + SP.flowFlags;
+    num->fc.err = w + 2 * SP.flowFlags;
+    num->fc.epoch = 0;
+  }
   rc = ensureVecs(num.get(), 1);
   if (rc) return rc;
   *out = num.release();
@@ -484,10 +508,12 @@ int amp_num_set_profiling(amp_num* num, int32_t enable) {
     const size_t nl = num->sym->dfp.launches.size();
     num->prof.events.resize(2 * nl + 2, nullptr);
     num->prof.kinds.resize(nl + 1, 0);
+    num->prof.levels.resize(nl + 1, 0);
     for (hipEvent_t& e : num->prof.events) HIP_TRY(hipEventCreate(&e));
     const size_t ns = num->sym->dsp.launches.size();
     num->solveProf.events.resize(2 * ns + 2, nullptr);
     num->solveProf.kinds.resize(ns + 1, 0);
+    num->solveProf.levels.resize(ns + 1, 0);
     for (hipEvent_t& e : num->solveProf.events) HIP_TRY(hipEventCreate(&e));
   }
   return AMP_OK;

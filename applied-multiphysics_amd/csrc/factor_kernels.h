@@ -20,10 +20,11 @@ struct DeviceFactorPlan {
   std::vector<Launch> launches;
 };
 
-// Optional per-launch timing: events[2 i], events[2 i + 1] bracket launch i; kinds[i] is its LaunchKind.
+// Optional per-launch timing: events[2 i], events[2 i + 1] bracket launch i; kinds[i] is its launch kind and
+// levels[i] the elimination-tree level it serves.
 struct ProfileSink {
   std::vector<hipEvent_t> events;
-  std::vector<int32_t> kinds;
+  std::vector<int32_t> kinds, levels;
   size_t used = 0;
 };
 
@@ -59,10 +60,21 @@ struct SolveArgs {
   int32_t ldlt;
 };
 
-// Solve q systems in place on the device: b (q vectors of n, original order) -> x. y (n q) and fv (fvEntries q) are
-// scratch of the working precision S; Lx / W / D are the factor's precision T.
+// Synchronisation state of the tiled-supernode solve launches, owned by one numeric handle (launches of one handle
+// run in stream order). Flags hold the pass number (`epoch`, raised once per pass of at most kFlowRhs right-hand
+// sides) of the last pass that completed the block, so they never need clearing. err is set when a wait gives up.
+struct FlowCtl {
+  int32_t* fwdFlag;
+  int32_t* bwdFlag;
+  int32_t* err;
+  int32_t epoch;
+};
+
+// Solve q systems in place on the device: b (q vectors of n, original order) -> x. y (n q), fv (fvEntries q) and pz
+// (initSlots x 64 x min(q, kFlowRhs)) are scratch of the working precision S; Lx / W / D are the factor's precision
+// T. Raises fc.epoch once per pass.
 template <typename T, typename S>
-void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q, S* y,
-                     S* fv, hipStream_t st, ProfileSink* prof);
+hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q,
+                           S* y, S* fv, S* pz, FlowCtl& fc, hipStream_t st, ProfileSink* prof);
 
 }  // namespace amp

@@ -253,6 +253,7 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
     if (prof && 2 * prof->used + 1 < prof->events.size()) {
       (void)hipEventRecord(prof->events[2 * prof->used], st);
       prof->kinds[prof->used] = L.kind;
+      prof->levels[prof->used] = L.level;
     }
     switch (L.kind) {
       case kZero:

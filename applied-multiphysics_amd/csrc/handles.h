@@ -80,9 +80,12 @@ struct amp_num {
   int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evR0 = nullptr, evR1 = nullptr;
-  amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv;
+  amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv, pz;
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
+  // tiled-supernode solve synchronisation: forward and backward block flags, the wait-failure flag
+  amp::DevBuf flow;
+  amp::FlowCtl fc{};
   bool ok = false;
   bool profiling = false;
   amp::ProfileSink prof, solveProf;

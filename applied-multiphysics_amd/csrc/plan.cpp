@@ -284,45 +284,54 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
   if (ns == 0) return P;
   std::vector<std::vector<int32_t>> byLevel(F.nlevels);
   for (int32_t s = 0; s < ns; s++) byLevel[F.level[s]].push_back(s);
+  std::vector<int32_t> flag0(ns, -1);
+  for (int32_t s = 0; s < ns; s++)
+    if (F.isBig[s]) {
+      flag0[s] = int32_t(P.flowFlags);
+      P.flowFlags += (frontCols(S, s) + kTileEdge - 1) / kTileEdge;
+    }
   auto push = [&P](SolveKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0) {
     if (count <= 0) return;
     P.launches.push_back(SolveLaunch{kind, level, first, count, maxM});
   };
+  auto nblocks = [&S](int32_t s) { return (frontCols(S, s) + kTileEdge - 1) / kTileEdge; };
   // forward, leaves first
   for (int32_t L = 0; L < F.nlevels; L++) {
     int64_t first = int64_t(P.fronts.size());
     int32_t maxM = 0;
-    for (int32_t s : byLevel[L])
-      if (!F.isBig[s]) {
-        P.fronts.push_back(s);
-        maxM = std::max(maxM, frontRows(S, s));
-      }
-    push(kSolveSmallFwd, L, first, int64_t(P.fronts.size()) - first, maxM);
     std::vector<int32_t> big;
-    for (int32_t s : byLevel[L])
-      if (F.isBig[s]) big.push_back(s);
+    for (int32_t s : byLevel[L]) {
+      if (F.isBig[s]) {
+        big.push_back(s);
+        continue;
+      }
+      P.fronts.push_back(s);
+      maxM = std::max(maxM, frontRows(S, s));
+    }
+    push(kSolveSmallFwd, L, first, int64_t(P.fronts.size()) - first, maxM);
     if (big.empty()) continue;
     first = int64_t(P.fronts.size());
     P.fronts.insert(P.fronts.end(), big.begin(), big.end());
     push(kSolveGather, L, first, int64_t(big.size()));
+    // every task depends only on tasks listed before it: own tiles by block index across the level's supernodes,
+    // then the below tiles (which consume every block of their supernode)
+    const int64_t bf = int64_t(P.blocks.size());
     int32_t maxBlk = 0;
-    for (int32_t s : big) maxBlk = std::max(maxBlk, (frontCols(S, s) + kTileEdge - 1) / kTileEdge);
-    for (int32_t b = 0; b < maxBlk; b++) {
-      const int64_t bf = int64_t(P.blocks.size());
+    for (int32_t s : big) maxBlk = std::max(maxBlk, nblocks(s));
+    for (int32_t t = 0; t < maxBlk; t++)
       for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
-        if (b * kTileEdge >= nc) continue;
-        const int32_t rs = std::min(nc, (b + 1) * kTileEdge);  // rows below the block
-        if (rs >= m) {
-          P.blocks.push_back(BlockTask{s, b, m, m, 1, 0});  // nothing below: still solves the block
-          continue;
-        }
-        for (int32_t r = rs; r < m; r += kSolveRows) P.blocks.push_back(BlockTask{s, b, r, std::min(m, r + kSolveRows), r == rs, 0});
+        const int32_t nc = frontCols(S, s);
+        if (t * kTileEdge < nc)
+          P.blocks.push_back(BlockTask{s, t, t * kTileEdge, std::min(nc, (t + 1) * kTileEdge), 1, flag0[s], 0, 0});
       }
-      push(kSolveFwdBlock, L, bf, int64_t(P.blocks.size()) - bf);
+    for (int32_t s : big) {
+      const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+      for (int32_t r = nc; r < m; r += kTileEdge)
+        P.blocks.push_back(BlockTask{s, nblocks(s), r, std::min(m, r + kTileEdge), 0, flag0[s], 0, 0});
     }
+    push(kSolveFwdFlow, L, bf, int64_t(P.blocks.size()) - bf);
   }
-  // backward, root first
+  // backward, root first; blocks of each supernode from the last one down
   for (int32_t L = F.nlevels - 1; L >= 0; L--) {
     std::vector<int32_t> big;
     int64_t first = int64_t(P.fronts.size());
@@ -337,30 +346,34 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     }
     push(kSolveSmallBwd, L, first, int64_t(P.fronts.size()) - first, maxM);
     if (big.empty()) continue;
-    // z_own = y_own - L21^T y[below], columns in chunks
+    first = int64_t(P.fronts.size());
+    P.fronts.insert(P.fronts.end(), big.begin(), big.end());
+    push(kSolveBwdGather, L, first, int64_t(big.size()));
+    // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk)
+    std::vector<int32_t> p0(ns, 0), np(ns, 0);
     int64_t bf = int64_t(P.blocks.size());
     for (int32_t s : big) {
-      const int32_t nc = frontCols(S, s);
-      for (int32_t c = 0; c < nc; c += kSolveInitCols)
-        P.blocks.push_back(BlockTask{s, -1, c, std::min(nc, c + kSolveInitCols), 0, 0});
+      const int32_t nc = frontCols(S, s), mu = frontRows(S, s) - nc;
+      np[s] = (mu + kInitRows - 1) / kInitRows;
+      p0[s] = int32_t(P.initSlots);
+      for (int32_t b = 0; b < nblocks(s); b++)
+        for (int32_t c = 0; c < np[s]; c++)
+          P.blocks.push_back(BlockTask{s, b, c * kInitRows, std::min(mu, (c + 1) * kInitRows), 0, flag0[s],
+                                       int32_t(P.initSlots++), 1});
     }
     push(kSolveBwdInit, L, bf, int64_t(P.blocks.size()) - bf);
+    bf = int64_t(P.blocks.size());
     int32_t maxBlk = 0;
-    for (int32_t s : big) maxBlk = std::max(maxBlk, (frontCols(S, s) + kTileEdge - 1) / kTileEdge);
-    for (int32_t b = maxBlk - 1; b >= 0; b--) {
-      bf = int64_t(P.blocks.size());
+    for (int32_t s : big) maxBlk = std::max(maxBlk, nblocks(s));
+    for (int32_t i = 0; i < maxBlk; i++)
       for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s);
-        if (b * kTileEdge >= nc) continue;
-        const int32_t c0 = b * kTileEdge;  // columns [0, c0) are updated by this block
-        if (c0 == 0) {
-          P.blocks.push_back(BlockTask{s, b, 0, 0, 1, 0});
-          continue;
-        }
-        for (int32_t c = 0; c < c0; c += kSolveCols) P.blocks.push_back(BlockTask{s, b, c, std::min(c0, c + kSolveCols), c == 0, 0});
+        const int32_t nc = frontCols(S, s), nb = nblocks(s);
+        if (i >= nb) continue;
+        const int32_t b = nb - 1 - i;
+        P.blocks.push_back(BlockTask{s, b, b * kTileEdge, std::min(nc, (b + 1) * kTileEdge), 1, flag0[s],
+                                     p0[s] + b * np[s], np[s]});
       }
-      push(kSolveBwdBlock, L, bf, int64_t(P.blocks.size()) - bf);
-    }
+    push(kSolveBwdFlow, L, bf, int64_t(P.blocks.size()) - bf);
   }
   return P;
 }

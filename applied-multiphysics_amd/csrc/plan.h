@@ -18,9 +18,6 @@ constexpr int kEaWidth = 16;    // parent columns per extend-add work item
 constexpr int kSuperPanel = 256;  // columns brought up to date together (left-looking) in a tiled front
 constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
 constexpr int kZeroChunk = 1 << 18;
-constexpr int kSolveRows = 128;     // rows per work item of the blocked forward solve
-constexpr int kSolveCols = 64;      // columns per work item of the blocked backward solve
-constexpr int kSolveInitCols = 16;  // columns per work item of the backward initialisation (z = y - L21^T y_below)
 
 enum Buf : int8_t { kBufLx = 0, kBufU = 1, kBufW = 2 };
 
@@ -96,32 +93,46 @@ FactorPlan buildFactorPlan(const Symbolic& S);
 // Every supernode s owns a front vector v_s of length m (at Sp[s] in a buffer of Sp[nsuper] entries per right-hand
 // side). Forward: v_s = [y_own; 0] + sum of the children's below parts (through rel); x_own = L11^-1 v_own is written
 // to y; v_below -= L21 x_own and stays in v_s for the parent. Backward: z = y_own - L21^T y[below];
-// x_own = L11^-T z. Supernodes with m <= kSmallMax are solved whole by one workgroup; tiled supernodes go in 64-column
-// blocks through their inverted diagonal blocks, one launch per block step over all tiled supernodes of the level.
+// x_own = L11^-T z. Supernodes with m <= kSmallMax are solved whole by one workgroup. Tiled supernodes are solved by
+// one dependency-driven launch per level and direction: a task per 64-row tile (forward) or 64-column block
+// (backward) waits on per-block completion flags of the blocks it consumes, so the block chain of a supernode runs
+// inside one launch.
+constexpr int kFlowRhs = 8;       // right-hand sides per pass of the tiled-supernode solve kernels
+constexpr int kInitRows = 256;    // below rows per task of the backward z = -L21^T y[below] partial products
+
+// One task of a tiled-supernode solve launch. Forward: rows [r0, r1) of supernode s after consuming the solutions of
+// blocks 0 .. blk - 1; an own-column tile (writer = 1) then solves block blk itself and raises its flag. Backward:
+// column block blk (columns [r0, r1)), consuming blocks blk + 1 .. nb - 1 and the np partial products at slots
+// p0 .. p0 + np - 1. Backward initialisation: partial product of column block blk over below rows [r0, r1) into slot
+// p0. `flag0` is the supernode's first flag.
 struct BlockTask {
-  int32_t s, blk;        // supernode, 64-column block index (-1: the backward initialisation)
-  int32_t r0, r1;        // forward: rows [r0, r1) of the front updated; backward: columns [r0, r1) updated
-  int32_t writer, pad;   // 1: this item also stores the block's solution into y
+  int32_t s, blk;
+  int32_t r0, r1;
+  int32_t writer, flag0;
+  int32_t p0, np;
 };
 
 enum SolveKind : int32_t {
   kSolveSmallFwd = 0,
   kSolveGather = 1,
-  kSolveFwdBlock = 2,
+  kSolveFwdFlow = 2,
   kSolveSmallBwd = 3,
-  kSolveBwdInit = 4,
-  kSolveBwdBlock = 5,
+  kSolveBwdFlow = 4,
+  kSolveBwdGather = 5,
+  kSolveBwdInit = 6,
 };
 
 struct SolveLaunch {
   SolveKind kind;
   int32_t level;
-  int64_t first, count;  // small / gather: range in `fronts`; block kinds: range in `blocks`; init: in `blocks`
+  int64_t first, count;  // small / gather: range in `fronts`; flow kinds: range in `blocks`
   int32_t maxM;
 };
 
 struct SolvePlan {
   int64_t fvEntries = 0;  // Sp[nsuper]
+  int64_t flowFlags = 0;  // 64-column blocks of all tiled supernodes (one flag each per direction)
+  int64_t initSlots = 0;  // 64-entry partial products of the backward initialisation
   std::vector<int32_t> fronts;
   std::vector<BlockTask> blocks;
   std::vector<SolveLaunch> launches;

@@ -4,19 +4,12 @@
 // for LDL^T): per supernode an in-block triangular solve and the product with the below block; the scatter into
 // ancestor rows is carried by each supernode's front vector and gathered by the parent through the relative map, so
 // no two workgroups ever write the same entry (deterministic, no atomics).
+#include <map>
+#include <mutex>
+
 #include "factor_kernels.h"
 
 namespace amp {
-
-namespace {
-
-template <typename S>
-__device__ inline S waveSum(S v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-}  // namespace
 
 template <typename S>
 __global__ void k_perm_in(const double* __restrict__ b, S* __restrict__ y, const int32_t* __restrict__ perm, int32_t n,
@@ -99,55 +92,184 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
   for (int32_t k = 0; k < q; k++) gatherFront<T, S>(a, s, y + int64_t(k) * a.n, fv + k * fvN, fv + k * fvN + a.Sp[s], nc, m);
 }
 
-// The inverted 64 x 64 diagonal block of block `blk` of supernode s into LDS (row i, column j at Wsh[i + j * 65]).
-template <typename T, typename S>
-__device__ inline void loadInverse(const SolveArgs<T>& a, int32_t s, int32_t blk, S* Wsh) {
-  const T* W = a.W + a.Woff[s] + int64_t(blk) * kTileEdge * kTileEdge;
-  for (int idx = threadIdx.x; idx < kTileEdge * kTileEdge; idx += blockDim.x)
-    Wsh[(idx & 63) + (idx >> 6) * (kTileEdge + 1)] = S(W[idx]);
+namespace {
+
+constexpr uint32_t kFlowSpinLimit = 1u << 22;  // about half a second of polling before a wait gives up
+constexpr int kFlowRun = 8;                    // blocks consumed per synchronisation of a flow task
+
+// Waits (thread 0 polls, then acquires at device scope) until block `k` is complete and returns the end of the run of
+// complete blocks that starts at k and stops at `lim` (exclusive; lim < k walks downwards). A block is published only
+// after every block it consumes, so when block lim -1 (lim + 1 downwards) is complete, so is the whole run; the two
+// flags are read together. A wait that exceeds the spin limit, or that finds another wait already failed, sets *err
+// and proceeds, so the launch always completes; the host reports the error.
+__device__ inline int32_t flowReady(const int32_t* flags, int32_t k, int32_t lim, int32_t want, int32_t* err) {
+  __shared__ int32_t s_r;
+  if (threadIdx.x == 0) {
+    const int32_t step = lim > k ? 1 : -1, last = lim - step;
+    uint32_t spins = 0;
+    int32_t fk, fl;
+    // relaxed polling (no cache invalidation per poll), one acquire fence once the flag is seen
+    for (;;) {
+      fk = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      fl = __hip_atomic_load(flags + last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (fk == want) break;
+      if (++spins >= kFlowSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        atomicOr(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_r = fl == want ? lim : k + step;
+  }
+  __syncthreads();
+  return s_r;
 }
 
-// Forward step of one 64-column block of a tiled supernode: x = Winv v_blk (every work item recomputes it from the
-// inverse staged in LDS), then v[r] -= L[r, blk] x for the item's rows, four threads per row over 16 columns each;
-// the writer item stores x into y.
-template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_solve_fwd_block(const BlockTask* __restrict__ tasks, SolveArgs<T> a, S* y,
-                                                         S* fv, int32_t q, int64_t fvN) {
-  constexpr int E = kTileEdge, P = kTileEdge + 1;
-  __shared__ S Wsh[E * P];
-  __shared__ S vb[E], xb[E];
-  __shared__ S part[4][kSolveRows];
-  const BlockTask t = tasks[blockIdx.x];
-  const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t c0 = t.blk * E, nb = min(E, nc - c0);
-  const T* L = a.Lx + a.Lxp[s];
+// Every thread makes its stores visible device-wide, then thread 0 raises the flag (release, device scope).
+__device__ inline void flowPublish(int32_t* flag, int32_t val) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace
+
+// Forward solve of the tiled supernodes of one level. A task owns up to 64 rows r of supernode s; thread (row, g)
+// accumulates sum_k L[r, block k] x_k over the 16 columns 16 g .. 16 g + 15 of each consumed block. L values are
+// loaded one block ahead; the consumed x blocks are staged in LDS a run of complete blocks at a time. Below tiles then
+// store v_below - sum into the front vector; own-column tiles form v_blk - sum, apply the block's inverted diagonal
+// block (x = W v) and publish x into y. Q is the number of right-hand sides the kernel is built for (q <= Q).
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, S* fv, int32_t q,
+                                        int64_t fvN, const FlowCtl& c, S (&part)[4][Q][kTileEdge],
+                                        S (&xs)[Q][kFlowRun * kTileEdge]) {
+  constexpr int E = kTileEdge;
   const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
-  loadInverse<T, S>(a, s, t.blk, Wsh);
-  for (int32_t k = 0; k < q; k++) {
-    S* fvs = fv + k * fvN + a.Sp[s];
-    if (tid < E) vb[tid] = tid < nb ? fvs[c0 + tid] : S(0);
-    __syncthreads();
-    if (tid < E) {
-      S sum = 0;
-#pragma unroll 16
-      for (int kk = 0; kk < E; kk++) sum += Wsh[tid + kk * P] * vb[kk];
-      xb[tid] = sum;
-    }
-    __syncthreads();
-    for (int32_t r0 = t.r0; r0 < t.r1; r0 += 64) {
-      const int32_t r = r0 + rr;
-      S sum = 0;
-      if (r < t.r1) {
-        const T* Lr = L + r + int64_t(c0 + 16 * g) * m;
+  const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
+  const T* L = a.Lx + a.Lxp[s];
+  const int32_t r = t.r0 + rr;
+  const bool live = r < t.r1;
+  const T* Lr = L + (live ? r : t.r0) + int64_t(16 * g) * m;
+  const int32_t nb = (nc + E - 1) / E;
+  // a below tile of a single-block supernode forms x_0 = W_0 v_own itself instead of waiting for it
+  const bool local0 = !t.writer && nb == 1;
+  T wv[16];
+  if (t.writer || local0) {
+    const T* Wt = a.W + a.Woff[s] + int64_t(t.writer ? t.blk : 0) * E * E;
 #pragma unroll
-        for (int j = 0; j < 16; j++) sum += S(Lr[int64_t(j) * m]) * xb[16 * g + j];
+    for (int j = 0; j < 16; j++) wv[j] = Wt[rr + (16 * g + j) * E];
+  }
+  S acc[Q];
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) acc[jj] = S(0);
+  const int32_t steps = t.blk;
+  T cur[16], nxt[16];
+  auto loadL = [&](T* dst, int32_t k) {
+    const int32_t cb = k * E + 16 * g;
+#pragma unroll
+    for (int j = 0; j < 16; j++) dst[j] = (live && cb + j < nc) ? Lr[int64_t(k * E + j) * m] : T(0);
+  };
+  if (steps > 0) loadL(cur, 0);
+  if (local0) {
+    if (tid < E)
+      for (int jj = 0; jj < q; jj++) xs[jj][E + rr] = rr < nc ? fv[jj * fvN + a.Sp[s] + rr] : S(0);
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < Q; jj++) {
+      S sum = S(0);
+      if (jj < q) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) sum += S(wv[j]) * xs[jj][E + 16 * g + j];
       }
-      part[g][rr] = sum;
-      __syncthreads();
-      if (g == 0 && r < t.r1) fvs[r] -= (part[0][rr] + part[1][rr]) + (part[2][rr] + part[3][rr]);
-      __syncthreads();
+      part[g][jj][rr] = sum;
     }
-    if (t.writer && tid < nb) y[int64_t(k) * a.n + f + c0 + tid] = xb[tid];
+    __syncthreads();
+    if (tid < E)
+      for (int jj = 0; jj < q; jj++) xs[jj][rr] = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < Q; jj++) {
+      if (jj < q) {
+        S sum = S(0);
+#pragma unroll
+        for (int j = 0; j < 16; j++) sum += S(cur[j]) * xs[jj][16 * g + j];
+        acc[jj] += sum;
+      }
+    }
+    __syncthreads();
+  }
+  for (int32_t k0 = 0; k0 < (local0 ? 0 : steps);) {
+    const int32_t k1 = flowReady(c.fwdFlag + t.flag0, k0, min(steps, k0 + kFlowRun), c.epoch, c.err);
+    for (int idx = tid; idx < (k1 - k0) * E; idx += 256) {
+      const int32_t col = k0 * E + idx;
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++)
+        if (jj < q) xs[jj][idx] = col < nc ? y[int64_t(jj) * a.n + f + col] : S(0);
+    }
+    __syncthreads();
+    for (int32_t k = k0; k < k1; k++) {
+      if (k + 1 < steps) loadL(nxt, k + 1);
+      const int xo = (k - k0) * E + 16 * g;
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        if (jj < q) {
+          S sum = S(0);
+#pragma unroll
+          for (int j = 0; j < 16; j++) sum += S(cur[j]) * xs[jj][xo + j];
+          acc[jj] += sum;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 16; j++) cur[j] = nxt[j];
+    }
+    __syncthreads();
+    k0 = k1;
+  }
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) part[g][jj][rr] = acc[jj];
+  __syncthreads();
+  S* vs = &xs[0][0];  // reused: v of the own block, Q x 64
+  if (tid < E) {
+    for (int jj = 0; jj < q; jj++) {
+      S v = S(0);
+      if (live) {
+        S* fvk = fv + jj * fvN + a.Sp[s];
+        v = fvk[r] - ((part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]));
+        if (!t.writer) fvk[r] = v;
+      }
+      vs[jj * E + rr] = v;
+    }
+  }
+  if (!t.writer) return;
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) {
+    S sum = S(0);
+    if (jj < q) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) sum += S(wv[j]) * vs[jj * E + 16 * g + j];
+    }
+    part[g][jj][rr] = sum;
+  }
+  __syncthreads();
+  if (tid < E && live)
+    for (int jj = 0; jj < q; jj++)
+      y[int64_t(jj) * a.n + f + r] = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
+  if (nb > 1) flowPublish(c.fwdFlag + t.flag0 + t.blk, c.epoch);  // single blocks have no waiters
+}
+
+// Persistent form: the grid is at most the number of co-resident workgroups (cooperative launch), and workgroup b
+// runs tasks b, b + grid, ... in order. Every task waits only on tasks listed before it, so the lowest unfinished task
+// never waits on an unfinished one and is being run: the launch always drains.
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_fwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
+                                                        SolveArgs<T> a, S* y, S* fv, int32_t q, int64_t fvN,
+                                                        FlowCtl c) {
+  __shared__ S part[4][Q][kTileEdge];
+  __shared__ S xs[Q][kFlowRun * kTileEdge];
+  for (int32_t i = blockIdx.x; i < ntasks; i += gridDim.x) {
+    fwdTask<T, S, Q>(tasks[i], a, y, fv, q, fvN, c, part, xs);
     __syncthreads();
   }
 }
@@ -187,90 +309,198 @@ __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restri
   }
 }
 
-// z_i = y_own[i] - L21[:, i] . y[below] for the item's (at most 16) columns: y[below] is staged through LDS in chunks,
-// each wave owns four columns and its lanes run over the rows.
+// y[below] of each tiled supernode of a level into the below part of its front vector (contiguous for the backward
+// flow).
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a, S* y,
-                                                        S* fv, int32_t q, int64_t fvN) {
-  constexpr int CH = 2048;
-  __shared__ S yb[CH];
-  const BlockTask t = tasks[blockIdx.x];
-  const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
-  const T* L = a.Lx + a.Lxp[s];
+__global__ __launch_bounds__(256) void k_solve_bwd_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a,
+                                                          const S* y, S* fv, int32_t q, int64_t fvN) {
+  const int32_t s = fronts[blockIdx.x];
+  const int32_t nc = a.superStart[s + 1] - a.superStart[s], mu = a.Sp[s + 1] - a.Sp[s] - nc;
   const int32_t* rows = a.Si + a.Sp[s] + nc;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int32_t i0 = t.r0 + 4 * wave;
-  for (int32_t k = 0; k < q; k++) {
-    const S* yk = y + int64_t(k) * a.n;
-    S acc[4] = {0, 0, 0, 0};
-    for (int32_t base = 0; base < mu; base += CH) {
-      const int32_t len = min(CH, mu - base);
-      __syncthreads();
-      for (int r = threadIdx.x; r < len; r += blockDim.x) yb[r] = yk[rows[base + r]];
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        if (i0 + c >= t.r1) continue;
-        const T* Li = L + nc + base + int64_t(i0 + c) * m;
-        S sum = 0;
-        for (int r = lane; r < len; r += 64) sum += S(Li[r]) * yb[r];
-        acc[c] += sum;
-      }
-    }
-    S* fvs = fv + k * fvN + a.Sp[s];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const S v = waveSum(acc[c]);
-      if (lane == 0 && i0 + c < t.r1) fvs[i0 + c] = yk[f + i0 + c] - v;
-    }
+  for (int32_t jj = 0; jj < q; jj++) {
+    S* dst = fv + jj * fvN + a.Sp[s] + nc;
+    const S* yk = y + int64_t(jj) * a.n;
+#pragma unroll 4
+    for (int32_t i = threadIdx.x; i < mu; i += blockDim.x) dst[i] = yk[rows[i]];
   }
 }
 
-// Backward step of one block: x = Winv^T z_blk (every item recomputes it from the inverse staged in LDS), then
-// z_i -= L[blk, i] . x for the item's columns i (one wave per column); the writer item stores x into y.
-template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_solve_bwd_block(const BlockTask* __restrict__ tasks, SolveArgs<T> a, S* y,
-                                                         S* fv, int32_t q, int64_t fvN) {
-  constexpr int E = kTileEdge, P = kTileEdge + 1;
-  __shared__ S Wsh[E * P];
-  __shared__ S zb[E], xb[E];
+// Partial products of the backward initialisation: for the task's column block and below rows [r0, r1),
+// pz[slot][jj][c] = sum_r L[nc + r, c] y[below r] (y[below] gathered into the front vector). Wave w handles columns
+// 16 w .. 16 w + 15 (lane & 15), lane >> 4 selects every fourth row, so each load instruction reads four consecutive
+// rows of 16 columns.
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a,
+                                                        const S* fv, S* pz, int32_t q, int64_t fvN) {
   const BlockTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cj = 16 * w + (lane & 15), rs = lane >> 4;
+  const int32_t s = t.s, nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s];
+  const int32_t col = t.blk * kTileEdge + cj;
+  const bool live = col < nc;
+  S acc[Q];
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) acc[jj] = S(0);
+  if (live) {
+    const T* Lb = a.Lx + a.Lxp[s] + int64_t(col) * m + nc;
+    const S* yb = fv + a.Sp[s] + nc;
+#pragma unroll 8
+    for (int32_t i = t.r0 + rs; i < t.r1; i += 4) {
+      const S lv = S(Lb[i]);
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++)
+        if (jj < q) acc[jj] += lv * yb[jj * fvN + i];
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) {
+    acc[jj] += __shfl_xor(acc[jj], 16, 64);
+    acc[jj] += __shfl_xor(acc[jj], 32, 64);
+    if (jj < q && rs == 0) pz[(int64_t(t.p0) * q + jj) * kTileEdge + cj] = acc[jj];
+  }
+}
+
+// Backward solve of the tiled supernodes of one level. A task owns column block blk of supernode s; wave w handles
+// its columns 16 w .. 16 w + 15 (lane & 15) and lane >> 4 selects every fourth row, so each load instruction reads
+// four consecutive rows of 16 columns. z starts from y_own minus the block's partial products of L21^T y[below]
+// (summed in slot order before any wait); then z -= L[block k, blk]^T x_k for the blocks k > blk, a run of complete blocks at a time
+// (x staged in LDS, L loaded one block ahead), and x = W^T z is published.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, const S* pz, int32_t q,
+                                        const FlowCtl& c, S (&xs)[Q][kFlowRun * kTileEdge], S (&zs)[Q][kTileEdge]) {
+  constexpr int E = kTileEdge;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cj = 16 * w + (lane & 15), rs = lane >> 4;
   const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t c0 = t.blk * E, nb = min(E, nc - c0);
-  const T* L = a.Lx + a.Lxp[s];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  loadInverse<T, S>(a, s, t.blk, Wsh);
-  for (int32_t k = 0; k < q; k++) {
-    S* fvs = fv + k * fvN + a.Sp[s];
-    if (tid < E) zb[tid] = tid < nb ? fvs[c0 + tid] : S(0);
-    __syncthreads();
-    if (tid < E) {
-      S sum = 0;
-#pragma unroll 16
-      for (int kk = 0; kk < E; kk++) sum += Wsh[kk + tid * P] * zb[kk];
-      xb[tid] = sum;
+  const int32_t nb = (nc + E - 1) / E, c0 = t.blk * E, col = c0 + cj;
+  const bool live = col < nc;
+  const T* Lc = a.Lx + a.Lxp[s] + int64_t(live ? col : c0) * m;
+  T wv[16];
+  {
+    const T* Wt = a.W + a.Woff[s] + int64_t(t.blk) * E * E;
+#pragma unroll
+    for (int i = 0; i < 16; i++) wv[i] = Wt[(rs + 4 * i) + cj * E];
+  }
+  // the partial products of L21^T y[below] (slot order), summed before any wait
+  S acc[Q];
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) {
+    S below = S(0);
+    if (jj < q && rs == 0)
+      for (int32_t p = 0; p < t.np; p++) below += pz[(int64_t(t.p0 + p) * q + jj) * E + cj];
+    acc[jj] = below;
+  }
+  T cur[16], nxt[16];
+  auto loadL = [&](T* dst, int32_t k) {
+    const int32_t rb = k * E + rs;
+#pragma unroll
+    for (int i = 0; i < 16; i++) dst[i] = (live && rb + 4 * i < nc) ? Lc[rb + 4 * i] : T(0);
+  };
+  const int32_t kEnd = t.blk;  // blocks nb - 1 down to blk + 1
+  if (nb - 1 > kEnd) loadL(cur, nb - 1);
+  for (int32_t k0 = nb - 1; k0 > kEnd;) {
+    const int32_t k1 = flowReady(c.bwdFlag + t.flag0, k0, max(kEnd, k0 - kFlowRun), c.epoch, c.err);
+    // blocks k0 down to k1 + 1 are complete; stage their x (block k at slot k0 - k)
+    for (int idx = tid; idx < (k0 - k1) * E; idx += 256) {
+      const int32_t row = (k0 - idx / E) * E + (idx % E);
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++)
+        if (jj < q) xs[jj][idx] = row < nc ? y[int64_t(jj) * a.n + f + row] : S(0);
     }
     __syncthreads();
-    for (int32_t i = t.r0 + wave; i < t.r1; i += 4) {
-      S v = lane < nb ? S(L[(c0 + lane) + int64_t(i) * m]) * xb[lane] : S(0);
-      v = waveSum(v);
-      if (lane == 0) fvs[i] -= v;
+    for (int32_t k = k0; k > k1; k--) {
+      if (k - 1 > kEnd) loadL(nxt, k - 1);
+      const int xo = (k0 - k) * E + rs;
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        if (jj < q) {
+          S sum = S(0);
+#pragma unroll
+          for (int i = 0; i < 16; i++) sum += S(cur[i]) * xs[jj][xo + 4 * i];
+          acc[jj] += sum;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) cur[i] = nxt[i];
     }
-    if (t.writer && tid < nb) y[int64_t(k) * a.n + f + c0 + tid] = xb[tid];
+    __syncthreads();
+    k0 = k1;
+  }
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) {
+    acc[jj] += __shfl_xor(acc[jj], 16, 64);
+    acc[jj] += __shfl_xor(acc[jj], 32, 64);
+  }
+  if (rs == 0)
+    for (int jj = 0; jj < q; jj++) zs[jj][cj] = live ? y[int64_t(jj) * a.n + f + col] - acc[jj] : S(0);
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < Q; jj++) {
+    S sum = S(0);
+    if (jj < q) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) sum += S(wv[i]) * zs[jj][rs + 4 * i];
+    }
+    sum += __shfl_xor(sum, 16, 64);
+    sum += __shfl_xor(sum, 32, 64);
+    if (jj < q && rs == 0 && live) y[int64_t(jj) * a.n + f + col] = sum;
+  }
+  if (t.blk > 0) flowPublish(c.bwdFlag + t.flag0 + t.blk, c.epoch);  // block 0 has no waiters
+}
+
+// Persistent form, as k_solve_fwd_flow.
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_bwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
+                                                        SolveArgs<T> a, S* y, const S* pz, int32_t q, FlowCtl c) {
+  __shared__ S xs[Q][kFlowRun * kTileEdge];
+  __shared__ S zs[Q][kTileEdge];
+  for (int32_t i = blockIdx.x; i < ntasks; i += gridDim.x) {
+    bwdTask<T, S, Q>(tasks[i], a, y, pz, q, c, xs, zs);
     __syncthreads();
   }
 }
 
+namespace {
+
+// Co-resident workgroups of 256 threads for `kernel` on device `dev` (cached per kernel and device).
+int residentGroups(const void* kernel, int dev) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({kernel, dev});
+  if (it != cache.end()) return it->second;
+  int cus = 0, perCu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0) != hipSuccess)
+    return 0;
+  return cache[{kernel, dev}] = cus * perCu;
+}
+
+// Cooperative launch of a persistent flow kernel over `ntasks` tasks with at most the co-resident number of
+// workgroups (the runtime refuses a cooperative grid that cannot be co-resident).
+template <typename... Args>
+hipError_t launchFlow(const void* kernel, int64_t ntasks, hipStream_t st, const BlockTask* tasks, Args... args) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const int cap = residentGroups(kernel, dev);
+  if (cap <= 0) return hipErrorInvalidConfiguration;
+  int32_t n = int32_t(ntasks);
+  const unsigned grid = unsigned(std::min<int64_t>(ntasks, cap));
+  void* ptrs[] = {(void*)&tasks, (void*)&n, (void*)&args...};
+  return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(256), ptrs, 0, st);
+}
+
 template <typename T, typename S>
-void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q, S* y,
-                     S* fv, hipStream_t st, ProfileSink* prof) {
+hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q, S* y, S* fv,
+               S* pz, const FlowCtl& fc, hipStream_t st, ProfileSink* prof) {
   const int32_t n = a.n;
-  if (n == 0 || q == 0) return;
   const unsigned g = unsigned(std::min<int64_t>((int64_t(n) * q + 255) / 256, 65536));
   hipLaunchKernelGGL(k_perm_in<S>, dim3(g), dim3(256), 0, st, b, y, a.perm, n, q);
   bool scaled = false;
+  hipError_t err = hipSuccess;
   for (const SolveLaunch& L : dp.launches) {
-    if (a.ldlt && !scaled && (L.kind == kSolveSmallBwd || L.kind == kSolveBwdInit)) {
+    if (a.ldlt && !scaled && L.kind >= kSolveSmallBwd) {
       hipLaunchKernelGGL((k_diag_scale<T, S>), dim3(g), dim3(256), 0, st, y, a.D, n, q);
       scaled = true;
     }
@@ -279,6 +509,7 @@ void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
     if (rec) {
       (void)hipEventRecord(prof->events[2 * prof->used], st);
       prof->kinds[prof->used] = 16 + L.kind;
+      prof->levels[prof->used] = L.level;
     }
     switch (L.kind) {
       case kSolveSmallFwd:
@@ -289,37 +520,67 @@ void launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
         hipLaunchKernelGGL((k_solve_gather<T, S>), dim3(grid), dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q,
                            dp.fvEntries);
         break;
-      case kSolveFwdBlock:
-        hipLaunchKernelGGL((k_solve_fwd_block<T, S>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a, y, fv, q,
-                           dp.fvEntries);
+      case kSolveFwdFlow:
+        err = q == 1 ? launchFlow((const void*)k_solve_fwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, a, y,
+                                  fv, q, dp.fvEntries, fc)
+                     : launchFlow((const void*)k_solve_fwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, a,
+                                  y, fv, q, dp.fvEntries, fc);
         break;
       case kSolveSmallBwd:
         hipLaunchKernelGGL((k_solve_small_bwd<T, S>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
                            dp.fronts + L.first, a, y, q);
         break;
-      case kSolveBwdInit:
-        hipLaunchKernelGGL((k_solve_bwd_init<T, S>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a, y, fv, q,
-                           dp.fvEntries);
+      case kSolveBwdGather:
+        hipLaunchKernelGGL((k_solve_bwd_gather<T, S>), dim3(grid), dim3(256), 0, st, dp.fronts + L.first, a, y, fv,
+                           q, dp.fvEntries);
         break;
-      case kSolveBwdBlock:
-        hipLaunchKernelGGL((k_solve_bwd_block<T, S>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a, y, fv, q,
-                           dp.fvEntries);
+      case kSolveBwdInit:
+        if (q == 1)
+          hipLaunchKernelGGL((k_solve_bwd_init<T, S, 1>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a, fv,
+                             pz, q, dp.fvEntries);
+        else
+          hipLaunchKernelGGL((k_solve_bwd_init<T, S, kFlowRhs>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a,
+                             fv, pz, q, dp.fvEntries);
+        break;
+      case kSolveBwdFlow:
+        err = q == 1 ? launchFlow((const void*)k_solve_bwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, a, y, pz,
+                                  q, fc)
+                     : launchFlow((const void*)k_solve_bwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, a,
+                                  y, pz, q, fc);
         break;
     }
     if (rec) {
       (void)hipEventRecord(prof->events[2 * prof->used + 1], st);
       prof->used++;
     }
+    if (err != hipSuccess) return err;
   }
   if (a.ldlt && !scaled) hipLaunchKernelGGL((k_diag_scale<T, S>), dim3(g), dim3(256), 0, st, y, a.D, n, q);
   hipLaunchKernelGGL(k_perm_out<S>, dim3(g), dim3(256), 0, st, y, x, a.perm, n, q);
+  return hipSuccess;
 }
 
-template void launchSolvePlan<double, double>(const DeviceSolvePlan&, const SolveArgs<double>&, const double*, double*,
-                                              int32_t, double*, double*, hipStream_t, ProfileSink*);
-template void launchSolvePlan<float, float>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                            int32_t, float*, float*, hipStream_t, ProfileSink*);
-template void launchSolvePlan<float, double>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                             int32_t, double*, double*, hipStream_t, ProfileSink*);
+}  // namespace
+
+template <typename T, typename S>
+hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q,
+                           S* y, S* fv, S* pz, FlowCtl& fc, hipStream_t st, ProfileSink* prof) {
+  const int32_t n = a.n;
+  if (n == 0 || q == 0) return hipSuccess;
+  for (int32_t k0 = 0; k0 < q; k0 += kFlowRhs) {
+    const int32_t qc = std::min(kFlowRhs, q - k0);
+    fc.epoch = fc.epoch == INT32_MAX ? 1 : fc.epoch + 1;
+    const hipError_t e = solvePass<T, S>(dp, a, b + int64_t(k0) * n, x + int64_t(k0) * n, qc, y, fv, pz, fc, st, prof);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template hipError_t launchSolvePlan<double, double>(const DeviceSolvePlan&, const SolveArgs<double>&, const double*, double*,
+                                              int32_t, double*, double*, double*, FlowCtl&, hipStream_t, ProfileSink*);
+template hipError_t launchSolvePlan<float, float>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
+                                            int32_t, float*, float*, float*, FlowCtl&, hipStream_t, ProfileSink*);
+template hipError_t launchSolvePlan<float, double>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
+                                             int32_t, double*, double*, double*, FlowCtl&, hipStream_t, ProfileSink*);
 
 }  // namespace amp

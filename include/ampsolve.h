@@ -150,7 +150,8 @@ int amp_sym_set_pattern(amp_sym *sym, const int32_t *rowptr, const int32_t *coli
 int amp_download_factor(amp_num *num, double *Lx, double *D);
 
 /* Device time (ms) of the last factor and of the last solve, and the total of each kernel class of the last
- * factor when profiling is enabled (amp_num_set_profiling). */
+ * factor when profiling is enabled (amp_num_set_profiling). `kernel` is a class name ("gemm_tile", "solve_fwd_flow",
+ * ...) or a class name and an elimination-tree level ("gemm_tile@23") for that level's launches only. */
 int amp_num_set_profiling(amp_num *num, int32_t enable);
 int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, int32_t *launches);
 

@@ -31,6 +31,8 @@ def sim():
     L.ps_factor.restype = C.c_int32
     L.ps_solve.argtypes = [C.c_void_p, f64p, f64p, C.c_int32]
     L.ps_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.ps_check_flow.argtypes = [C.c_void_p]
+    L.ps_check_flow.restype = C.c_int64
     return L
 
 
@@ -59,6 +61,7 @@ def test_plan_cholesky_and_ldlt(sim, name):
     S = O.Sym(pat)
     h = sim.ps_create(n, pat["rowptr"], pat["colidx"], None, 1)
     try:
+        assert sim.ps_check_flow(h) == 0  # every solve task waits only on tasks listed before it
         stats = (C.c_int64 * 6)()
         sim.ps_stats(h, stats)
         ok, Lo = O.chol_factor(S, v)
