@@ -17,60 +17,86 @@ __device__ inline float laneValue(float v, int l) {
 
 }  // namespace
 
-// One wave per block, no barriers. Lane i keeps row i of the block in registers (right-looking elimination: at step k
-// every lane reads the pivot from lane k, lane k records it, every lane scales its entry of column k, and the column's
-// entries are broadcast lane by lane through scalar registers for the rank-1 update of each row). The inverse
-// W = L^-1 is built in the same steps by Gauss-Jordan elimination with lane c keeping column c of W:
-// W[k][c] /= l_kk, then W[i][c] -= l_ik W[k][c] for i > k, with the same broadcast column. All loops over the 64
-// columns are unrolled, so every operand is a register.
+// Four waves per block. Wave w owns the columns j = 4 jj + w of the block (lane i keeps A[i][j] in rA[jj]) and the
+// rows i = 4 ii + w of the inverse W = L^-1 (lane c keeps W[i][c] in rW[ii]), so at every elimination step the
+// columns and rows still active are spread evenly over the waves. The pivot step of column k (by its owner wave):
+// read the pivot from lane k (every lane evaluates the pivot rule on the broadcast value, lane k records it), scale
+// column k, finish row k of W (Gauss-Jordan: W[k][:] *= 1 / l_kk) and publish both through LDS (double-buffered by
+// step parity, one barrier per step). The update of step k: every wave applies the rank-1 update to its active
+// columns (A[i][j] -= l_ik l_jk d, l_jk read from the column's own lane through a scalar register) and rows of W
+// (W[i][:] -= l_jk W[k][:]). Lookahead: the owner of column k + 1 updates that column first and runs its pivot step
+// while the other waves finish the update of step k. The step loop is unrolled, so every register index is a
+// constant.
 template <typename T>
-__global__ __launch_bounds__(64) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
-  constexpr int E = kTileEdge, P = kTileEdge + 1;
+__global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
+  constexpr int E = kTileEdge, P = kTileEdge + 1, J = kTileEdge / 4;
+  __shared__ T colL[2][E], rowW[2][E];
+  __shared__ T s_mult[2];
   __shared__ T Ws[E * P];
   const DiagTask t = tasks[blockIdx.x];
-  const int lane = threadIdx.x, nb = t.nb;
+  const int lane = threadIdx.x & 63, nb = t.nb;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index, held in a scalar register
   T* F = a.Lx + t.f_off;
-  T r[E], w[E];
+  T rA[J], rW[J];
 #pragma unroll
-  for (int j = 0; j < E; j++) {
-    r[j] = (j < nb && lane < nb && lane >= j) ? F[lane + int64_t(j) * t.ld] : T(0);
-    w[j] = j == lane ? T(1) : T(0);
+  for (int jj = 0; jj < J; jj++) {
+    const int j = 4 * jj + w;
+    rA[jj] = (j < nb && lane < nb && lane >= j) ? F[lane + int64_t(j) * t.ld] : T(0);
+    rW[jj] = j == lane ? T(1) : T(0);  // row j of W
   }
-#pragma unroll
+  auto pivotStep = [&](const int k) {
+    const int kk = k >> 2, p = k & 1;
+    T inv, mult;
+    const T dk = laneValue(rA[kk], k);
+    const T piv = pivotRule<T>(dk, t.col0 + k, a, &inv, &mult, lane == k);
+    if (lane == k) rA[kk] = piv;
+    if (lane > k) rA[kk] *= inv;
+    const T wk = a.ldlt ? rW[kk] : rW[kk] * inv;
+    rW[kk] = wk;
+    colL[p][lane] = rA[kk];
+    rowW[p][lane] = wk;
+    if (lane == 0) s_mult[p] = mult;
+  };
+  if (nb > 0 && w == 0) pivotStep(0);
+  __syncthreads();
+#pragma clang loop unroll(full)
   for (int k = 0; k < E; k++) {
-    if (k < nb) {
-      // every lane evaluates the pivot rule on the same value; only lane k records it (D, flags, failure)
-      T inv, mult;
-      const T dk = laneValue(r[k], k);
-      const T piv = pivotRule<T>(dk, t.col0 + k, a, &inv, &mult, lane == k);
-      if (lane == k) r[k] = piv;
-      if (lane > k) r[k] *= inv;
-      const T lik = r[k];
-      const T lkk = laneValue(r[k], k);
-      const T wk = a.ldlt ? w[k] : w[k] / lkk;
-      w[k] = wk;
+    if (k >= nb) continue;  // uniform
+    const int p = k & 1, k1 = k + 1;
+    const T cl = colL[p][lane], rw = rowW[p][lane], mult = s_mult[p];
+    const bool ahead = k1 < nb && w == (k1 & 3);
+    if (k1 < E && ahead) {
+      const int jj = k1 >> 2;
+      const T ljk = laneValue(cl, k1);
+      if (lane >= k1) rA[jj] -= cl * (ljk * mult);
+      rW[jj] -= ljk * rw;
+      pivotStep(k1);
+    }
 #pragma unroll
-      for (int j = k + 1; j < E; j++) {
-        const T ljk = laneValue(lik, j);
-        if (lane >= j) r[j] -= lik * (ljk * mult);
-        w[j] -= ljk * wk;
+    for (int jj = 0; jj < J; jj++) {
+      const int j = 4 * jj + w;
+      if (4 * jj + 3 > k && j > k && !(ahead && j == k1)) {
+        const T ljk = laneValue(cl, j);
+        if (lane >= j) rA[jj] -= cl * (ljk * mult);
+        rW[jj] -= ljk * rw;
       }
     }
+    __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < E; j++)
-    if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = r[j];
-  // W column `lane` -> LDS (element (i, lane)), then stores along columns of W
-#pragma unroll
-  for (int i = 0; i < E; i++) Ws[i * P + lane] = (i < nb && lane < nb) ? w[i] : T(0);
+  for (int jj = 0; jj < J; jj++) {
+    const int j = 4 * jj + w;
+    if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = rA[jj];
+    Ws[j * P + lane] = (j < nb && lane < nb) ? rW[jj] : T(0);  // W[j][lane]
+  }
   __syncthreads();
-  T* Wt = a.W + t.w_off;
-  for (int j = 0; j < E; j++) Wt[lane + j * E] = Ws[lane * P + j];
+  T* Wt = a.W + t.w_off;  // column-major: Wt[i + c E] = W[i][c]
+  for (int idx = threadIdx.x; idx < E * E; idx += 256) Wt[idx] = Ws[(idx & 63) * P + (idx >> 6)];
 }
 
 template <typename T>
 void launchDiagBlocks(const DiagTask* tasks, unsigned count, const FactorArgs<T>& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_block<T>, dim3(count), dim3(64), 0, st, tasks, a);
+  hipLaunchKernelGGL(k_diag_block<T>, dim3(count), dim3(256), 0, st, tasks, a);
 }
 
 template void launchDiagBlocks<double>(const DiagTask*, unsigned, const FactorArgs<double>&, hipStream_t);

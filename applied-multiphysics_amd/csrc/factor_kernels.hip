@@ -129,39 +129,46 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
   }
 }
 
-// 64 x 64 output tile of C (+)= -A diag(d) B^T (modes 0, 1) or C = A B^T diag(d)^-1 (mode 2) on the matrix cores.
-// Four waves, each owning a 32 x 32 quarter (2 x 2 MFMA blocks of 16 x 16). The depth is staged through LDS in chunks
-// of 32 (A and B depth-major, so a lane's operand is one LDS read); the next chunk is loaded into registers while the
-// current one feeds the matrix cores. The tile is written back through the same LDS so every global access runs
-// along columns.
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U, T* W,
-                                                  const T* __restrict__ D, int32_t ldlt) {
+// Output tiles of up to G x G (G = kGemmEdge or kTileEdge). G = 128: eight waves in 4 x 2, each owning 32 rows x 64
+// columns; G = 64: four waves in 2 x 2, each 32 x 32 (MFMA blocks of 16 x 16), so every SIMD runs two waves of a
+// 128-tile. The matrix cores are fed the transposed product (operand roles of A and B swapped), so the accumulator
+// components of a lane hold 16 consecutive rows of one column of C and the epilogue reads and writes C straight from
+// registers along columns. The depth is staged through LDS in chunks of KC (A and B depth-major, one LDS read per
+// operand fragment); the next chunk is loaded into registers while the current one feeds the matrix cores.
+// Workgroups are dispatched round-robin over the eight XCDs; each XCD is given a contiguous run of the task list,
+// whose neighbouring tiles share operand panels in that XCD's L2.
+template <typename T, int G>
+__global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U,
+                                                                   T* W, const T* __restrict__ D, int32_t ldlt) {
   using V4 = typename Vec4<T>::type;
-  constexpr int E = kTileEdge, KC = 32, PA = kTileEdge + 4, PC = kTileEdge + 1, NL = KC / 4;
-  static_assert(2 * KC * PA >= E * PC, "epilogue staging must fit in the operand buffers");
-  __shared__ T smem[2 * KC * PA];
-  T* As = smem;
-  T* Bs = smem + KC * PA;
-  T* Cs = smem;
-  const GemmTask t = tasks[blockIdx.x];
+  constexpr int WRW = G == 128 ? 4 : 2, WCW = 2, NT = 64 * WRW * WCW;  // wave grid, threads
+  constexpr int WTR = G / WRW, WTC = G / WCW, NBI = WTR / 16, NBJ = WTC / 16;
+  constexpr int KC = G == 128 ? 16 : 32, PA = G + 4;  // depth chunk, LDS row pitch
+  constexpr int LS = NT / G, NL = KC / LS;            // loader: depth stride, depths per thread
+  __shared__ T As[KC * PA];
+  __shared__ T Bs[KC * PA];
+  const int nblk = gridDim.x, q8 = nblk / 8, r8 = nblk % 8, x8 = blockIdx.x % 8;
+  const GemmTask t = tasks[x8 * q8 + min(x8, r8) + blockIdx.x / 8];
   T* const bases[3] = {Lx, U, W};
   const T* A = bases[t.a_buf] + t.a_off;
   const T* B = bases[t.b_buf] + t.b_off;
   T* C = bases[t.c_buf] + t.c_off;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+  const int wr = (wave % WRW) * WTR, wc = (wave / WRW) * WTC;
+  const bool active = wr < t.rows && wc < t.cols;  // uniform per wave
   const bool scaleDepth = ldlt && t.mode != 2;
-  const int li = tid & 63, lk = tid >> 6;  // loader: row li, depths lk + 4 r
+  const int li = tid % G, lk = tid / G;  // loader: row li, depths lk + LS r
   const bool rowA = li < t.rows, rowB = li < t.cols;
-  V4 acc[2][2];
-  for (int bi = 0; bi < 2; bi++)
-    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = V4{0, 0, 0, 0};
+  V4 acc[NBI][NBJ];
+#pragma unroll
+  for (int bi = 0; bi < NBI; bi++)
+#pragma unroll
+    for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = V4{0, 0, 0, 0};
   T pa[NL], pb[NL];
   auto load = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < NL; r++) {
-      const int k = k0 + lk + 4 * r;
+      const int k = k0 + lk + LS * r;
       const bool kin = k < t.k;
       pa[r] = (kin && rowA) ? A[li + int64_t(k) * t.lda] : T(0);
       T bv = (kin && rowB) ? B[li + int64_t(k) * t.ldb] : T(0);
@@ -173,43 +180,52 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
   for (int k0 = 0; k0 < t.k; k0 += KC) {
 #pragma unroll
     for (int r = 0; r < NL; r++) {
-      As[(lk + 4 * r) * PA + li] = pa[r];
-      Bs[(lk + 4 * r) * PA + li] = pb[r];
+      As[(lk + LS * r) * PA + li] = pa[r];
+      Bs[(lk + LS * r) * PA + li] = pb[r];
     }
     __syncthreads();
     if (k0 + KC < t.k) load(k0 + KC);
+    if (active) {
 #pragma unroll
-    for (int k4 = 0; k4 < KC; k4 += 4) {
-      const int kk = k4 + (lane >> 4);
-      T av[2], bv[2];
-      for (int bi = 0; bi < 2; bi++) av[bi] = As[kk * PA + wr + bi * 16 + (lane & 15)];
-      for (int bj = 0; bj < 2; bj++) bv[bj] = Bs[kk * PA + wc + bj * 16 + (lane & 15)];
-      for (int bi = 0; bi < 2; bi++)
-        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = mfma16x16x4(av[bi], bv[bj], acc[bi][bj]);
+      for (int k4 = 0; k4 < KC; k4 += 4) {
+        const int kk = (k4 + (lane >> 4)) * PA + (lane & 15);
+        T af[NBI], bf[NBJ];
+#pragma unroll
+        for (int b = 0; b < NBI; b++) af[b] = As[kk + wr + 16 * b];
+#pragma unroll
+        for (int b = 0; b < NBJ; b++) bf[b] = Bs[kk + wc + 16 * b];
+#pragma unroll
+        for (int bi = 0; bi < NBI; bi++)
+#pragma unroll
+          for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = mfma16x16x4(bf[bj], af[bi], acc[bi][bj]);
+      }
     }
     __syncthreads();
   }
-  for (int bi = 0; bi < 2; bi++)
-    for (int bj = 0; bj < 2; bj++)
-      for (int r = 0; r < 4; r++) {
-        const int row = wr + bi * 16 + mfmaRow<T>(lane, r), col = wc + bj * 16 + (lane & 15);
-        Cs[row + col * PC] = acc[bi][bj][r];
+  if (!active) return;
+  // component r of acc[bi][bj]: C row wr + 16 bi + (lane & 15), C column wc + 16 bj + mfmaRow(lane, r)
+  const int row0 = wr + (lane & 15);
+#pragma unroll
+  for (int bj = 0; bj < NBJ; bj++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = wc + 16 * bj + mfmaRow<T>(lane, r);
+      if (col >= t.cols) continue;
+      const T dinv = (ldlt && t.mode == 2) ? T(1) / D[t.d_off + col] : T(1);
+      T* Cc = C + int64_t(col) * t.ldc;
+#pragma unroll
+      for (int bi = 0; bi < NBI; bi++) {
+        const int row = row0 + 16 * bi;
+        if (row >= t.rows) continue;
+        const T v = acc[bi][bj][r];
+        if (t.mode == 0)
+          Cc[row] = Cc[row] - v;
+        else if (t.mode == 1)
+          Cc[row] = -v;
+        else
+          Cc[row] = ldlt ? v * dinv : v;
       }
-  __syncthreads();
-  for (int idx = tid; idx < E * E; idx += 256) {
-    const int i = idx & 63, j = idx >> 6;
-    if (i >= t.rows || j >= t.cols) continue;
-    T v = Cs[i + j * PC];
-    T& dst = C[i + int64_t(j) * t.ldc];
-    if (t.mode == 0)
-      dst = dst - v;
-    else if (t.mode == 1)
-      dst = -v;
-    else {
-      if (ldlt) v = v / D[t.d_off + j];
-      dst = v;
     }
-  }
 }
 
 // nNeg / nPerturbed over the columns before the first failure (the counts the reference returns, ldlt.mjs:505-519).
@@ -281,8 +297,12 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         launchDiagBlocks<T>(dp.diag + L.first, grid, a, st);
         break;
       case kGemm:
-        hipLaunchKernelGGL(k_gemm_tile<T>, dim3(grid), dim3(256), 0, st,
-                           reinterpret_cast<const GemmTask*>(dp.gemm) + L.first, a.Lx, a.U, a.W, a.D, a.ldlt);
+        if (L.edge == kGemmEdge)
+          hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge>), dim3(grid), dim3(512), 0, st,
+                             reinterpret_cast<const GemmTask*>(dp.gemm) + L.first, a.Lx, a.U, a.W, a.D, a.ldlt);
+        else
+          hipLaunchKernelGGL((k_gemm_tile<T, kTileEdge>), dim3(grid), dim3(256), 0, st,
+                             reinterpret_cast<const GemmTask*>(dp.gemm) + L.first, a.Lx, a.U, a.W, a.D, a.ldlt);
         break;
     }
     if (prof && 2 * prof->used + 1 < prof->events.size()) {

@@ -2,6 +2,7 @@
 #include "plan.h"
 
 #include <algorithm>
+#include <functional>
 
 namespace amp {
 
@@ -79,9 +80,24 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
   }
 
   auto push = [&P](LaunchKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0,
-                   int32_t threads = 0) {
+                   int32_t threads = 0, int32_t edge = 0) {
     if (count <= 0) return;
-    P.launches.push_back(Launch{kind, level, first, count, maxM, threads});
+    P.launches.push_back(Launch{kind, level, first, count, maxM, threads, edge});
+  };
+  // one product launch: its tiles at edge kGemmEdge when there are enough of them to fill the device, else at
+  // kTileEdge; `add(edge, out)` appends the tiles of every front of the launch
+  auto gemmLaunch = [&P, &push](int32_t level, const std::function<void(int32_t, std::vector<GemmTask>&)>& add) {
+    std::vector<GemmTask> tl;
+    int32_t edge = kGemmEdge;
+    add(edge, tl);
+    if (int64_t(tl.size()) < kWideLaunch) {
+      tl.clear();
+      edge = kTileEdge;
+      add(edge, tl);
+    }
+    const int64_t first = int64_t(P.gemm.size());
+    P.gemm.insert(P.gemm.end(), tl.begin(), tl.end());
+    push(kGemm, level, first, int64_t(tl.size()), 0, 0, edge);
   };
 
   // zero the L blocks of tiled fronts, then scatter A into them
@@ -125,37 +141,34 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
       }
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }
-    // panel factorization, all tiled fronts of the level in lockstep: super-panels of kSuperPanel columns are first
-    // brought up to date left-looking (one product of depth = all earlier columns), then factored right-looking in
-    // 64-column steps whose trailing updates stay inside the super-panel
+    // panel factorization, all tiled fronts of the level in lockstep, right-looking in super-panels of kSuperPanel
+    // columns: a super-panel is factored in 64-column steps whose trailing updates stay inside it, then one product
+    // of depth kSuperPanel brings every later own column of the front up to date
     int32_t maxNc = 0;
     for (int32_t s : big) maxNc = std::max(maxNc, frontCols(S, s));
-    for (int32_t c0 = 0; c0 < maxNc; c0 += kSuperPanel) {
-      if (c0 > 0) {
-        const int64_t first = int64_t(P.gemm.size());
-        for (int32_t s : big) {
-          const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
-          if (nc <= c0) continue;
-          const int32_t c1 = std::min(nc, c0 + kSuperPanel);
-          const int64_t base = S.Lxp[s];
-          for (int32_t j0 = c0; j0 < c1; j0 += kTileEdge)
-            for (int32_t i0 = j0; i0 < m; i0 += kTileEdge) {
-              GemmTask t{};
-              t.a_off = base + i0;
-              t.b_off = base + j0;
-              t.c_off = base + i0 + int64_t(j0) * m;
-              t.lda = t.ldb = t.ldc = m;
-              t.rows = std::min(kTileEdge, m - i0);
-              t.cols = std::min(kTileEdge, c1 - j0);
-              t.k = c0;
-              t.d_off = f;
-              t.a_buf = t.b_buf = t.c_buf = kBufLx;
-              t.mode = 0;
-              P.gemm.push_back(t);
-            }
+    // C[i0.., j0..] -= L[i0.., d0 : d0 + depth] L[j0.., d0 : d0 + depth]^T over columns [j0from, j0to) and rows
+    // [j0, m) of front s (lower tiles only), as tiles of the given edge
+    auto lowerUpdate = [&S](std::vector<GemmTask>& out, int32_t edge, int32_t s, int32_t j0from, int32_t j0to,
+                            int32_t d0, int32_t depth) {
+      const int32_t m = frontRows(S, s), f = S.superStart[s];
+      const int64_t base = S.Lxp[s];
+      for (int32_t j0 = j0from; j0 < j0to; j0 += edge)
+        for (int32_t i0 = j0; i0 < m; i0 += edge) {
+          GemmTask t{};
+          t.a_off = base + i0 + int64_t(d0) * m;
+          t.b_off = base + j0 + int64_t(d0) * m;
+          t.c_off = base + i0 + int64_t(j0) * m;
+          t.lda = t.ldb = t.ldc = m;
+          t.rows = std::min(edge, m - i0);
+          t.cols = std::min(edge, j0to - j0);
+          t.k = depth;
+          t.d_off = f + d0;
+          t.a_buf = t.b_buf = t.c_buf = kBufLx;
+          t.mode = 0;
+          out.push_back(t);
         }
-        push(kGemm, L, first, int64_t(P.gemm.size()) - first);
-      }
+    };
+    for (int32_t c0 = 0; c0 < maxNc; c0 += kSuperPanel) {
       for (int32_t k0 = c0; k0 < std::min(maxNc, c0 + kSuperPanel); k0 += kTileEdge) {
         const int64_t dFirst = int64_t(P.diag.size());
         for (int32_t s : big) {
@@ -194,61 +207,51 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
             P.gemm.push_back(t);
           }
         }
-        push(kGemm, L, tFirst, int64_t(P.gemm.size()) - tFirst);
+        push(kGemm, L, tFirst, int64_t(P.gemm.size()) - tFirst, 0, 0, kTileEdge);
         // trailing update of the remaining columns of this super-panel
-        const int64_t uFirst = int64_t(P.gemm.size());
-        for (int32_t s : big) {
-          const int32_t nc = frontCols(S, s), m = frontRows(S, s), f = S.superStart[s];
-          if (nc <= k0) continue;
-          const int32_t nb = std::min(kTileEdge, nc - k0);
-          const int32_t c1 = std::min(nc, c0 + kSuperPanel);
-          const int64_t base = S.Lxp[s];
-          for (int32_t j0 = k0 + nb; j0 < c1; j0 += kTileEdge)
-            for (int32_t i0 = j0; i0 < m; i0 += kTileEdge) {
-              GemmTask t{};
-              t.a_off = base + i0 + int64_t(k0) * m;
-              t.b_off = base + j0 + int64_t(k0) * m;
-              t.c_off = base + i0 + int64_t(j0) * m;
-              t.lda = t.ldb = t.ldc = m;
-              t.rows = std::min(kTileEdge, m - i0);
-              t.cols = std::min(kTileEdge, c1 - j0);
-              t.k = nb;
-              t.d_off = f + k0;
-              t.a_buf = t.b_buf = t.c_buf = kBufLx;
-              t.mode = 0;
-              P.gemm.push_back(t);
-            }
-        }
-        push(kGemm, L, uFirst, int64_t(P.gemm.size()) - uFirst);
+        gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
+          for (int32_t s : big) {
+            const int32_t nc = frontCols(S, s);
+            if (nc <= k0) continue;
+            const int32_t nb = std::min(kTileEdge, nc - k0);
+            lowerUpdate(out, edge, s, k0 + nb, std::min(nc, c0 + kSuperPanel), k0, nb);
+          }
+        });
       }
+      // every later own column of the front, from this super-panel
+      gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), c1 = c0 + kSuperPanel;
+          if (nc <= c1) continue;
+          lowerUpdate(out, edge, s, c1, nc, c0, kSuperPanel);
+        }
+      });
     }
     // Schur complement U = -L21 D L21^T into the update matrix (write only)
-    {
-      const int64_t first = int64_t(P.gemm.size());
+    gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
       for (int32_t s : big) {
         const int32_t nc = frontCols(S, s), m = frontRows(S, s), mu = m - nc, f = S.superStart[s];
         if (mu == 0) continue;
         const int64_t base = S.Lxp[s];
-        for (int32_t j0 = 0; j0 < mu; j0 += kTileEdge)
-          for (int32_t i0 = j0; i0 < mu; i0 += kTileEdge) {
+        for (int32_t j0 = 0; j0 < mu; j0 += edge)
+          for (int32_t i0 = j0; i0 < mu; i0 += edge) {
             GemmTask t{};
             t.a_off = base + nc + i0;
             t.b_off = base + nc + j0;
             t.c_off = P.Uoff[s] + i0 + int64_t(j0) * mu;
             t.lda = t.ldb = m;
             t.ldc = mu;
-            t.rows = std::min(kTileEdge, mu - i0);
-            t.cols = std::min(kTileEdge, mu - j0);
+            t.rows = std::min(edge, mu - i0);
+            t.cols = std::min(edge, mu - j0);
             t.k = nc;
             t.d_off = f;
             t.a_buf = t.b_buf = kBufLx;
             t.c_buf = kBufU;
             t.mode = 1;
-            P.gemm.push_back(t);
+            out.push_back(t);
           }
       }
-      push(kGemm, L, first, int64_t(P.gemm.size()) - first);
-    }
+    });
     // children's contributions to the update matrix
     {
       const int64_t first = int64_t(P.ea.size());

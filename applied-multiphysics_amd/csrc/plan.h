@@ -13,7 +13,9 @@
 
 namespace amp {
 
-constexpr int kTileEdge = 64;   // panel width / update-tile edge of tiled fronts; block size of their solves
+constexpr int kTileEdge = 64;   // panel width of tiled fronts (diagonal blocks); block size of their solves
+constexpr int kGemmEdge = 128;  // output-tile edge of update products in launches with enough tiles
+constexpr int kWideLaunch = 512;  // tiles of edge kGemmEdge a launch needs to use them (else kTileEdge)
 constexpr int kEaWidth = 16;    // parent columns per extend-add work item
 constexpr int kSuperPanel = 256;  // columns brought up to date together (left-looking) in a tiled front
 constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
@@ -63,6 +65,7 @@ struct Launch {
   int64_t first, count;  // range in the task array of this kind (kSmall: in smallFronts; kAssemble: lower entries)
   int32_t maxM;          // kSmall: largest front of the launch (LDS sizing)
   int32_t threads;       // kSmall: workgroup size
+  int32_t edge;          // kGemm: output-tile edge of the launch's tasks (kTileEdge or kGemmEdge)
 };
 
 struct FactorPlan {
