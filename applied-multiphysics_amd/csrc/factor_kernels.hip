@@ -90,7 +90,7 @@ __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ 
 
 // Extend-add into parent columns [j0, j1) of a tiled front. Each child column maps to one parent column through the
 // increasing relative map, so the child columns of the range are contiguous; children are added one after another
-// (reference order), each child's entries land on distinct parent entries.
+// (reference order), each child's entries land on distinct parent entries, so the waves split a child's columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks, FactorArgs<T> a) {
   const EaTask t = tasks[blockIdx.x];
@@ -114,16 +114,25 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
       if (rel[mid] < t.j1) lo = mid + 1; else hi = mid;
     }
     const int32_t b1 = lo;
-    for (int32_t b = b0; b < b1; b++) {  // child column b lands in parent column rel[b], rows rel[r] for r >= b
+    // child column b lands in parent column rel[b], rows rel[r] for r >= b; each wave takes its own columns (distinct
+    // parent columns) and keeps four rows per lane in flight
+    const int lane = threadIdx.x & 63;
+    for (int32_t b = b0 + int32_t(threadIdx.x >> 6); b < b1; b += int32_t(blockDim.x >> 6)) {
       const int32_t j = rel[b];
       const T* src = Uc + int64_t(b) * muc;
-      if (j < nc) {
-        T* dst = L + int64_t(j) * m;
-        for (int32_t r = b + threadIdx.x; r < muc; r += blockDim.x) dst[rel[r]] += src[r];
-      } else {
-        T* dst = Us + int64_t(j - nc) * mu;
-        for (int32_t r = b + threadIdx.x; r < muc; r += blockDim.x) dst[rel[r] - nc] += src[r];
+      T* dst = j < nc ? L + int64_t(j) * m : Us + int64_t(j - nc) * mu;
+      const int32_t sh = j < nc ? 0 : nc;  // parent row -> row of the column's storage
+      int32_t r = b + lane;
+      for (; r + 192 < muc; r += 256) {
+        const int32_t d0 = rel[r] - sh, d1 = rel[r + 64] - sh, d2 = rel[r + 128] - sh, d3 = rel[r + 192] - sh;
+        const T s0 = src[r], s1 = src[r + 64], s2 = src[r + 128], s3 = src[r + 192];
+        const T v0 = dst[d0], v1 = dst[d1], v2 = dst[d2], v3 = dst[d3];
+        dst[d0] = v0 + s0;
+        dst[d1] = v1 + s1;
+        dst[d2] = v2 + s2;
+        dst[d3] = v3 + s3;
       }
+      for (; r < muc; r += 64) dst[rel[r] - sh] += src[r];
     }
     __syncthreads();
   }
