@@ -151,7 +151,9 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
     a.tau = pivotTol * (amax > 0 ? amax : 1);
   }
   HIP_TRY(hipEventRecord(num->ev0, num->stream));
-  if (n > 0) amp::launchFactorPlan<T>(h->dfp, a, d_values, num->stream, num->profiling ? &num->prof : nullptr);
+  if (n > 0)
+    amp::launchFactorPlan<T>(h->dfp, a, d_values, num->stream, num->side, num->syncEv,
+                             num->profiling ? &num->prof : nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(num->ev1, num->stream));
   int32_t counts[2] = {0, 0}, failCol = INT32_MAX;
@@ -415,6 +417,9 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   const size_t es = precision == AMP_F64 ? 8 : 4;
   const amp::FactorPlan& F = h->F;
   HIP_TRY(hipStreamCreateWithFlags(&num->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&num->side, hipStreamNonBlocking));
+  num->syncEv.resize(size_t(F.signals) + 1, nullptr);
+  for (hipEvent_t& e : num->syncEv) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&num->ev0));
   HIP_TRY(hipEventCreate(&num->ev1));
   HIP_TRY(num->Lx.alloc(size_t(F.nnzLStored) * es));

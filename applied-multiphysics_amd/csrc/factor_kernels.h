@@ -28,9 +28,11 @@ struct ProfileSink {
   size_t used = 0;
 };
 
+// Enqueue the factor plan on `main`, with the launches marked kSyncSide on `side`; `sync` holds at least
+// FactorPlan::signals + 1 events. The side stream is joined into `main` before returning.
 template <typename T>
-void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t st,
-                      ProfileSink* prof);
+void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t main,
+                      hipStream_t side, const std::vector<hipEvent_t>& sync, ProfileSink* prof);
 template <typename T>
 void launchLdltCounts(const FactorArgs<T>& a, int32_t n, int32_t* out, hipStream_t st);
 

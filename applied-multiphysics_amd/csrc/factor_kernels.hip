@@ -269,12 +269,17 @@ __global__ void k_set_int(int32_t* p, int32_t v) { *p = v; }
 
 // ---- launchers ------------------------------------------------------------------------------------------------
 template <typename T>
-void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t st,
-                      ProfileSink* prof) {
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, a.failCol, INT32_MAX);
+void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t main,
+                      hipStream_t side, const std::vector<hipEvent_t>& sync, ProfileSink* prof) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, main, a.failCol, INT32_MAX);
   if (prof) prof->used = 0;
+  hipEvent_t last[2] = {nullptr, nullptr};  // most recent signal of each stream
+  size_t ev = 0;
   for (const Launch& L : dp.launches) {
     const unsigned grid = unsigned(L.count);
+    const int si = (L.sync & kSyncSide) ? 1 : 0;
+    hipStream_t st = si ? side : main;
+    if ((L.sync & kSyncWait) && last[1 - si]) (void)hipStreamWaitEvent(st, last[1 - si], 0);
     if (prof && 2 * prof->used + 1 < prof->events.size()) {
       (void)hipEventRecord(prof->events[2 * prof->used], st);
       prof->kinds[prof->used] = L.kind;
@@ -318,6 +323,14 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
       (void)hipEventRecord(prof->events[2 * prof->used + 1], st);
       prof->used++;
     }
+    if ((L.sync & kSyncSignal) && ev < sync.size()) {
+      (void)hipEventRecord(sync[ev], st);
+      last[si] = sync[ev++];
+    }
+  }
+  if (last[1] && ev < sync.size()) {  // join the side stream into the main stream
+    (void)hipEventRecord(sync[ev], side);
+    (void)hipStreamWaitEvent(main, sync[ev], 0);
   }
 }
 
@@ -327,9 +340,9 @@ void launchLdltCounts(const FactorArgs<T>& a, int32_t n, int32_t* out, hipStream
 }
 
 template void launchFactorPlan<double>(const DeviceFactorPlan&, const FactorArgs<double>&, const double*, hipStream_t,
-                                       ProfileSink*);
+                                       hipStream_t, const std::vector<hipEvent_t>&, ProfileSink*);
 template void launchFactorPlan<float>(const DeviceFactorPlan&, const FactorArgs<float>&, const double*, hipStream_t,
-                                      ProfileSink*);
+                                      hipStream_t, const std::vector<hipEvent_t>&, ProfileSink*);
 template void launchLdltCounts<double>(const FactorArgs<double>&, int32_t, int32_t*, hipStream_t);
 template void launchLdltCounts<float>(const FactorArgs<float>&, int32_t, int32_t*, hipStream_t);
 

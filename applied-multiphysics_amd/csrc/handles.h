@@ -78,7 +78,8 @@ struct amp_sym {
 struct amp_num {
   amp_sym* sym = nullptr;
   int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, side = nullptr;  // side: bulk trailing updates overlapping the panel chain
+  std::vector<hipEvent_t> syncEv;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evR0 = nullptr, evR1 = nullptr;
   amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv, pz;
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
@@ -98,6 +99,8 @@ struct amp_num {
     if (evR0) (void)hipEventDestroy(evR0);
     if (evR1) (void)hipEventDestroy(evR1);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : syncEv) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

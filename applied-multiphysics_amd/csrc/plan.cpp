@@ -79,14 +79,22 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
     }
   }
 
-  auto push = [&P](LaunchKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0,
-                   int32_t threads = 0, int32_t edge = 0) {
-    if (count <= 0) return;
-    P.launches.push_back(Launch{kind, level, first, count, maxM, threads, edge});
+  bool joinPending = false;  // the next main-stream launch must wait for the side stream
+  auto push = [&P, &joinPending](LaunchKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0,
+                                 int32_t threads = 0, int32_t edge = 0, int32_t sync = 0) {
+    if (count <= 0) return false;
+    if (joinPending && !(sync & kSyncSide)) {
+      sync |= kSyncWait;
+      joinPending = false;
+    }
+    P.launches.push_back(Launch{kind, level, first, count, maxM, threads, edge, sync});
+    if (sync & kSyncSignal) P.signals++;
+    return true;
   };
   // one product launch: its tiles at edge kGemmEdge when there are enough of them to fill the device, else at
-  // kTileEdge; `add(edge, out)` appends the tiles of every front of the launch
-  auto gemmLaunch = [&P, &push](int32_t level, const std::function<void(int32_t, std::vector<GemmTask>&)>& add) {
+  // kTileEdge; `add(edge, out)` appends the tiles of every front of the launch. Returns whether a launch was made.
+  auto gemmLaunch = [&P, &push](int32_t level, const std::function<void(int32_t, std::vector<GemmTask>&)>& add,
+                                int32_t sync = 0) {
     std::vector<GemmTask> tl;
     int32_t edge = kGemmEdge;
     add(edge, tl);
@@ -97,7 +105,7 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
     }
     const int64_t first = int64_t(P.gemm.size());
     P.gemm.insert(P.gemm.end(), tl.begin(), tl.end());
-    push(kGemm, level, first, int64_t(tl.size()), 0, 0, edge);
+    return push(kGemm, level, first, int64_t(tl.size()), 0, 0, edge, sync);
   };
 
   // zero the L blocks of tiled fronts, then scatter A into them
@@ -142,10 +150,15 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }
     // panel factorization, all tiled fronts of the level in lockstep, right-looking in super-panels of kSuperPanel
-    // columns: a super-panel is factored in 64-column steps whose trailing updates stay inside it, then one product
-    // of depth kSuperPanel brings every later own column of the front up to date
+    // columns: a super-panel is factored in 64-column steps whose trailing updates stay inside it (main stream);
+    // then the next super-panel's columns are brought up to date from it (main stream, lookahead) while every later
+    // column is updated from it on the side stream (bulk), concurrently with the next super-panel's factorization.
+    // Ordering: the bulk update of super-panel s waits for s to be factored (signal after its last step); the
+    // lookahead update from s + 1 waits for the bulk update of s (the side stream's last signal at that point), which
+    // also wrote the columns of s + 2; the Schur complement waits for the last bulk update.
     int32_t maxNc = 0;
     for (int32_t s : big) maxNc = std::max(maxNc, frontCols(S, s));
+    bool sideUsed = false;
     // C[i0.., j0..] -= L[i0.., d0 : d0 + depth] L[j0.., d0 : d0 + depth]^T over columns [j0from, j0to) and rows
     // [j0, m) of front s (lower tiles only), as tiles of the given edge
     auto lowerUpdate = [&S](std::vector<GemmTask>& out, int32_t edge, int32_t s, int32_t j0from, int32_t j0to,
@@ -218,16 +231,27 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
           }
         });
       }
-      // every later own column of the front, from this super-panel
+      // signal: super-panel c0 factored (its last step is the last main-stream launch so far)
+      P.launches.back().sync |= kSyncSignal;
+      P.signals++;
+      const int32_t c1 = c0 + kSuperPanel, c2 = c1 + kSuperPanel;
+      // lookahead: the next super-panel's columns, after the previous bulk update (which also wrote them)
       gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
         for (int32_t s : big) {
-          const int32_t nc = frontCols(S, s), c1 = c0 + kSuperPanel;
-          if (nc <= c1) continue;
-          lowerUpdate(out, edge, s, c1, nc, c0, kSuperPanel);
+          const int32_t nc = frontCols(S, s);
+          if (nc > c1) lowerUpdate(out, edge, s, c1, std::min(nc, c2), c0, kSuperPanel);
         }
-      });
+      }, sideUsed ? kSyncWait : 0);
+      // bulk: every later column, on the side stream once the super-panel is factored
+      if (gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
+            for (int32_t s : big) {
+              const int32_t nc = frontCols(S, s);
+              if (nc > c2) lowerUpdate(out, edge, s, c2, nc, c0, kSuperPanel);
+            }
+          }, kSyncSide | kSyncWait | kSyncSignal))
+        sideUsed = true;
     }
-    // Schur complement U = -L21 D L21^T into the update matrix (write only)
+    // Schur complement U = -L21 D L21^T into the update matrix (write only), after the last bulk update
     gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
       for (int32_t s : big) {
         const int32_t nc = frontCols(S, s), m = frontRows(S, s), mu = m - nc, f = S.superStart[s];
@@ -251,7 +275,9 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
             out.push_back(t);
           }
       }
-    });
+    }, sideUsed ? kSyncWait : 0);
+    // no Schur launch made (every front of the level is a root): the next main-stream launch joins the side stream
+    if (sideUsed && (P.launches.back().sync & kSyncSide)) joinPending = true;
     // children's contributions to the update matrix
     {
       const int64_t first = int64_t(P.ea.size());

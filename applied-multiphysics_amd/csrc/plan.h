@@ -66,7 +66,13 @@ struct Launch {
   int32_t maxM;          // kSmall: largest front of the launch (LDS sizing)
   int32_t threads;       // kSmall: workgroup size
   int32_t edge;          // kGemm: output-tile edge of the launch's tasks (kTileEdge or kGemmEdge)
+  int32_t sync;          // kSyncSide | kSyncWait | kSyncSignal
 };
+
+// Two streams: the main stream and a side stream for the bulk trailing updates of large fronts, which overlap the
+// next super-panel's factorization on the main stream. kSyncSide: run on the side stream; kSyncWait: first wait for
+// the other stream's most recent signal (enqueue order); kSyncSignal: signal after the launch.
+constexpr int32_t kSyncSide = 1, kSyncWait = 2, kSyncSignal = 4;
 
 struct FactorPlan {
   int32_t n = 0, nsuper = 0, nlevels = 0;
@@ -88,6 +94,7 @@ struct FactorPlan {
   std::vector<DiagTask> diag;
   std::vector<GemmTask> gemm;
   std::vector<Launch> launches;
+  int32_t signals = 0;  // launches with kSyncSignal
 };
 
 FactorPlan buildFactorPlan(const Symbolic& S);
