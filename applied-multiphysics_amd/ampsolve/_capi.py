@@ -78,6 +78,11 @@ SIGNATURES = {
                            C.POINTER(RefineInfo)], C.c_int),
     "amp_sym_set_pattern": ([C.c_void_p, i32p, i32p], C.c_int),
     "amp_download_factor": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "amp_graph_create": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                          C.POINTER(C.c_void_p)], C.c_int),
+    "amp_graph_launch": ([C.c_void_p], C.c_int),
+    "amp_graph_sync": ([C.c_void_p, C.POINTER(FactorInfo)], C.c_int),
+    "amp_graph_free": ([C.c_void_p], None),
     "amp_num_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "amp_num_get_kernel_ms": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
     "amp_num_free": ([C.c_void_p], None),

@@ -135,6 +135,37 @@ def checkSymbolic(sym, pat):
         raise ValueError("symbolic analysis is for a different sparsity pattern")
 
 
+class FactorGraph:
+    """A captured factor + solve + refinement sequence (amp_graph_*): launch() replays it on the factor's stream,
+    sync() waits and returns the factor status of the replay with its device time."""
+
+    def __init__(self, num, values, B, X, nrhs, irSteps):
+        self.num = num
+        self._keep = (values, B, X)  # the graph reads these device buffers at every replay
+        h = C.c_void_p()
+        A.check(A.lib().amp_graph_create(num._h, A.ptr(values), A.ptr(B), A.ptr(X), int(nrhs), int(irSteps),
+                                         C.byref(h)))
+        self._h = h
+
+    def launch(self):
+        A.check(A.lib().amp_graph_launch(self._h))
+
+    def sync(self):
+        info = A.FactorInfo()
+        A.check(A.lib().amp_graph_sync(self._h, C.byref(info)))
+        self.num.ok = bool(info.ok)
+        return {"ok": bool(info.ok), "failColumn": info.failColumn, "ms": info.ms}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                A.lib().amp_graph_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 class NumericFactor:
     """A factor resident in HBM (the NumericFactor of types.d.mts:21-26): sym, ok, kernel, info."""
 
@@ -177,6 +208,11 @@ class NumericFactor:
         x = np.zeros_like(b) if out is None else out
         A.check(A.lib().amp_solve(self._h, A.ptr(b) if b.size else None, A.ptr(x) if x.size else None, int(q)))
         return x
+
+    def graph(self, values, B, X, nrhs, irSteps):
+        """Capture factor(values) + solve of the nrhs right-hand sides in B into X + irSteps FP64 refinement steps as
+        one replayable device graph (device tensors, read again at every replay). Cholesky handles only."""
+        return FactorGraph(self, values, B, X, nrhs, irSteps)
 
     def refine(self, values, b, method="ir", maxSteps=3, restart=40, tol=1e-14, out=None):
         """Refine the factor's solution of one right-hand side on the device (FP64 residual and update):

@@ -223,3 +223,36 @@ def maxwell_edge(ncell=20, sigma=50.0, seed=3, rhs_seed=4, eps_choices=(1.0, 2.1
     R, _ = restrict(A, ~on)
     b = lcg_normal(rhs_seed, R.shape[0])
     return csr_dict(R), R.data.astype(np.float64), b
+
+
+def heat2d(nx, kel, Dirichlet=True):
+    """2D P1 heat on the unit square, nx x nx cells split into triangles (rect_tri_mesh), per-triangle conductivity
+    kel (len 2 nx^2), Dirichlet on every edge. Returns the restricted CSR matrix."""
+    nodes, tris = rect_tri_mesh(nx, nx)
+    Ke = p1_element_stiffness(nodes[tris]) * np.asarray(kel, dtype=np.float64)[:, None, None]
+    A = assemble(len(nodes), tris, Ke)
+    x, y = nodes[:, 0], nodes[:, 1]
+    eps = 1e-12
+    free = (x > eps) & (x < 1 - eps) & (y > eps) & (y < 1 - eps)
+    R, _ = restrict(A, free)
+    return R
+
+
+def batch_heat2d(nsys=512, lo=20, hi=120, nrhs=8, seed=5, rhs_seed=6):
+    """Config: nsys independent 2D P1 heat systems on structured unit-square grids of side s ~ U{lo..hi} cells (LCG),
+    per-triangle conductivity log-uniform in [1, 100] (10^(2u), one LCG stream over all systems in order), Dirichlet
+    on every edge, nrhs normal right-hand sides each. The systems are assembled into one block-diagonal pattern
+    (system i at rows offsets[i] .. offsets[i + 1]); right-hand side k of the union stacks the systems' k-th
+    right-hand sides. Returns (pat, values, B (nrhs x n), sides, offsets)."""
+    sides = lo + np.minimum((lcg_uniform(seed, nsys) * (hi - lo + 1)).astype(np.int64), hi - lo)
+    ntri = 2 * sides ** 2
+    kel = 10.0 ** (2.0 * lcg_uniform(seed + 1, int(ntri.sum())))
+    kofs = np.concatenate([[0], np.cumsum(ntri)])
+    blocks = [heat2d(int(s), kel[kofs[i]:kofs[i + 1]]) for i, s in enumerate(sides)]
+    sizes = np.array([b.shape[0] for b in blocks], dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    A = sp.block_diag(blocks, format="csr")
+    A.sort_indices()
+    n = int(offsets[-1])
+    B = lcg_normal(rhs_seed, nrhs * n).reshape(nrhs, n)
+    return csr_dict(A), A.data.astype(np.float64), B, sides, offsets

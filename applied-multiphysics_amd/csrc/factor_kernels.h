@@ -63,20 +63,22 @@ struct SolveArgs {
 };
 
 // Synchronisation state of the tiled-supernode solve launches, owned by one numeric handle (launches of one handle
-// run in stream order). Flags hold the pass number (`epoch`, raised once per pass of at most kFlowRhs right-hand
-// sides) of the last pass that completed the block, so they never need clearing. err is set when a wait gives up.
+// run in stream order). Flags hold the pass number of the last pass that completed the block, so they never need
+// clearing; the pass number lives in device memory (*epoch) and is raised by a kernel at the start of every pass
+// of at most kFlowRhs right-hand sides, so a captured graph raises it on every replay. err is set when a wait gives
+// up.
 struct FlowCtl {
   int32_t* fwdFlag;
   int32_t* bwdFlag;
   int32_t* err;
-  int32_t epoch;
+  int32_t* epoch;
 };
 
 // Solve q systems in place on the device: b (q vectors of n, original order) -> x. y (n q), fv (fvEntries q) and pz
 // (initSlots x 64 x min(q, kFlowRhs)) are scratch of the working precision S; Lx / W / D are the factor's precision
-// T. Raises fc.epoch once per pass.
+// T. Raises *fc.epoch once per pass. Enqueues kernels only (no host synchronisation), so it can be captured.
 template <typename T, typename S>
 hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q,
-                           S* y, S* fv, S* pz, FlowCtl& fc, hipStream_t st, ProfileSink* prof);
+                           S* y, S* fv, S* pz, const FlowCtl& fc, hipStream_t st, ProfileSink* prof);
 
 }  // namespace amp

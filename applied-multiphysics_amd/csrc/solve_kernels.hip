@@ -11,6 +11,11 @@
 
 namespace amp {
 
+// Raise the pass number of the tiled-supernode solve flags (wrapping to 1; flags start at 0).
+__global__ void k_epoch_bump(int32_t* epoch) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
+}
+
 template <typename S>
 __global__ void k_perm_in(const double* __restrict__ b, S* __restrict__ y, const int32_t* __restrict__ perm, int32_t n,
                           int32_t q) {
@@ -53,34 +58,39 @@ __device__ inline void gatherFront(const SolveArgs<T>& a, int32_t s, const S* yk
   }
 }
 
-template <typename T, typename S>
+// Supernodes with m <= kSmallMax, one workgroup each, all right-hand sides of the pass together: per column the q
+// solution values are formed by q threads and broadcast through LDS, then every thread updates its rows for all q.
+template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_small_fwd(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y,
                                                          S* fv, int32_t q, int64_t fvN) {
-  __shared__ S v[kSmallMax];
-  __shared__ S s_x;
+  __shared__ S v[Q][kSmallMax];
+  __shared__ S xc[Q];
   const int32_t s = fronts[blockIdx.x];
   const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
   const T* L = a.Lx + a.Lxp[s];
+  for (int32_t k = 0; k < q; k++) gatherFront<T, S>(a, s, y + int64_t(k) * a.n, fv + k * fvN, v[k], nc, m);
+  for (int col = 0; col < nc; col++) {
+    if (threadIdx.x < q) {
+      const S x = a.ldlt ? v[threadIdx.x][col] : v[threadIdx.x][col] / S(L[col + col * m]);
+      v[threadIdx.x][col] = x;
+      xc[threadIdx.x] = x;
+    }
+    __syncthreads();
+    for (int r = col + 1 + threadIdx.x; r < m; r += blockDim.x) {
+      const S l = S(L[r + col * m]);
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) v[k][r] -= l * xc[k];
+    }
+    __syncthreads();
+  }
   for (int32_t k = 0; k < q; k++) {
     S* yk = y + int64_t(k) * a.n;
     S* fk = fv + k * fvN;
-    gatherFront<T, S>(a, s, yk, fk, v, nc, m);
-    for (int col = 0; col < nc; col++) {
-      if (threadIdx.x == 0) {
-        const S xc = a.ldlt ? v[col] : v[col] / S(L[col + col * m]);
-        v[col] = xc;
-        s_x = xc;
-      }
-      __syncthreads();
-      const S xc = s_x;
-      for (int r = col + 1 + threadIdx.x; r < m; r += blockDim.x) v[r] -= S(L[r + col * m]) * xc;
-      __syncthreads();
-    }
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      if (i < nc) yk[f + i] = v[i];
-      fk[a.Sp[s] + i] = v[i];
+      if (i < nc) yk[f + i] = v[k][i];
+      fk[a.Sp[s] + i] = v[k][i];
     }
-    __syncthreads();
   }
 }
 
@@ -142,7 +152,7 @@ __device__ inline void flowPublish(int32_t* flag, int32_t val) {
 // block (x = W v) and publish x into y. Q is the number of right-hand sides the kernel is built for (q <= Q).
 template <typename T, typename S, int Q>
 __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, S* fv, int32_t q,
-                                        int64_t fvN, const FlowCtl& c, S (&part)[4][Q][kTileEdge],
+                                        int64_t fvN, const FlowCtl& c, int32_t want, S (&part)[4][Q][kTileEdge],
                                         S (&xs)[Q][kFlowRun * kTileEdge]) {
   constexpr int E = kTileEdge;
   const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
@@ -200,7 +210,7 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
     __syncthreads();
   }
   for (int32_t k0 = 0; k0 < (local0 ? 0 : steps);) {
-    const int32_t k1 = flowReady(c.fwdFlag + t.flag0, k0, min(steps, k0 + kFlowRun), c.epoch, c.err);
+    const int32_t k1 = flowReady(c.fwdFlag + t.flag0, k0, min(steps, k0 + kFlowRun), want, c.err);
     for (int idx = tid; idx < (k1 - k0) * E; idx += 256) {
       const int32_t col = k0 * E + idx;
 #pragma unroll
@@ -256,7 +266,7 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
   if (tid < E && live)
     for (int jj = 0; jj < q; jj++)
       y[int64_t(jj) * a.n + f + r] = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
-  if (nb > 1) flowPublish(c.fwdFlag + t.flag0 + t.blk, c.epoch);  // single blocks have no waiters
+  if (nb > 1) flowPublish(c.fwdFlag + t.flag0 + t.blk, want);  // single blocks have no waiters
 }
 
 // Persistent form: the grid is at most the number of co-resident workgroups (cooperative launch), and workgroup b
@@ -268,45 +278,57 @@ __global__ __launch_bounds__(256) void k_solve_fwd_flow(const BlockTask* __restr
                                                         FlowCtl c) {
   __shared__ S part[4][Q][kTileEdge];
   __shared__ S xs[Q][kFlowRun * kTileEdge];
+  const int32_t want = *c.epoch;  // this pass's number (raised by the kernel that starts the pass)
   for (int32_t i = blockIdx.x; i < ntasks; i += gridDim.x) {
-    fwdTask<T, S, Q>(tasks[i], a, y, fv, q, fvN, c, part, xs);
+    fwdTask<T, S, Q>(tasks[i], a, y, fv, q, fvN, c, want, part, xs);
     __syncthreads();
   }
 }
 
-template <typename T, typename S>
+template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y,
                                                          int32_t q) {
-  __shared__ S z[kSmallMax], xb[kSmallMax];
-  __shared__ S s_x;
+  __shared__ S z[Q][kSmallMax], xb[Q][kSmallMax];
+  __shared__ S xc[Q];
   const int32_t s = fronts[blockIdx.x];
   const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   const T* L = a.Lx + a.Lxp[s];
   const int32_t* rows = a.Si + a.Sp[s] + nc;
-  for (int32_t k = 0; k < q; k++) {
-    S* yk = y + int64_t(k) * a.n;
-    for (int r = threadIdx.x; r < mu; r += blockDim.x) xb[r] = yk[rows[r]];
-    __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-      S sum = yk[f + i];
-      for (int r = 0; r < mu; r++) sum -= S(L[nc + r + i * m]) * xb[r];
-      z[i] = sum;
+  for (int32_t k = 0; k < q; k++)
+    for (int r = threadIdx.x; r < mu; r += blockDim.x) xb[k][r] = y[int64_t(k) * a.n + rows[r]];
+  __syncthreads();
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    S sum[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) sum[k] = k < q ? y[int64_t(k) * a.n + f + i] : S(0);
+    for (int r = 0; r < mu; r++) {
+      const S l = S(L[nc + r + i * m]);
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) sum[k] -= l * xb[k][r];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; k++)
+      if (k < q) z[k][i] = sum[k];
+  }
+  __syncthreads();
+  for (int col = nc - 1; col >= 0; col--) {
+    if (threadIdx.x < q) {
+      const S x = a.ldlt ? z[threadIdx.x][col] : z[threadIdx.x][col] / S(L[col + col * m]);
+      z[threadIdx.x][col] = x;
+      xc[threadIdx.x] = x;
     }
     __syncthreads();
-    for (int col = nc - 1; col >= 0; col--) {
-      if (threadIdx.x == 0) {
-        const S xc = a.ldlt ? z[col] : z[col] / S(L[col + col * m]);
-        z[col] = xc;
-        s_x = xc;
-      }
-      __syncthreads();
-      const S xc = s_x;
-      for (int i = threadIdx.x; i < col; i += blockDim.x) z[i] -= S(L[col + i * m]) * xc;
-      __syncthreads();
+    for (int i = threadIdx.x; i < col; i += blockDim.x) {
+      const S l = S(L[col + i * m]);
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) z[k][i] -= l * xc[k];
     }
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) yk[f + i] = z[i];
     __syncthreads();
   }
+  for (int32_t k = 0; k < q; k++)
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) y[int64_t(k) * a.n + f + i] = z[k][i];
 }
 
 // y[below] of each tiled supernode of a level into the below part of its front vector (contiguous for the backward
@@ -367,7 +389,8 @@ __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restr
 // (x staged in LDS, L loaded one block ahead), and x = W^T z is published.
 template <typename T, typename S, int Q>
 __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, const S* pz, int32_t q,
-                                        const FlowCtl& c, S (&xs)[Q][kFlowRun * kTileEdge], S (&zs)[Q][kTileEdge]) {
+                                        const FlowCtl& c, int32_t want, S (&xs)[Q][kFlowRun * kTileEdge],
+                                        S (&zs)[Q][kTileEdge]) {
   constexpr int E = kTileEdge;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int cj = 16 * w + (lane & 15), rs = lane >> 4;
@@ -399,7 +422,7 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
   const int32_t kEnd = t.blk;  // blocks nb - 1 down to blk + 1
   if (nb - 1 > kEnd) loadL(cur, nb - 1);
   for (int32_t k0 = nb - 1; k0 > kEnd;) {
-    const int32_t k1 = flowReady(c.bwdFlag + t.flag0, k0, max(kEnd, k0 - kFlowRun), c.epoch, c.err);
+    const int32_t k1 = flowReady(c.bwdFlag + t.flag0, k0, max(kEnd, k0 - kFlowRun), want, c.err);
     // blocks k0 down to k1 + 1 are complete; stage their x (block k at slot k0 - k)
     for (int idx = tid; idx < (k0 - k1) * E; idx += 256) {
       const int32_t row = (k0 - idx / E) * E + (idx % E);
@@ -445,7 +468,7 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
     sum += __shfl_xor(sum, 32, 64);
     if (jj < q && rs == 0 && live) y[int64_t(jj) * a.n + f + col] = sum;
   }
-  if (t.blk > 0) flowPublish(c.bwdFlag + t.flag0 + t.blk, c.epoch);  // block 0 has no waiters
+  if (t.blk > 0) flowPublish(c.bwdFlag + t.flag0 + t.blk, want);  // block 0 has no waiters
 }
 
 // Persistent form, as k_solve_fwd_flow.
@@ -454,8 +477,9 @@ __global__ __launch_bounds__(256) void k_solve_bwd_flow(const BlockTask* __restr
                                                         SolveArgs<T> a, S* y, const S* pz, int32_t q, FlowCtl c) {
   __shared__ S xs[Q][kFlowRun * kTileEdge];
   __shared__ S zs[Q][kTileEdge];
+  const int32_t want = *c.epoch;
   for (int32_t i = blockIdx.x; i < ntasks; i += gridDim.x) {
-    bwdTask<T, S, Q>(tasks[i], a, y, pz, q, c, xs, zs);
+    bwdTask<T, S, Q>(tasks[i], a, y, pz, q, c, want, xs, zs);
     __syncthreads();
   }
 }
@@ -513,8 +537,12 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
     }
     switch (L.kind) {
       case kSolveSmallFwd:
-        hipLaunchKernelGGL((k_solve_small_fwd<T, S>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
-                           dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
+        if (q == 1)
+          hipLaunchKernelGGL((k_solve_small_fwd<T, S, 1>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+                             dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
+        else
+          hipLaunchKernelGGL((k_solve_small_fwd<T, S, kFlowRhs>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+                             dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
       case kSolveGather:
         hipLaunchKernelGGL((k_solve_gather<T, S>), dim3(grid), dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q,
@@ -527,8 +555,12 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                                   y, fv, q, dp.fvEntries, fc);
         break;
       case kSolveSmallBwd:
-        hipLaunchKernelGGL((k_solve_small_bwd<T, S>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
-                           dp.fronts + L.first, a, y, q);
+        if (q == 1)
+          hipLaunchKernelGGL((k_solve_small_bwd<T, S, 1>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+                             dp.fronts + L.first, a, y, q);
+        else
+          hipLaunchKernelGGL((k_solve_small_bwd<T, S, kFlowRhs>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+                             dp.fronts + L.first, a, y, q);
         break;
       case kSolveBwdGather:
         hipLaunchKernelGGL((k_solve_bwd_gather<T, S>), dim3(grid), dim3(256), 0, st, dp.fronts + L.first, a, y, fv,
@@ -564,12 +596,12 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
 
 template <typename T, typename S>
 hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const double* b, double* x, int32_t q,
-                           S* y, S* fv, S* pz, FlowCtl& fc, hipStream_t st, ProfileSink* prof) {
+                           S* y, S* fv, S* pz, const FlowCtl& fc, hipStream_t st, ProfileSink* prof) {
   const int32_t n = a.n;
   if (n == 0 || q == 0) return hipSuccess;
   for (int32_t k0 = 0; k0 < q; k0 += kFlowRhs) {
     const int32_t qc = std::min(kFlowRhs, q - k0);
-    fc.epoch = fc.epoch == INT32_MAX ? 1 : fc.epoch + 1;
+    hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(64), 0, st, fc.epoch);
     const hipError_t e = solvePass<T, S>(dp, a, b + int64_t(k0) * n, x + int64_t(k0) * n, qc, y, fv, pz, fc, st, prof);
     if (e != hipSuccess) return e;
   }
@@ -577,10 +609,10 @@ hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, con
 }
 
 template hipError_t launchSolvePlan<double, double>(const DeviceSolvePlan&, const SolveArgs<double>&, const double*, double*,
-                                              int32_t, double*, double*, double*, FlowCtl&, hipStream_t, ProfileSink*);
+                                              int32_t, double*, double*, double*, const FlowCtl&, hipStream_t, ProfileSink*);
 template hipError_t launchSolvePlan<float, float>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                            int32_t, float*, float*, float*, FlowCtl&, hipStream_t, ProfileSink*);
+                                            int32_t, float*, float*, float*, const FlowCtl&, hipStream_t, ProfileSink*);
 template hipError_t launchSolvePlan<float, double>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                             int32_t, double*, double*, double*, FlowCtl&, hipStream_t, ProfileSink*);
+                                             int32_t, double*, double*, double*, const FlowCtl&, hipStream_t, ProfileSink*);
 
 }  // namespace amp

@@ -34,6 +34,7 @@ extern "C" {
 
 typedef struct amp_sym amp_sym; /* symbolic analysis + execution plan (host), device copy of the plan */
 typedef struct amp_num amp_num; /* numeric factor resident in HBM */
+typedef struct amp_graph amp_graph; /* captured factor + solve + refinement sequence of one amp_num */
 
 typedef struct {
   int32_t n, nsuper, maxFront, nlevels;
@@ -156,6 +157,20 @@ int amp_num_set_profiling(amp_num *num, int32_t enable);
 int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, int32_t *launches);
 
 void amp_num_free(amp_num *num);
+
+/* Whole-sequence capture (no counterpart in the reference, whose factor and solve are separate synchronous calls):
+ * the factor of the values at d_values, the solve of the nrhs right-hand sides at d_b (device, n doubles each, column
+ * after column) into d_x, then irSteps steps of iterative refinement x += solve(b - A x) (FP64 residual on the CSR
+ * pattern), captured once into a HIP graph and replayed by amp_graph_launch with no host synchronisation inside --
+ * for callers that refactor and resolve the same pattern many times, such as a batch of independent systems
+ * assembled into one block-diagonal pattern. The device buffers are read at every replay (new values / right-hand
+ * sides in place). Cholesky handles only (AMP_ERR_ARG otherwise); the pattern must be known for irSteps > 0.
+ * amp_graph_sync waits for the last replay and reports its factor status (info->ms is the replay's device time). */
+int amp_graph_create(amp_num *num, const double *d_values, const double *d_b, double *d_x, int32_t nrhs,
+                     int32_t irSteps, amp_graph **out);
+int amp_graph_launch(amp_graph *g);
+int amp_graph_sync(amp_graph *g, amp_factor_info *info);
+void amp_graph_free(amp_graph *g);
 
 /* Message of the last failure on the calling thread ("" when none). */
 const char *amp_last_error(void);
