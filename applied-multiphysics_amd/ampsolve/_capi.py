@@ -58,6 +58,12 @@ class FactorInfo(C.Structure):
                 ("ms", C.c_double)]
 
 
+class TuneInfo(C.Structure):
+    _fields_ = [("smallMax", C.c_int32), ("smallNarrow", C.c_int32), ("wideLaunch", C.c_int32),
+                ("superPanel", C.c_int32), ("candidates", C.c_int32), ("defaultMs", C.c_double),
+                ("bestMs", C.c_double)]
+
+
 # every entry point of include/ampsolve.h with its ctypes signature
 SIGNATURES = {
     "amp_amd_order": ([C.c_int32, i32p, i32p, C.c_double, i32p], C.c_int),
@@ -78,6 +84,8 @@ SIGNATURES = {
                            C.POINTER(RefineInfo)], C.c_int),
     "amp_sym_set_pattern": ([C.c_void_p, i32p, i32p], C.c_int),
     "amp_download_factor": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "amp_sym_autotune": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneInfo)], C.c_int),
+    "amp_sym_set_params": ([C.c_void_p, C.POINTER(TuneInfo)], C.c_int),
     "amp_graph_create": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                           C.POINTER(C.c_void_p)], C.c_int),
     "amp_graph_launch": ([C.c_void_p], C.c_int),

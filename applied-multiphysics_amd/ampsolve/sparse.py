@@ -91,6 +91,28 @@ class Symbolic:
         self.Lxp = _copy(arr.Lxp, ns + 1, np.int64)
         self.timing = {}
 
+    def autotune(self, values, mode=0, kind="hip"):
+        """Time the factor and one solve of `values` (a device tensor on the pattern) on the device for candidate
+        shape thresholds and keep the fastest plan (amp_sym_autotune). Call before creating numeric factors."""
+        t = A.TuneInfo()
+        prec = A.AMP_F32 if kind == "hip-f32" else A.AMP_F64
+        A.check(A.lib().amp_sym_autotune(self._h, A.ptr(values), int(mode), prec, C.byref(t)))
+        self._refresh()
+        return {f: getattr(t, f) for f, _ in A.TuneInfo._fields_}
+
+    def set_params(self, params):
+        """Apply shape thresholds (a dict as autotune returns) and rebuild the plans (amp_sym_set_params)."""
+        t = A.TuneInfo()
+        for f in ("smallMax", "smallNarrow", "wideLaunch", "superPanel"):
+            setattr(t, f, int(params[f]))
+        A.check(A.lib().amp_sym_set_params(self._h, C.byref(t)))
+        self._refresh()
+
+    def _refresh(self):
+        info = A.SymInfo()
+        A.check(A.lib().amp_sym_get_info(self._h, C.byref(info)))
+        self.gemmFlops = info.gemmFlops
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

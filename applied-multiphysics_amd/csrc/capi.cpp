@@ -31,7 +31,7 @@ namespace {
 
 void finishSym(amp_sym* h) {
   const amp::Symbolic& S = h->S;
-  h->F = amp::buildFactorPlan(S);
+  h->F = amp::buildFactorPlan(S, h->params);
   h->SP = amp::buildSolvePlan(S, h->F);
   h->diagSrc.assign(S.n, -1);
   for (int32_t j = 0; j < S.n; j++)
@@ -413,6 +413,7 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   if (rc) return rc;
   std::unique_ptr<amp_num> num(new amp_num());
   num->sym = h;
+  h->liveNums++;
   num->mode = mode;
   num->prec = precision;
   const size_t es = precision == AMP_F64 ? 8 : 4;
@@ -655,5 +656,122 @@ int amp_graph_sync(amp_graph* g, amp_factor_info* info) {
 }
 
 void amp_graph_free(amp_graph* g) { delete g; }
+
+}  // extern "C"
+
+namespace {
+
+int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
+  h->params = p;
+  h->F = amp::buildFactorPlan(h->S, p);
+  h->SP = amp::buildSolvePlan(h->S, h->F);
+  h->onDevice = false;
+  return symToDevice(h);
+}
+
+bool validParams(const amp::PlanParams& p) {
+  return p.smallMax >= 1 && p.smallMax <= amp::kSmallMax && p.smallNarrow >= 1 && p.smallNarrow <= p.smallMax &&
+         p.wideLaunch >= 1 && p.superPanel >= amp::kTileEdge && p.superPanel % amp::kTileEdge == 0;
+}
+
+// factor + one solve (zero right-hand side), device milliseconds of the second of two runs
+int timePlan(amp_sym* h, const double* d_values, int32_t mode, int32_t precision, double* ms) {
+  amp_num* raw = nullptr;
+  int rc = amp_num_create(h, mode, precision, &raw);
+  if (rc) return rc;
+  std::unique_ptr<amp_num> num(raw);
+  HIP_TRY(hipMemset(num->b.p, 0, size_t(std::max(h->S.n, 1)) * 8));
+  for (int rep = 0; rep < 2; rep++) {
+    amp_factor_info fi{};
+    rc = num->prec == AMP_F64 ? runFactor<double>(num.get(), d_values, 1e-13, &fi)
+                              : runFactor<float>(num.get(), d_values, 1e-13, &fi);
+    if (rc) return rc;
+    if (!fi.ok) return fail(AMP_ERR_ARG, "amp_sym_autotune: the factorization fails on these values");
+    rc = runSolve(num.get(), num->b.as<double>(), num->x.as<double>(), 1);
+    if (rc) return rc;
+    *ms = fi.ms + num->solveMs;
+  }
+  return AMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amp_sym_set_params(amp_sym* h, const amp_tune_info* t) {
+  if (!h || !t) return fail(AMP_ERR_ARG, "amp_sym_set_params: bad arguments");
+  if (h->liveNums) return fail(AMP_ERR_STATE, "amp_sym_set_params: numeric handles of this analysis exist");
+  amp::PlanParams p;
+  p.smallMax = t->smallMax;
+  p.smallNarrow = t->smallNarrow;
+  p.wideLaunch = t->wideLaunch;
+  p.superPanel = t->superPanel;
+  if (!validParams(p)) return fail(AMP_ERR_ARG, "amp_sym_set_params: thresholds out of range");
+  return rebuildPlans(h, p);
+}
+
+int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t precision, amp_tune_info* info) {
+  if (!h || !d_values || (mode != AMP_CHOLESKY && mode != AMP_LDLT) || (precision != AMP_F64 && precision != AMP_F32))
+    return fail(AMP_ERR_ARG, "amp_sym_autotune: bad arguments");
+  if (h->liveNums) return fail(AMP_ERR_STATE, "amp_sym_autotune: numeric handles of this analysis exist");
+  if (h->S.n == 0) return fail(AMP_ERR_ARG, "amp_sym_autotune: empty system");
+  amp::PlanParams best;  // the defaults
+  int rc = rebuildPlans(h, best);
+  if (rc) return rc;
+  double bestMs = 0;
+  rc = timePlan(h, d_values, mode, precision, &bestMs);
+  if (rc) return rc;
+  const double defaultMs = bestMs;
+  int32_t tried = 1;
+  // one threshold at a time, from the defaults
+  const int32_t superPanels[] = {128, 512}, wides[] = {256, 1024, 1 << 30}, smalls[] = {64}, narrows[] = {16, 64};
+  auto attempt = [&](amp::PlanParams p) -> int {
+    if (!validParams(p)) return AMP_OK;
+    int r = rebuildPlans(h, p);
+    if (r) return r;
+    double ms = 0;
+    r = timePlan(h, d_values, mode, precision, &ms);
+    if (r) return r;
+    tried++;
+    if (ms < bestMs) {
+      bestMs = ms;
+      best = p;
+    }
+    return AMP_OK;
+  };
+  for (int32_t v : superPanels) {
+    amp::PlanParams p = best;
+    p.superPanel = v;
+    if ((rc = attempt(p))) return rc;
+  }
+  for (int32_t v : wides) {
+    amp::PlanParams p = best;
+    p.wideLaunch = v;
+    if ((rc = attempt(p))) return rc;
+  }
+  for (int32_t v : smalls) {
+    amp::PlanParams p = best;
+    p.smallMax = v;
+    p.smallNarrow = std::min(p.smallNarrow, v);
+    if ((rc = attempt(p))) return rc;
+  }
+  for (int32_t v : narrows) {
+    amp::PlanParams p = best;
+    p.smallNarrow = std::min(v, p.smallMax);
+    if ((rc = attempt(p))) return rc;
+  }
+  rc = rebuildPlans(h, best);
+  if (rc) return rc;
+  if (info) {
+    info->smallMax = best.smallMax;
+    info->smallNarrow = best.smallNarrow;
+    info->wideLaunch = best.wideLaunch;
+    info->superPanel = best.superPanel;
+    info->candidates = tried;
+    info->defaultMs = defaultMs;
+    info->bestMs = bestMs;
+  }
+  return AMP_OK;
+}
 
 }  // extern "C"

@@ -73,10 +73,12 @@ struct amp_sym {
   amp::DevBuf zeros, smallFronts, ea, diag, gemm, sFronts, sBlocks;
   amp::DeviceFactorPlan dfp;
   amp::DeviceSolvePlan dsp;
+  amp::PlanParams params;  // the thresholds the plans were built with
+  int32_t liveNums = 0;    // numeric handles on this analysis (their storage is laid out by the plans)
 };
 
 struct amp_num {
-  amp_sym* sym = nullptr;
+  amp_sym* sym = nullptr;  // counted in sym->liveNums while the handle lives
   int32_t mode = AMP_CHOLESKY, prec = AMP_F64;
   hipStream_t stream = nullptr, side = nullptr;  // side: bulk trailing updates overlapping the panel chain
   std::vector<hipEvent_t> syncEv;
@@ -93,6 +95,7 @@ struct amp_num {
   float solveMs = 0;
   std::map<std::string, std::pair<double, int32_t>> kernelMs;
   ~amp_num() {
+    if (sym) sym->liveNums--;
     for (hipEvent_t e : prof.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : solveProf.events) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);

@@ -13,7 +13,8 @@ inline int32_t frontCols(const Symbolic& S, int32_t s) { return S.superStart[s +
 
 }  // namespace
 
-FactorPlan buildFactorPlan(const Symbolic& S) {
+FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
+  const int32_t SPW = std::max(kTileEdge, prm.superPanel / kTileEdge * kTileEdge);  // super-panel width
   FactorPlan P;
   const int32_t n = S.n, ns = S.nsuper;
   P.n = n;
@@ -38,7 +39,7 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
   int64_t uTotal = 0, wTotal = 0;
   for (int32_t s = 0; s < ns; s++) {
     const int32_t m = frontRows(S, s), nc = frontCols(S, s), mu = m - nc;
-    P.isBig[s] = m > kSmallMax;
+    P.isBig[s] = m > std::min(prm.smallMax, kSmallMax);
     if (mu > 0) {
       P.Uoff[s] = uTotal;
       uTotal += int64_t(mu) * mu;
@@ -93,12 +94,12 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
   };
   // one product launch: its tiles at edge kGemmEdge when there are enough of them to fill the device, else at
   // kTileEdge; `add(edge, out)` appends the tiles of every front of the launch. Returns whether a launch was made.
-  auto gemmLaunch = [&P, &push](int32_t level, const std::function<void(int32_t, std::vector<GemmTask>&)>& add,
+  auto gemmLaunch = [&P, &push, &prm](int32_t level, const std::function<void(int32_t, std::vector<GemmTask>&)>& add,
                                 int32_t sync = 0) {
     std::vector<GemmTask> tl;
     int32_t edge = kGemmEdge;
     add(edge, tl);
-    if (int64_t(tl.size()) < kWideLaunch) {
+    if (int64_t(tl.size()) < prm.wideLaunch) {
       tl.clear();
       edge = kTileEdge;
       add(edge, tl);
@@ -128,7 +129,7 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
       int32_t maxM = 0;
       for (int32_t s : fronts) {
         const int32_t m = frontRows(S, s);
-        if (P.isBig[s] || (cls == 0) != (m <= 32)) continue;
+        if (P.isBig[s] || (cls == 0) != (m <= prm.smallNarrow)) continue;
         P.smallFronts.push_back(s);
         maxM = std::max(maxM, m);
       }
@@ -149,7 +150,7 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
       }
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }
-    // panel factorization, all tiled fronts of the level in lockstep, right-looking in super-panels of kSuperPanel
+    // panel factorization, all tiled fronts of the level in lockstep, right-looking in super-panels of SPW columns
     // columns: a super-panel is factored in 64-column steps whose trailing updates stay inside it (main stream);
     // then the next super-panel's columns are brought up to date from it (main stream, lookahead) while every later
     // column is updated from it on the side stream (bulk), concurrently with the next super-panel's factorization.
@@ -181,8 +182,8 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
           out.push_back(t);
         }
     };
-    for (int32_t c0 = 0; c0 < maxNc; c0 += kSuperPanel) {
-      for (int32_t k0 = c0; k0 < std::min(maxNc, c0 + kSuperPanel); k0 += kTileEdge) {
+    for (int32_t c0 = 0; c0 < maxNc; c0 += SPW) {
+      for (int32_t k0 = c0; k0 < std::min(maxNc, c0 + SPW); k0 += kTileEdge) {
         const int64_t dFirst = int64_t(P.diag.size());
         for (int32_t s : big) {
           const int32_t nc = frontCols(S, s), m = frontRows(S, s);
@@ -227,26 +228,26 @@ FactorPlan buildFactorPlan(const Symbolic& S) {
             const int32_t nc = frontCols(S, s);
             if (nc <= k0) continue;
             const int32_t nb = std::min(kTileEdge, nc - k0);
-            lowerUpdate(out, edge, s, k0 + nb, std::min(nc, c0 + kSuperPanel), k0, nb);
+            lowerUpdate(out, edge, s, k0 + nb, std::min(nc, c0 + SPW), k0, nb);
           }
         });
       }
       // signal: super-panel c0 factored (its last step is the last main-stream launch so far)
       P.launches.back().sync |= kSyncSignal;
       P.signals++;
-      const int32_t c1 = c0 + kSuperPanel, c2 = c1 + kSuperPanel;
+      const int32_t c1 = c0 + SPW, c2 = c1 + SPW;
       // lookahead: the next super-panel's columns, after the previous bulk update (which also wrote them)
       gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
         for (int32_t s : big) {
           const int32_t nc = frontCols(S, s);
-          if (nc > c1) lowerUpdate(out, edge, s, c1, std::min(nc, c2), c0, kSuperPanel);
+          if (nc > c1) lowerUpdate(out, edge, s, c1, std::min(nc, c2), c0, SPW);
         }
       }, sideUsed ? kSyncWait : 0);
       // bulk: every later column, on the side stream once the super-panel is factored
       if (gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
             for (int32_t s : big) {
               const int32_t nc = frontCols(S, s);
-              if (nc > c2) lowerUpdate(out, edge, s, c2, nc, c0, kSuperPanel);
+              if (nc > c2) lowerUpdate(out, edge, s, c2, nc, c0, SPW);
             }
           }, kSyncSide | kSyncWait | kSyncSignal))
         sideUsed = true;

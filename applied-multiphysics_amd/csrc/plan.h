@@ -97,7 +97,16 @@ struct FactorPlan {
   int32_t signals = 0;  // launches with kSyncSignal
 };
 
-FactorPlan buildFactorPlan(const Symbolic& S);
+// Thresholds that pick kernels by supernode shape; the defaults are the constants above, and a device autotuner may
+// choose others (each at most its constant, which sizes the kernels' static storage).
+struct PlanParams {
+  int32_t smallMax = kSmallMax;       // height m at or below which a front is factored and solved whole in LDS
+  int32_t smallNarrow = 32;           // height at or below which a whole-front workgroup has 64 threads (else 256)
+  int32_t wideLaunch = kWideLaunch;   // tiles a product launch needs to use kGemmEdge tiles
+  int32_t superPanel = kSuperPanel;   // columns of a super-panel (a multiple of kTileEdge)
+};
+
+FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams());
 
 // ---- solves ------------------------------------------------------------------------------------------------------
 // Every supernode s owns a front vector v_s of length m (at Sp[s] in a buffer of Sp[nsuper] entries per right-hand

@@ -158,6 +158,21 @@ int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, in
 
 void amp_num_free(amp_num *num);
 
+/* Shape thresholds of the device plans (no counterpart in the reference, whose kernel choice is per front,
+ * front-real.mjs:189-244): fronts with height m <= smallMax are factored and solved whole in LDS (64 threads when
+ * m <= smallNarrow, else 256); product launches use 128 x 128 tiles when they have at least wideLaunch of them (else
+ * 64 x 64); tiled fronts are factored in super-panels of superPanel columns. amp_sym_autotune times the factor and one
+ * solve of d_values (device, pattern order) on the device for a set of candidate thresholds, starting from the
+ * defaults and changing one threshold at a time, and keeps the fastest; amp_sym_set_params applies given thresholds.
+ * Both rebuild the plans, so they refuse (AMP_ERR_STATE) while numeric handles of the analysis exist. */
+typedef struct {
+  int32_t smallMax, smallNarrow, wideLaunch, superPanel;
+  int32_t candidates;        /* out: plans timed */
+  double defaultMs, bestMs;  /* out: factor + solve device time with the default and with the chosen thresholds */
+} amp_tune_info;
+int amp_sym_autotune(amp_sym *sym, const double *d_values, int32_t mode, int32_t precision, amp_tune_info *info);
+int amp_sym_set_params(amp_sym *sym, const amp_tune_info *params);
+
 /* Whole-sequence capture (no counterpart in the reference, whose factor and solve are separate synchronous calls):
  * the factor of the values at d_values, the solve of the nrhs right-hand sides at d_b (device, n doubles each, column
  * after column) into d_x, then irSteps steps of iterative refinement x += solve(b - A x) (FP64 residual on the CSR

@@ -3,8 +3,8 @@ log-uniform conductivity, Dirichlet edges, 2.8M DOFs in all) with 8 normal right
 block-diagonal pattern, so one analysis, one plan and one multi-right-hand-side solve cover the whole batch.
 
 Pins: the FP64 factor and solve of the union against the oracle's (the same forest); the captured FP32 factor + FP64
-refinement sequence (amp_graph_*) reaches |b - A x| / |b| <= 1e-10 for every system and right-hand side, agrees
-with the FP64 direct solution, and gives the exact scaling x / 2 when replayed on 2 A."""
+refinement sequence (amp_graph_*, on plans autotuned for the batch) reaches |b - A x| / |b| <= 1e-10 for every system
+and right-hand side, agrees with the FP64 direct solution, and gives the exact scaling x / 2 when replayed on 2 A."""
 import time
 
 import numpy as np
@@ -55,8 +55,10 @@ def test_captured_mixed_sequence(batch, device):
 
     print("batched graph", flush=True)
     pat, v, B, offs, A, sym, Xo = batch
-    num = amp.NumericFactor(sym, 0, "hip-f32")
     dv = torch.from_numpy(v).to(device)
+    tune = sym.autotune(dv, kind="hip-f32")  # shape thresholds for this batch, timed on the device
+    print(f"batched autotune: {tune}", flush=True)
+    num = amp.NumericFactor(sym, 0, "hip-f32")
     dB = torch.from_numpy(B.ravel().copy()).to(device)
     dX = torch.zeros_like(dB)
     g = num.graph(dv, dB, dX, NRHS, 3)

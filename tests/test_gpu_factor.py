@@ -212,3 +212,28 @@ def test_device_pointers_and_determinism(device):
     So = O.Sym(pat)
     _, Lo = O.chol_factor(So, v)
     assert relmax(x1, O.chol_solve(So, Lo, b)) < 1e-12
+
+
+def test_autotune_keeps_parity(device):
+    """The tuner rebuilds the plans with other shape thresholds: the factor must stay the oracle's to round-off, the
+    chosen thresholds must be no slower than the defaults on the measured runs, and plans cannot change under a
+    live numeric factor."""
+    import torch
+
+    pat, v, b = W.heat3d(16)
+    sym = amp.analyze(pat)
+    dv = torch.from_numpy(v).to(device)
+    t = sym.autotune(dv)
+    assert t["candidates"] >= 2 and t["bestMs"] <= t["defaultMs"]
+    assert 1 <= t["smallNarrow"] <= t["smallMax"] <= 96 and t["superPanel"] % 64 == 0
+    So = O.Sym(pat)
+    _, Lo = O.chol_factor(So, v)
+    xo = O.chol_solve(So, Lo, b)
+    num = amp.factorNumeric(sym, v)
+    assert num.ok and relmax(amp.solveNumeric(num, b), xo) < 1e-12
+    with pytest.raises(RuntimeError):
+        sym.set_params({"smallMax": 64, "smallNarrow": 32, "wideLaunch": 256, "superPanel": 128})
+    del num
+    sym.set_params({"smallMax": 64, "smallNarrow": 32, "wideLaunch": 256, "superPanel": 128})
+    num = amp.factorNumeric(sym, v)
+    assert num.ok and relmax(amp.solveNumeric(num, b), xo) < 1e-12
