@@ -64,6 +64,17 @@ class TuneInfo(C.Structure):
                 ("bestMs", C.c_double)]
 
 
+class PcgOpts(C.Structure):
+    _fields_ = [("maxIter", C.c_int32), ("tol", C.c_double)]
+
+
+class PcgInfo(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("relResidual", C.c_double),
+                ("initialResidual", C.c_double), ("ms", C.c_double)]
+
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
 # every entry point of include/ampsolve.h with its ctypes signature
 SIGNATURES = {
     "amp_amd_order": ([C.c_int32, i32p, i32p, C.c_double, i32p], C.c_int),
@@ -86,6 +97,18 @@ SIGNATURES = {
     "amp_download_factor": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amp_sym_autotune": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneInfo)], C.c_int),
     "amp_sym_set_params": ([C.c_void_p, C.POINTER(TuneInfo)], C.c_int),
+    "amp_pcg_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts), C.POINTER(PcgInfo)],
+                       C.c_int),
+    "amp_mlp_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
+    "amp_mlp_param_count": ([C.c_void_p], C.c_int64),
+    "amp_mlp_set_params": ([C.c_void_p, f32p, C.c_int64], C.c_int),
+    "amp_mlp_get_params": ([C.c_void_p, f32p, C.c_int64], C.c_int),
+    "amp_mlp_get_grads": ([C.c_void_p, f32p, C.c_int64], C.c_int),
+    "amp_mlp_forward": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "amp_mlp_gradient": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+    "amp_mlp_adam": ([C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
+    "amp_mlp_free": ([C.c_void_p], None),
+    "amp_num_release_workspace": ([C.c_void_p], C.c_int),
     "amp_graph_create": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                           C.POINTER(C.c_void_p)], C.c_int),
     "amp_graph_launch": ([C.c_void_p], C.c_int),

@@ -266,6 +266,10 @@ class NumericFactor:
     def set_profiling(self, enable=True):
         A.check(A.lib().amp_num_set_profiling(self._h, 1 if enable else 0))
 
+    def release_workspace(self):
+        """Free the update-matrix workspace (solves keep working; the next factor allocates it again)."""
+        A.check(A.lib().amp_num_release_workspace(self._h))
+
     def kernel_ms(self, name):
         ms, cnt = C.c_double(), C.c_int32()
         A.check(A.lib().amp_num_get_kernel_ms(self._h, name.encode(), C.byref(ms), C.byref(cnt)))

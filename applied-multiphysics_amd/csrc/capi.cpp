@@ -139,6 +139,7 @@ template <typename T>
 int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_info* info) {
   amp_sym* h = num->sym;
   const int32_t n = h->S.n;
+  if (!num->U.p && h->F.uEntries > 0) HIP_TRY(num->U.alloc(size_t(h->F.uEntries) * sizeof(T)));  // released before
   amp::FactorArgs<T> a = factorArgs<T>(num);
   if (num->mode == AMP_LDLT && n > 0) {
     // tau = pivotTol * max |a_jj| needs the diagonal on the host before the launch sequence
@@ -232,11 +233,14 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
   num->solveMs = ms;
+  for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
+    it = it->first.rfind("solve_", 0) == 0 ? num->kernelMs.erase(it) : std::next(it);
+  auto& total = num->kernelMs["solve_total"];  // recorded with or without profiling
+  total.first = ms;
+  total.second = 1;
   if (prof) {
     static const char* names[] = {"solve_small_fwd", "solve_gather", "solve_fwd_flow", "solve_small_bwd",
                                   "solve_bwd_flow", "solve_bwd_gather", "solve_bwd_init"};
-    for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
-      it = it->first.rfind("solve_", 0) == 0 ? num->kernelMs.erase(it) : std::next(it);
     for (size_t i = 0; i < prof->used; i++) {
       float k = 0;
       HIP_TRY(hipEventElapsedTime(&k, prof->events[2 * i], prof->events[2 * i + 1]));
@@ -247,9 +251,6 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
       lv.first += k;
       lv.second += 1;
     }
-    auto& total = num->kernelMs["solve_total"];
-    total.first = ms;
-    total.second = 1;
   }
   return AMP_OK;
 }
@@ -531,6 +532,13 @@ int amp_num_get_kernel_ms(amp_num* num, const char* kernel, double* total_ms, in
   auto it = num->kernelMs.find(kernel);
   *total_ms = it == num->kernelMs.end() ? 0.0 : it->second.first;
   *launches = it == num->kernelMs.end() ? 0 : it->second.second;
+  return AMP_OK;
+}
+
+int amp_num_release_workspace(amp_num* num) {
+  if (!num) return fail(AMP_ERR_ARG, "amp_num_release_workspace: bad arguments");
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  num->U.reset();
   return AMP_OK;
 }
 

@@ -208,9 +208,92 @@ int refineDevice(amp_num* num, const double* d_values, const double* d_b, double
   return AMP_OK;
 }
 
+// Preconditioned conjugate gradients from the guess in d_x (Hestenes-Stiefel recurrences; two dot products per
+// iteration read back for the step lengths and the stopping test on the recursively updated residual, with the true
+// residual recomputed at exit).
+int runPcg(amp_num* num, const double* d_values, const double* d_b, double* d_x, const amp_pcg_opts* o,
+           amp_pcg_info* info) {
+  amp_sym* h = num->sym;
+  const int32_t n = h->S.n;
+  if (!num->ok) return fail(AMP_ERR_STATE, "pcg: no successful numeric factorization");
+  if (o->maxIter < 0) return fail(AMP_ERR_ARG, "pcg: negative iteration count");
+  int rc = patternToDevice(h);
+  if (rc) return rc;
+  const size_t vb = size_t(std::max(n, 1)) * 8;
+  if ((rc = ensure(num->r, vb)) || (rc = ensure(num->dvec, vb)) || (rc = ensure(num->w, vb)) ||
+      (rc = ensure(num->V, 2 * vb)))
+    return rc;
+  if ((rc = ensure(num->partial, size_t(amp::partialEntries(amp::kMaxRestart + 2)) * 8))) return rc;
+  if ((rc = ensure(num->small, size_t(amp::kMaxRestart + 2) * 8))) return rc;
+  if (!num->evR0) {
+    HIP_TRY(hipEventCreate(&num->evR0));
+    HIP_TRY(hipEventCreate(&num->evR1));
+  }
+  HIP_TRY(hipEventRecord(num->evR0, num->stream));
+  Ctx c{num, n, num->stream, h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_values, num->small.as<double>(),
+        num->partial.as<double>()};
+  double* r = num->r.as<double>();
+  double* z = num->dvec.as<double>();
+  double* q = num->w.as<double>();
+  double* p = num->V.as<double>();
+  double bn = 0, rn = 0;
+  if ((rc = c.norm(d_b, &bn))) return rc;
+  if (!(bn != 0) || std::isnan(bn)) bn = 1;
+  if ((rc = c.residual(d_b, d_x, r, &rn))) return rc;
+  info->initialResidual = rn / bn;
+  double rel = rn / bn, rz = 0, pq = 0;
+  int32_t it = 0;
+  if (rel > o->tol && o->maxIter > 0) {
+    if ((rc = amp::solveOnDevice(num, r, z, 1))) return rc;
+    HIP_TRY(hipMemcpyAsync(p, z, vb, hipMemcpyDeviceToDevice, num->stream));
+    if ((rc = c.dots(r, 0, 1, z, &rz))) return rc;
+    for (it = 1; it <= o->maxIter; it++) {
+      amp::launchResidual(c.rp, c.ci, d_values, p, nullptr, q, n, num->stream);  // q = A p
+      if ((rc = c.dots(p, 0, 1, q, &pq))) return rc;
+      if (!(pq > 0)) break;  // the preconditioned operator is not positive on p: stop with the current iterate
+      const double alpha = rz / pq;
+      amp::launchAxpby(alpha, p, 1.0, d_x, n, num->stream);
+      amp::launchAxpby(-alpha, q, 1.0, r, n, num->stream);
+      if ((rc = c.norm(r, &rn))) return rc;
+      rel = rn / bn;
+      if (rel <= o->tol) break;
+      if ((rc = amp::solveOnDevice(num, r, z, 1))) return rc;
+      double rzNew = 0;
+      if ((rc = c.dots(r, 0, 1, z, &rzNew))) return rc;
+      amp::launchAxpby(1.0, z, rzNew / rz, p, n, num->stream);  // p = z + beta p
+      rz = rzNew;
+    }
+    it = std::min(it, o->maxIter);
+    if ((rc = c.residual(d_b, d_x, r, &rn))) return rc;  // true residual at exit
+    rel = rn / bn;
+  }
+  HIP_TRY(hipEventRecord(num->evR1, num->stream));
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, num->evR0, num->evR1));
+  info->iters = it;
+  info->relResidual = rel;
+  info->converged = rel <= o->tol;
+  info->ms = ms;
+  return AMP_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int amp_pcg_device(amp_num* num, const double* d_values, const double* d_b, double* d_x, const amp_pcg_opts* o,
+                   amp_pcg_info* info) {
+  if (!num || !o || !info || (num->sym->S.n > 0 && (!d_values || !d_b || !d_x)))
+    return fail(AMP_ERR_ARG, "amp_pcg_device: bad arguments");
+  *info = amp_pcg_info{};
+  if (num->sym->S.n == 0) {
+    info->converged = 1;
+    return AMP_OK;
+  }
+  return runPcg(num, d_values, d_b, d_x, o, info);
+}
+
 
 int amp_sym_set_pattern(amp_sym* h, const int32_t* rowptr, const int32_t* colidx) {
   if (!h || !rowptr || (h->S.n > 0 && !colidx)) return fail(AMP_ERR_ARG, "amp_sym_set_pattern: bad arguments");

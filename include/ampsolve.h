@@ -35,6 +35,7 @@ extern "C" {
 typedef struct amp_sym amp_sym; /* symbolic analysis + execution plan (host), device copy of the plan */
 typedef struct amp_num amp_num; /* numeric factor resident in HBM */
 typedef struct amp_graph amp_graph; /* captured factor + solve + refinement sequence of one amp_num */
+typedef struct amp_mlp amp_mlp;     /* warm-start network of the parametric sweep */
 
 typedef struct {
   int32_t n, nsuper, maxFront, nlevels;
@@ -150,13 +151,16 @@ int amp_sym_set_pattern(amp_sym *sym, const int32_t *rowptr, const int32_t *coli
  * diagonal block is unspecified, as in the reference) and D (n, LDL^T pivots; ignored for Cholesky; may be NULL). */
 int amp_download_factor(amp_num *num, double *Lx, double *D);
 
-/* Device time (ms) of the last factor and of the last solve, and the total of each kernel class of the last
- * factor when profiling is enabled (amp_num_set_profiling). `kernel` is a class name ("gemm_tile", "solve_fwd_flow",
+/* Device time (ms) of the last factor and of the last solve ("solve_total", always recorded), and the total of each
+ * kernel class of the last factor and solve when profiling is enabled (amp_num_set_profiling). `kernel` is a class name ("gemm_tile", "solve_fwd_flow",
  * ...) or a class name and an elimination-tree level ("gemm_tile@23") for that level's launches only. */
 int amp_num_set_profiling(amp_num *num, int32_t enable);
 int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, int32_t *launches);
 
 void amp_num_free(amp_num *num);
+/* Free the factor's update-matrix workspace (needed only while factoring; the next factor allocates it again), so
+ * that many factors can stay resident for solves only, as the parametric sweep's factor cache keeps them. */
+int amp_num_release_workspace(amp_num *num);
 
 /* Shape thresholds of the device plans (no counterpart in the reference, whose kernel choice is per front,
  * front-real.mjs:189-244): fronts with height m <= smallMax are factored and solved whole in LDS (64 threads when
@@ -186,6 +190,37 @@ int amp_graph_create(amp_num *num, const double *d_values, const double *d_b, do
 int amp_graph_launch(amp_graph *g);
 int amp_graph_sync(amp_graph *g, amp_factor_info *info);
 void amp_graph_free(amp_graph *g);
+
+/* Preconditioned conjugate gradients on the device (no counterpart in the reference; the parametric sweep's solver):
+ * A x = b with A given by d_values on the analysed CSR pattern, the preconditioner being the factor held by `num`
+ * (typically of a nearby matrix), starting from the guess in d_x; stops when |b - A x| / |b| <= tol or after maxIter
+ * iterations. FP64 vectors in HBM. */
+typedef struct {
+  int32_t maxIter;
+  double tol;
+} amp_pcg_opts;
+typedef struct {
+  int32_t iters, converged;
+  double relResidual, initialResidual, ms;
+} amp_pcg_info;
+int amp_pcg_device(amp_num *num, const double *d_values, const double *d_b, double *d_x, const amp_pcg_opts *opts,
+                   amp_pcg_info *info);
+
+/* Warm-start network (no counterpart in the reference): an FP32 multilayer perceptron nIn -> nHidden -> nHidden -> nOut
+ * (tanh hidden layers, linear output) with its training on the device. Parameters (flat, amp_mlp_param_count):
+ * W1 (nHidden x nIn, row-major), b1, W2 (nHidden x nHidden), b2, W3 (nOut x nHidden), b3. Activations are batch-major
+ * (batch x features, device pointers, batch <= maxBatch). amp_mlp_gradient forms the gradient of the mean squared
+ * error against the targets d_t into the handle (amp_mlp_get_grads copies it out) and returns the error;
+ * amp_mlp_adam takes one Adam step with it. */
+int amp_mlp_create(int32_t nIn, int32_t nHidden, int32_t nOut, int32_t maxBatch, amp_mlp **out);
+int64_t amp_mlp_param_count(const amp_mlp *mlp);
+int amp_mlp_set_params(amp_mlp *mlp, const float *params, int64_t count);
+int amp_mlp_get_params(amp_mlp *mlp, float *params, int64_t count);
+int amp_mlp_get_grads(amp_mlp *mlp, float *grads, int64_t count);
+int amp_mlp_forward(amp_mlp *mlp, const float *d_x, int32_t batch, float *d_y);
+int amp_mlp_gradient(amp_mlp *mlp, const float *d_x, const float *d_t, int32_t batch, double *loss);
+int amp_mlp_adam(amp_mlp *mlp, float lr, float beta1, float beta2, float eps);
+void amp_mlp_free(amp_mlp *mlp);
 
 /* Message of the last failure on the calling thread ("" when none). */
 const char *amp_last_error(void);
