@@ -152,7 +152,8 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   using V4 = typename Vec4<T>::type;
   constexpr int WRW = G == 128 ? 4 : 2, WCW = 2, NT = 64 * WRW * WCW;  // wave grid, threads
   constexpr int WTR = G / WRW, WTC = G / WCW, NBI = WTR / 16, NBJ = WTC / 16;
-  constexpr int KC = G == 128 ? 16 : 32, PA = G + 4;  // depth chunk, LDS row pitch
+  // depth chunk, LDS row pitch (FP64 128-tiles keep 16 so that two workgroups fit per CU)
+  constexpr int KC = (G == 128 && sizeof(T) == 8) ? 16 : 32, PA = G + 4;
   constexpr int LS = NT / G, NL = KC / LS;            // loader: depth stride, depths per thread
   __shared__ T As[KC * PA];
   __shared__ T Bs[KC * PA];

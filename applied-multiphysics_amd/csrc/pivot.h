@@ -34,17 +34,40 @@ __device__ inline T pivotValue(T d, int32_t col, const FactorArgs<T>& a, T* inv,
   return T(1);
 }
 
+// 1 / sqrt(d) and 1 / d from the hardware estimates plus Newton steps (y <- y (3/2 - d y^2 / 2), y <- y (2 - d y)):
+// within an ulp or two of the correctly rounded values at a fraction of the latency of the IEEE square root and
+// division, which sit on the serial chain of every elimination step of a diagonal block. Non-positive d gives NaN,
+// as sqrt does.
+__device__ inline double invSqrt(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = y * (1.5 - h * y * y);
+  return y * (1.5 - h * y * y);
+}
+__device__ inline float invSqrt(float d) {
+  const float y = __builtin_amdgcn_rsqf(d);
+  return y * (1.5f - 0.5f * d * y * y);
+}
+__device__ inline double recip(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  y = y * (2.0 - d * y);
+  return y * (2.0 - d * y);
+}
+__device__ inline float recip(float d) {
+  const float y = __builtin_amdgcn_rcpf(d);
+  return y * (2.0f - d * y);
+}
 
-// The same rule evaluated by every lane of a wave on a broadcast value; only the lane with `record` writes D, the
-// perturbation flag and the failure column (so the stores happen once).
+// The rule of pivotValue evaluated by every lane of a wave on a broadcast value (scaling from invSqrt / recip); only
+// the lane with `record` writes D, the perturbation flag and the failure column (so the stores happen once).
 template <typename T>
 __device__ inline T pivotRule(T d, int32_t col, const FactorArgs<T>& a, T* inv, T* mult, bool record) {
   if (!a.ldlt) {
     if (record && !(d > T(0))) atomicMin(a.failCol, col);
-    const T sq = sqrt(d);
-    *inv = T(1) / sq;
+    const T r = invSqrt(d);
+    *inv = r;
     *mult = T(1);
-    return sq;
+    return d * r;
   }
   const T tau = T(a.tau);
   uint8_t perturbed = 0;
@@ -60,7 +83,7 @@ __device__ inline T pivotRule(T d, int32_t col, const FactorArgs<T>& a, T* inv, 
     a.pert[col] = perturbed;
     a.D[col] = d;
   }
-  *inv = T(1) / d;
+  *inv = recip(d);
   *mult = d;
   return T(1);
 }
