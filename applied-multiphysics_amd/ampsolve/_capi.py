@@ -152,9 +152,17 @@ def check(rc):
 
 
 def ptr(a):
-    """Address of a numpy array or a torch tensor (device pointers pass through unchanged)."""
+    """Address of a numpy array or a torch tensor (device pointers pass through unchanged). The library's device
+    entry points order their work after the null stream; for a device tensor whose producer may still be queued on
+    another current torch stream, that stream is synchronised here."""
     if a is None:
         return None
     if hasattr(a, "data_ptr"):
+        if getattr(a, "is_cuda", False):
+            import torch
+
+            s = torch.cuda.current_stream(a.device)
+            if s.cuda_stream != 0:
+                s.synchronize()
         return C.c_void_p(a.data_ptr())
     return a.ctypes.data_as(C.c_void_p)

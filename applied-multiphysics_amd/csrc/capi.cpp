@@ -423,6 +423,7 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   HIP_TRY(hipStreamCreateWithFlags(&num->side, hipStreamNonBlocking));
   num->syncEv.resize(size_t(F.signals) + 1, nullptr);
   for (hipEvent_t& e : num->syncEv) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&num->inEv, hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&num->ev0));
   HIP_TRY(hipEventCreate(&num->ev1));
   HIP_TRY(num->Lx.alloc(size_t(F.nnzLStored) * es));
@@ -456,6 +457,7 @@ int amp_factor_device(amp_num* num, const double* d_values, int64_t nvalues, dou
     return fail(AMP_ERR_PATTERN, "factorNumeric: " + std::to_string(nvalues) + " values for a pattern with " +
                                      std::to_string(num->sym->S.nnzPattern) + " entries");
   num->ok = false;
+  HIP_TRY(amp::afterCaller(num->stream, num->inEv));
   return num->prec == AMP_F64 ? runFactor<double>(num, d_values, pivotTol, info)
                               : runFactor<float>(num, d_values, pivotTol, info);
 }
@@ -472,6 +474,7 @@ int amp_factor(amp_num* num, const double* values, int64_t nvalues, double pivot
 
 int amp_solve_device(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   if (!num || nrhs < 0 || (nrhs > 0 && num->sym->S.n > 0 && (!d_b || !d_x))) return fail(AMP_ERR_ARG, "amp_solve: bad arguments");
+  HIP_TRY(amp::afterCaller(num->stream, num->inEv));
   return runSolve(num, d_b, d_x, nrhs);
 }
 
@@ -618,6 +621,7 @@ int amp_graph_create(amp_num* num, const double* d_values, const double* d_b, do
   HIP_TRY(g->d.alloc(size_t(n) * nrhs * 8));
   HIP_TRY(hipEventCreate(&g->e0));
   HIP_TRY(hipEventCreate(&g->e1));
+  HIP_TRY(amp::afterCaller(num->stream, num->inEv));
   // one uncaptured pass first: kernel occupancy queries and first-use runtime work stay out of the capture
   rc = num->prec == AMP_F64 ? enqueueSequence<double>(g.get(), d_values, d_b, d_x, nrhs, 0)
                             : enqueueSequence<float>(g.get(), d_values, d_b, d_x, nrhs, 0);
@@ -636,6 +640,7 @@ int amp_graph_create(amp_num* num, const double* d_values, const double* d_b, do
 
 int amp_graph_launch(amp_graph* g) {
   if (!g) return fail(AMP_ERR_ARG, "amp_graph_launch: bad arguments");
+  HIP_TRY(amp::afterCaller(g->num->stream, g->num->inEv));
   HIP_TRY(hipEventRecord(g->e0, g->num->stream));
   HIP_TRY(hipGraphLaunch(g->exec, g->num->stream));
   HIP_TRY(hipEventRecord(g->e1, g->num->stream));
@@ -723,6 +728,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     return fail(AMP_ERR_ARG, "amp_sym_autotune: bad arguments");
   if (h->liveNums) return fail(AMP_ERR_STATE, "amp_sym_autotune: numeric handles of this analysis exist");
   if (h->S.n == 0) return fail(AMP_ERR_ARG, "amp_sym_autotune: empty system");
+  HIP_TRY(hipDeviceSynchronize());  // d_values complete, whichever stream produced it
   amp::PlanParams best;  // the defaults
   int rc = rebuildPlans(h, best);
   if (rc) return rc;

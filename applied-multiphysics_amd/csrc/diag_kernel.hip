@@ -3,20 +3,6 @@
 
 namespace amp {
 
-namespace {
-
-// Broadcast lane `l`'s value to the whole wave through a scalar register (l is a compile-time constant here).
-__device__ inline double laneValue(double v, int l) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-  return __hiloint2double(hi, lo);
-}
-__device__ inline float laneValue(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-}  // namespace
-
 // Four waves per block. Wave w owns the columns j = 4 jj + w of the block (lane i keeps A[i][j] in rA[jj]) and the
 // rows i = 4 ii + w of the inverse W = L^-1 (lane c keeps W[i][c] in rW[ii]), so at every elimination step the
 // columns and rows still active are spread evenly over the waves. The pivot step of column k (by its owner wave):

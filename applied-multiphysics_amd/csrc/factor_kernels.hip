@@ -88,6 +88,68 @@ __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ 
   }
 }
 
+// Fronts with m <= MB (16 or 32), one wave each and four per workgroup. The wave assembles its front in its own LDS
+// block as k_small_front does; then lane j takes column j of the symmetric front into registers (rows i < j from row
+// j of the stored lower triangle) and the nc elimination steps run there: the pivot column's entries reach every lane
+// through readlane, and lane j finds its own multiplier in its row k, so no step needs LDS or a barrier. Columns
+// return through LDS (lower triangle, upper zero) for coalesced stores of L and U.
+template <typename T, int MB>
+__global__ __launch_bounds__(256) void k_small_front_wave(const int32_t* __restrict__ fronts, int32_t count,
+                                                          FactorArgs<T> a, const double* __restrict__ values) {
+  __shared__ T Fs[4][MB * MB];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int64_t idx = int64_t(blockIdx.x) * 4 + w;
+  if (idx >= count) return;  // whole waves only: no workgroup barrier below
+  T* F = Fs[w];
+  const int32_t s = fronts[idx];
+  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
+  for (int i = lane; i < m * m; i += 64) F[i] = T(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int32_t q = a.lowerP[f] + lane; q < a.lowerP[f + nc]; q += 64) F[a.amap[q]] = T(0.0 + values[a.lowerSrc[q]]);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t* rel = a.rel + a.Sp[c] + ncc;
+    const T* Uc = a.U + a.Uoff[c];
+    for (int i = lane; i < muc * muc; i += 64) {
+      const int r = i % muc, b = i / muc;
+      if (r >= b) F[rel[r] + rel[b] * m] += Uc[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  const bool own = lane < m;
+  T cv[MB];
+#pragma unroll
+  for (int i = 0; i < MB; i++) cv[i] = (own && i < m) ? (i >= lane ? F[i + lane * m] : F[lane + i * m]) : T(0);
+#pragma unroll
+  for (int k = 0; k < MB; k++) {
+    if (k >= nc) continue;
+    T inv, mult;
+    const T piv = pivotRule<T>(laneValue(cv[k], k), f + k, a, &inv, &mult, lane == 0);
+    const T ljm = lane > k ? cv[k] * inv * mult : T(0);  // l_j d_k (row k of column j is F[j][k] by symmetry)
+    if (lane == k) cv[k] = piv;
+#pragma unroll
+    for (int i = k + 1; i < MB; i++) {
+      const T li = laneValue(cv[i], k) * inv;
+      cv[i] = lane == k ? li : cv[i] - li * ljm;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+  for (int i = 0; i < MB; i++)
+    if (own && i < m) F[i + lane * m] = i >= lane ? cv[i] : T(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  T* L = a.Lx + a.Lxp[s];
+  for (int i = lane; i < m * nc; i += 64) L[i] = F[i];
+  if (mu > 0) {
+    T* Us = a.U + a.Uoff[s];
+    for (int i = lane; i < mu * mu; i += 64) {
+      const int r = i % mu, b = i / mu;
+      Us[i] = F[(nc + r) + (nc + b) * m];
+    }
+  }
+}
+
 // Extend-add into parent columns [j0, j1) of a tiled front. Each child column maps to one parent column through the
 // increasing relative map, so the child columns of the range are contiguous; children are added one after another
 // (reference order), each child's entries land on distinct parent entries, so the waves split a child's columns.
@@ -296,6 +358,16 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         break;
       }
       case kSmall: {
+        if (L.threads == 0) {  // one wave per front
+          const unsigned wg = unsigned((L.count + 3) / 4);
+          if (L.maxM <= 16)
+            hipLaunchKernelGGL((k_small_front_wave<T, 16>), dim3(wg), dim3(256), 0, st, dp.smallFronts + L.first,
+                               int32_t(L.count), a, values);
+          else
+            hipLaunchKernelGGL((k_small_front_wave<T, kWaveFrontMax>), dim3(wg), dim3(256), 0, st,
+                               dp.smallFronts + L.first, int32_t(L.count), a, values);
+          break;
+        }
         const size_t lds = (size_t(L.maxM) * L.maxM + L.maxM) * sizeof(T);
         if (L.threads == 64)
           hipLaunchKernelGGL((k_small_front<T, 64>), dim3(grid), dim3(64), lds, st, dp.smallFronts + L.first, a, values,

@@ -22,6 +22,15 @@ int fail(int code, const std::string& msg);
     if (e_ != hipSuccess) return ::amp::fail(AMP_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+// The *_device entry points read device inputs that the caller may have produced with work still queued on the null
+// (legacy default) stream, the default stream of most frameworks: before its first launch, the handle's stream waits
+// on an event recorded there. Work on other streams of the caller must be complete before the call.
+inline hipError_t afterCaller(hipStream_t st, hipEvent_t ev) {
+  hipError_t e = hipEventRecord(ev, nullptr);
+  if (e == hipSuccess) e = hipStreamWaitEvent(st, ev, 0);
+  return e;
+}
+
 // Owning device allocation.
 struct DevBuf {
   void* p = nullptr;
@@ -83,6 +92,7 @@ struct amp_num {
   hipStream_t stream = nullptr, side = nullptr;  // side: bulk trailing updates overlapping the panel chain
   std::vector<hipEvent_t> syncEv;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evR0 = nullptr, evR1 = nullptr;
+  hipEvent_t inEv = nullptr;  // afterCaller
   amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv, pz;
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
@@ -102,6 +112,7 @@ struct amp_num {
     if (evR0) (void)hipEventDestroy(evR0);
     if (evR1) (void)hipEventDestroy(evR1);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (inEv) (void)hipEventDestroy(inEv);
     for (hipEvent_t e : syncEv) (void)hipEventDestroy(e);
     if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);

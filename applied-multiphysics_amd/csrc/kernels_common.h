@@ -41,6 +41,16 @@ __device__ inline int mfmaRow<float>(int lane, int r) {
   return 4 * (lane >> 4) + r;
 }
 
+// Broadcast lane `l`'s value to the whole wave through a scalar register (l must be the same in every lane).
+__device__ inline double laneValue(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float laneValue(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // Kernel-side view of the factor storage and of the plan's per-supernode arrays.
 template <typename T>
 struct FactorArgs {

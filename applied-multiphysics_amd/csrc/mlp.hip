@@ -180,10 +180,12 @@ struct amp_mlp {
   int32_t nIn = 0, nH = 0, nOut = 0, maxBatch = 0;
   int64_t nParams = 0, step = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t inEv = nullptr;  // afterCaller
   DevBuf params, grads, m, v, h1, h2, y, partial;
   DevBuf dh;       // dz2, dz1 (2 x maxBatch x nHidden)
   DevBuf scratch;  // depth slices of split products
   ~amp_mlp() {
+    if (inEv) (void)hipEventDestroy(inEv);
     if (stream) (void)hipStreamDestroy(stream);
   }
   // parameter offsets
@@ -230,6 +232,7 @@ int amp_mlp_create(int32_t nIn, int32_t nHidden, int32_t nOut, int32_t maxBatch,
   n->maxBatch = maxBatch;
   n->nParams = n->ob3() + nOut;
   HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&n->inEv, hipEventDisableTiming));
   const size_t pb = size_t(n->nParams) * 4;
   HIP_TRY(n->params.alloc(pb));
   HIP_TRY(n->grads.alloc(pb));
@@ -275,6 +278,7 @@ int amp_mlp_get_grads(amp_mlp* n, float* host, int64_t count) {
 
 int amp_mlp_forward(amp_mlp* n, const float* d_x, int32_t batch, float* d_y) {
   if (!n || !d_x || !d_y || batch < 1 || batch > n->maxBatch) return fail(AMP_ERR_ARG, "amp_mlp_forward: bad arguments");
+  HIP_TRY(amp::afterCaller(n->stream, n->inEv));
   forward(n, d_x, batch, d_y);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(n->stream));
@@ -287,6 +291,7 @@ int amp_mlp_gradient(amp_mlp* n, const float* d_x, const float* d_t, int32_t B, 
   using amp::Mat;
   if (!n || !d_x || !d_t || B < 1 || B > n->maxBatch) return fail(AMP_ERR_ARG, "amp_mlp_gradient: bad arguments");
   hipStream_t st = n->stream;
+  HIP_TRY(amp::afterCaller(st, n->inEv));
   float* y = n->y.as<float>();
   forward(n, d_x, B, y);
   const int64_t ny = int64_t(B) * n->nOut;

@@ -123,17 +123,19 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
 
   for (int32_t L = 0; L < P.nlevels; L++) {
     const std::vector<int32_t>& fronts = byLevel[L];
-    // fronts factored in LDS, two size classes (workgroup of 64 threads up to 32 rows, else 256)
-    for (int cls = 0; cls < 2; cls++) {
+    // fronts factored in LDS, four size classes: one wave per front up to 16 rows and up to kWaveFrontMax rows, then
+    // a workgroup of 64 threads up to smallNarrow rows, else 256
+    for (int cls = 0; cls < 4; cls++) {
       const int64_t first = int64_t(P.smallFronts.size());
       int32_t maxM = 0;
       for (int32_t s : fronts) {
         const int32_t m = frontRows(S, s);
-        if (P.isBig[s] || (cls == 0) != (m <= prm.smallNarrow)) continue;
+        const int c = m <= 16 ? 0 : m <= kWaveFrontMax ? 1 : m <= prm.smallNarrow ? 2 : 3;
+        if (P.isBig[s] || c != cls) continue;
         P.smallFronts.push_back(s);
         maxM = std::max(maxM, m);
       }
-      push(kSmall, L, first, int64_t(P.smallFronts.size()) - first, maxM, cls == 0 ? 64 : 256);
+      push(kSmall, L, first, int64_t(P.smallFronts.size()) - first, maxM, cls < 2 ? 0 : cls == 2 ? 64 : 256);
     }
     std::vector<int32_t> big;
     for (int32_t s : fronts)
@@ -325,22 +327,28 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     P.launches.push_back(SolveLaunch{kind, level, first, count, maxM});
   };
   auto nblocks = [&S](int32_t s) { return (frontCols(S, s) + kTileEdge - 1) / kTileEdge; };
+  // the whole-front supernodes of a level: those with m <= kSolveWaveMax (one wave each) and the rest, one launch each
+  auto pushSmall = [&](SolveKind kind, int32_t L) {
+    for (int pass = 0; pass < 2; pass++) {
+      const int64_t first = int64_t(P.fronts.size());
+      int32_t maxM = 0;
+      for (int32_t s : byLevel[L]) {
+        const int32_t m = frontRows(S, s);
+        if (F.isBig[s] || (m <= kSolveWaveMax) != (pass == 0)) continue;
+        P.fronts.push_back(s);
+        maxM = std::max(maxM, m);
+      }
+      push(kind, L, first, int64_t(P.fronts.size()) - first, maxM);
+    }
+  };
   // forward, leaves first
   for (int32_t L = 0; L < F.nlevels; L++) {
-    int64_t first = int64_t(P.fronts.size());
-    int32_t maxM = 0;
     std::vector<int32_t> big;
-    for (int32_t s : byLevel[L]) {
-      if (F.isBig[s]) {
-        big.push_back(s);
-        continue;
-      }
-      P.fronts.push_back(s);
-      maxM = std::max(maxM, frontRows(S, s));
-    }
-    push(kSolveSmallFwd, L, first, int64_t(P.fronts.size()) - first, maxM);
+    for (int32_t s : byLevel[L])
+      if (F.isBig[s]) big.push_back(s);
+    pushSmall(kSolveSmallFwd, L);
     if (big.empty()) continue;
-    first = int64_t(P.fronts.size());
+    int64_t first = int64_t(P.fronts.size());
     P.fronts.insert(P.fronts.end(), big.begin(), big.end());
     push(kSolveGather, L, first, int64_t(big.size()));
     // every task depends only on tasks listed before it: own tiles by block index across the level's supernodes,
@@ -364,19 +372,11 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
   // backward, root first; blocks of each supernode from the last one down
   for (int32_t L = F.nlevels - 1; L >= 0; L--) {
     std::vector<int32_t> big;
-    int64_t first = int64_t(P.fronts.size());
-    int32_t maxM = 0;
-    for (int32_t s : byLevel[L]) {
-      if (F.isBig[s]) {
-        big.push_back(s);
-        continue;
-      }
-      P.fronts.push_back(s);
-      maxM = std::max(maxM, frontRows(S, s));
-    }
-    push(kSolveSmallBwd, L, first, int64_t(P.fronts.size()) - first, maxM);
+    for (int32_t s : byLevel[L])
+      if (F.isBig[s]) big.push_back(s);
+    pushSmall(kSolveSmallBwd, L);
     if (big.empty()) continue;
-    first = int64_t(P.fronts.size());
+    int64_t first = int64_t(P.fronts.size());
     P.fronts.insert(P.fronts.end(), big.begin(), big.end());
     push(kSolveBwdGather, L, first, int64_t(big.size()));
     // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk)

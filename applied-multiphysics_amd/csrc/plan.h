@@ -19,6 +19,7 @@ constexpr int kWideLaunch = 512;  // tiles of edge kGemmEdge a launch needs to u
 constexpr int kEaWidth = 16;    // parent columns per extend-add work item
 constexpr int kSuperPanel = 256;  // columns brought up to date together (left-looking) in a tiled front
 constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
+constexpr int kWaveFrontMax = 32;  // fronts with m <= this are factored by one wave each (four per workgroup)
 constexpr int kZeroChunk = 1 << 18;
 
 enum Buf : int8_t { kBufLx = 0, kBufU = 1, kBufW = 2 };
@@ -64,7 +65,7 @@ struct Launch {
   int32_t level;
   int64_t first, count;  // range in the task array of this kind (kSmall: in smallFronts; kAssemble: lower entries)
   int32_t maxM;          // kSmall: largest front of the launch (LDS sizing)
-  int32_t threads;       // kSmall: workgroup size
+  int32_t threads;       // kSmall: workgroup size, or 0 for one wave per front (maxM <= kWaveFrontMax)
   int32_t edge;          // kGemm: output-tile edge of the launch's tasks (kTileEdge or kGemmEdge)
   int32_t sync;          // kSyncSide | kSyncWait | kSyncSignal
 };
@@ -101,7 +102,8 @@ struct FactorPlan {
 // choose others (each at most its constant, which sizes the kernels' static storage).
 struct PlanParams {
   int32_t smallMax = kSmallMax;       // height m at or below which a front is factored and solved whole in LDS
-  int32_t smallNarrow = 32;           // height at or below which a whole-front workgroup has 64 threads (else 256)
+  int32_t smallNarrow = 32;           // height at or below which a whole-front workgroup has 64 threads (else 256);
+                                      // fronts of at most kWaveFrontMax rows take one wave each regardless
   int32_t wideLaunch = kWideLaunch;   // tiles a product launch needs to use kGemmEdge tiles
   int32_t superPanel = kSuperPanel;   // columns of a super-panel (a multiple of kTileEdge)
 };
@@ -117,6 +119,7 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams
 // (backward) waits on per-block completion flags of the blocks it consumes, so the block chain of a supernode runs
 // inside one launch.
 constexpr int kFlowRhs = 8;       // right-hand sides per pass of the tiled-supernode solve kernels
+constexpr int kSolveWaveMax = 64;  // whole-front supernodes with m <= this are solved by one wave each
 constexpr int kInitRows = 256;    // below rows per task of the backward z = -L21^T y[below] partial products
 
 // One task of a tiled-supernode solve launch. Forward: rows [r0, r1) of supernode s after consuming the solutions of

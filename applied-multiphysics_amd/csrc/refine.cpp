@@ -291,6 +291,7 @@ int amp_pcg_device(amp_num* num, const double* d_values, const double* d_b, doub
     info->converged = 1;
     return AMP_OK;
   }
+  HIP_TRY(amp::afterCaller(num->stream, num->inEv));
   return runPcg(num, d_values, d_b, d_x, o, info);
 }
 
@@ -315,6 +316,7 @@ int amp_refine_device(amp_num* num, const double* d_values, const double* d_b, d
     info->converged = 1;
     return AMP_OK;
   }
+  HIP_TRY(amp::afterCaller(num->stream, num->inEv));
   return refineDevice(num, d_values, d_b, d_x, o, info);
 }
 

@@ -94,6 +94,79 @@ __global__ __launch_bounds__(256) void k_solve_small_fwd(const int32_t* __restri
   }
 }
 
+// Supernodes with m <= kSolveWaveMax, one wave each and four per workgroup: lane r keeps row r of the front vector
+// of every right-hand side in registers, each column's solution value is broadcast with readlane, so the column loop
+// needs no barrier and no LDS; the columns of L are loaded eight at a time, one group ahead.
+constexpr int kWaveGroup = 8;
+
+// out[j] = p[(c0 + dir j) stride] for the indices in [0, n) when `valid`, else 0: one group of independent loads.
+template <typename T>
+__device__ inline void loadGroup(const T* p, int64_t stride, int32_t c0, int dir, int32_t n, bool valid,
+                                 T (&out)[kWaveGroup]) {
+#pragma unroll
+  for (int j = 0; j < kWaveGroup; j++) {
+    const int32_t c = c0 + dir * j;
+    out[j] = (valid && c >= 0 && c < n) ? p[int64_t(c) * stride] : T(0);
+  }
+}
+
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_wave_fwd(const int32_t* __restrict__ fronts, int32_t count,
+                                                        SolveArgs<T> a, S* y, S* fv, int32_t q, int64_t fvN) {
+  __shared__ S v[4][Q][kSolveWaveMax];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int64_t idx = int64_t(blockIdx.x) * 4 + w;
+  if (idx >= count) return;  // whole waves only: no workgroup barrier below
+  const int32_t s = fronts[idx];
+  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
+  const T* L = a.Lx + a.Lxp[s];
+#pragma unroll
+  for (int k = 0; k < Q; k++)
+    if (k < q) v[w][k][lane] = lane < nc ? y[int64_t(k) * a.n + f + lane] : S(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t o = a.Sp[c] + ncc;
+    if (lane < muc) {
+      const int32_t r = a.rel[o + lane];
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) v[w][k][r] += fv[k * fvN + o + lane];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  S xr[Q];
+#pragma unroll
+  for (int k = 0; k < Q; k++) xr[k] = k < q ? v[w][k][lane] : S(0);
+  T cur[kWaveGroup], nxt[kWaveGroup];
+  const bool rowValid = lane < m;
+  loadGroup(L + lane, m, 0, 1, nc, rowValid, cur);
+  for (int32_t c0 = 0; c0 < nc; c0 += kWaveGroup) {
+    if (c0 + kWaveGroup < nc) loadGroup(L + lane, m, c0 + kWaveGroup, 1, nc, rowValid, nxt);
+#pragma unroll
+    for (int j = 0; j < kWaveGroup; j++) {
+      const int32_t col = c0 + j;
+      if (col >= nc) break;
+      const S l = lane > col ? S(cur[j]) : S(0);
+      const S d = a.ldlt ? S(1) : S(laneValue(cur[j], col));
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        if (k >= q) continue;
+        const S x = a.ldlt ? laneValue(xr[k], col) : laneValue(xr[k], col) / d;
+        xr[k] = lane == col ? x : xr[k] - l * x;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kWaveGroup; j++) cur[j] = nxt[j];
+  }
+#pragma unroll
+  for (int k = 0; k < Q; k++) {
+    if (k >= q) continue;
+    if (lane < nc) y[int64_t(k) * a.n + f + lane] = xr[k];
+    if (lane < m) fv[k * fvN + a.Sp[s] + lane] = xr[k];
+  }
+}
+
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y, S* fv,
                                                       int32_t q, int64_t fvN) {
@@ -331,6 +404,69 @@ __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restri
     for (int i = threadIdx.x; i < nc; i += blockDim.x) y[int64_t(k) * a.n + f + i] = z[k][i];
 }
 
+// Backward counterpart of k_solve_wave_fwd: lane i keeps solution row i of the supernode; the below rows' values are
+// held by lanes 0..mu-1 and broadcast with readlane, lane i walks column i of L.
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_wave_bwd(const int32_t* __restrict__ fronts, int32_t count,
+                                                        SolveArgs<T> a, S* y, int32_t q) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int64_t idx = int64_t(blockIdx.x) * 4 + w;
+  if (idx >= count) return;
+  const int32_t s = fronts[idx];
+  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
+  const T* L = a.Lx + a.Lxp[s];
+  const int32_t* rows = a.Si + a.Sp[s] + nc;
+  S xb[Q], z[Q];
+  const int32_t br = lane < mu ? rows[lane] : 0;
+#pragma unroll
+  for (int k = 0; k < Q; k++) {
+    xb[k] = (k < q && lane < mu) ? y[int64_t(k) * a.n + br] : S(0);
+    z[k] = (k < q && lane < nc) ? y[int64_t(k) * a.n + f + lane] : S(0);
+  }
+  // z_i -= sum_r L[nc + r][i] xb_r: lane i walks the below rows of its column i
+  const bool colValid = lane < nc;
+  const T* Li = L + int64_t(colValid ? lane : 0) * m;
+  T cur[kWaveGroup], nxt[kWaveGroup];
+  loadGroup(Li + nc, 1, 0, 1, mu, colValid, cur);
+  for (int32_t r0 = 0; r0 < mu; r0 += kWaveGroup) {
+    if (r0 + kWaveGroup < mu) loadGroup(Li + nc, 1, r0 + kWaveGroup, 1, mu, colValid, nxt);
+#pragma unroll
+    for (int j = 0; j < kWaveGroup; j++) {
+      const int32_t r = r0 + j;
+      if (r >= mu) break;
+      const S l = S(cur[j]);
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) z[k] -= l * laneValue(xb[k], r);
+    }
+#pragma unroll
+    for (int j = 0; j < kWaveGroup; j++) cur[j] = nxt[j];
+  }
+  // in-block backward substitution, last column first: lane i < col reads L[col][i] from its column i
+  loadGroup(Li, 1, nc - 1, -1, nc, colValid, cur);
+  for (int32_t c0 = nc - 1; c0 >= 0; c0 -= kWaveGroup) {
+    if (c0 - kWaveGroup >= 0) loadGroup(Li, 1, c0 - kWaveGroup, -1, nc, colValid, nxt);
+#pragma unroll
+    for (int j = 0; j < kWaveGroup; j++) {
+      const int32_t col = c0 - j;
+      if (col < 0) break;
+      const S l = lane < col ? S(cur[j]) : S(0);
+      const S d = a.ldlt ? S(1) : S(laneValue(cur[j], col));
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        if (k >= q) continue;
+        const S x = a.ldlt ? laneValue(z[k], col) : laneValue(z[k], col) / d;
+        z[k] = lane == col ? x : z[k] - l * x;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kWaveGroup; j++) cur[j] = nxt[j];
+  }
+#pragma unroll
+  for (int k = 0; k < Q; k++)
+    if (k < q && lane < nc) y[int64_t(k) * a.n + f + lane] = z[k];
+}
+
 // y[below] of each tiled supernode of a level into the below part of its front vector (contiguous for the backward
 // flow).
 template <typename T, typename S>
@@ -537,7 +673,15 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
     }
     switch (L.kind) {
       case kSolveSmallFwd:
-        if (q == 1)
+        if (L.maxM <= kSolveWaveMax) {
+          const unsigned wg = unsigned((L.count + 3) / 4);
+          if (q == 1)
+            hipLaunchKernelGGL((k_solve_wave_fwd<T, S, 1>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
+                               int32_t(L.count), a, y, fv, q, dp.fvEntries);
+          else
+            hipLaunchKernelGGL((k_solve_wave_fwd<T, S, kFlowRhs>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
+                               int32_t(L.count), a, y, fv, q, dp.fvEntries);
+        } else if (q == 1)
           hipLaunchKernelGGL((k_solve_small_fwd<T, S, 1>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
                              dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         else
@@ -555,7 +699,15 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                                   y, fv, q, dp.fvEntries, fc);
         break;
       case kSolveSmallBwd:
-        if (q == 1)
+        if (L.maxM <= kSolveWaveMax) {
+          const unsigned wg = unsigned((L.count + 3) / 4);
+          if (q == 1)
+            hipLaunchKernelGGL((k_solve_wave_bwd<T, S, 1>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
+                               int32_t(L.count), a, y, q);
+          else
+            hipLaunchKernelGGL((k_solve_wave_bwd<T, S, kFlowRhs>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
+                               int32_t(L.count), a, y, q);
+        } else if (q == 1)
           hipLaunchKernelGGL((k_solve_small_bwd<T, S, 1>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
                              dp.fronts + L.first, a, y, q);
         else

@@ -102,6 +102,11 @@ void amp_sym_free(amp_sym *sym);
 
 /* ---- numeric phase (device) --------------------------------------------------------------------------------- */
 
+/* Device pointers (the *_device calls, amp_graph_*, amp_pcg_device, amp_mlp_forward / amp_mlp_gradient): each call
+ * orders its work after everything queued on the null (legacy default) stream before it; inputs produced on any
+ * other stream must be complete before the call. The calls return with their outputs complete (amp_graph_launch:
+ * after amp_graph_sync). */
+
 /* Storage of one numeric factor in HBM (the NumericFactor / SupernodalLDLT storage of sparse.mjs:946-949 and
  * ldlt.mjs:428-448): mode AMP_CHOLESKY or AMP_LDLT, precision AMP_F64 or AMP_F32. Builds the device plan on first
  * use of the analysis. */

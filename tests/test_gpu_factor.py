@@ -237,3 +237,27 @@ def test_autotune_keeps_parity(device):
     sym.set_params({"smallMax": 64, "smallNarrow": 32, "wideLaunch": 256, "superPanel": 128})
     num = amp.factorNumeric(sym, v)
     assert num.ok and relmax(amp.solveNumeric(num, b), xo) < 1e-12
+
+
+def test_inputs_ordered_after_the_callers_stream(device):
+    """Inputs whose producing kernels are still queued on torch's stream when the call is made (behind ~1 TFLOP of
+    unrelated products) must be read only after they are written: same bits as with completed inputs."""
+    import torch
+
+    pat, v, b = W.heat3d(12)
+    sym = amp.analyze(pat)
+    num = amp.NumericFactor(sym, 0, "hip")
+    dv = torch.from_numpy(v).to(device)
+    db = torch.from_numpy(b).to(device)
+    assert num.factor(dv)
+    L1, _ = num.download()
+    x1 = num.solve(db).cpu().numpy()
+    torch.cuda.synchronize()
+    g = torch.randn(4096, 4096, device=device, generator=torch.Generator(device).manual_seed(3))
+    for _ in range(8):
+        g = (g @ g) / 64.0
+    late = 1.0 + 0.0 * g[0, 0].double()  # exactly 1, available only after the products
+    assert num.factor(dv * late)
+    L2, _ = num.download()
+    x2 = num.solve(db * late).cpu().numpy()
+    assert np.array_equal(L1, L2) and np.array_equal(x1, x2)
