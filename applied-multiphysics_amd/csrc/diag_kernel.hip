@@ -8,53 +8,55 @@ namespace amp {
 // columns and rows still active are spread evenly over the waves. The pivot step of column k (by its owner wave):
 // read the pivot from lane k (every lane evaluates the pivot rule on the broadcast value, lane k records it), scale
 // column k, finish row k of W (Gauss-Jordan: W[k][:] *= 1 / l_kk) and publish both through LDS (double-buffered by
-// step parity, one barrier per step). The update of step k: every wave applies the rank-1 update to its active
-// columns (A[i][j] -= l_ik l_jk d, l_jk read from the column's own lane through a scalar register) and rows of W
+// step parity, one barrier per step); the column is published twice, as l and as l d, both zero at and above the
+// pivot row. The update of step k: every wave applies the rank-1 update to its active columns
+// (A[i][j] -= (l_ik d) l_jk, l_jk read from the column's own lane through a scalar register; rows at or above the
+// pivot receive zero, and rows above the diagonal of column j are never read or stored) and rows of W
 // (W[i][:] -= l_jk W[k][:]). Lookahead: the owner of column k + 1 updates that column first and runs its pivot step
 // while the other waves finish the update of step k. The step loop is unrolled, so every register index is a
 // constant.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
   constexpr int E = kTileEdge, P = kTileEdge + 1, J = kTileEdge / 4;
-  __shared__ T colL[2][E], rowW[2][E];
-  __shared__ T s_mult[2];
+  __shared__ T colL[2][E], colD[2][E], rowW[2][E];
   __shared__ T Ws[E * P];
   const DiagTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63, nb = t.nb;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index, held in a scalar register
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   T* F = a.Lx + t.f_off;
   T rA[J], rW[J];
 #pragma unroll
   for (int jj = 0; jj < J; jj++) {
     const int j = 4 * jj + w;
     rA[jj] = (j < nb && lane < nb && lane >= j) ? F[lane + int64_t(j) * t.ld] : T(0);
-    rW[jj] = j == lane ? T(1) : T(0);  // row j of W
+    rW[jj] = j == lane ? T(1) : T(0);
   }
   auto pivotStep = [&](const int k) {
     const int kk = k >> 2, p = k & 1;
     T inv, mult;
     const T dk = laneValue(rA[kk], k);
     const T piv = pivotRule<T>(dk, t.col0 + k, a, &inv, &mult, lane == k);
+    const T l = lane > k ? rA[kk] * inv : T(0);
     if (lane == k) rA[kk] = piv;
-    if (lane > k) rA[kk] *= inv;
+    if (lane > k) rA[kk] = l;
     const T wk = a.ldlt ? rW[kk] : rW[kk] * inv;
     rW[kk] = wk;
-    colL[p][lane] = rA[kk];
+    colL[p][lane] = l;
+    colD[p][lane] = l * mult;
     rowW[p][lane] = wk;
-    if (lane == 0) s_mult[p] = mult;
   };
   if (nb > 0 && w == 0) pivotStep(0);
   __syncthreads();
 #pragma clang loop unroll(full)
   for (int k = 0; k < E; k++) {
-    if (k >= nb) continue;  // uniform
+    if (k >= nb) continue;
     const int p = k & 1, k1 = k + 1;
-    const T cl = colL[p][lane], rw = rowW[p][lane], mult = s_mult[p];
+    const T cl = colL[p][lane], cd = colD[p][lane], rw = rowW[p][lane];
     const bool ahead = k1 < nb && w == (k1 & 3);
     if (k1 < E && ahead) {
       const int jj = k1 >> 2;
       const T ljk = laneValue(cl, k1);
-      if (lane >= k1) rA[jj] -= cl * (ljk * mult);
+      rA[jj] -= cd * ljk;
       rW[jj] -= ljk * rw;
       pivotStep(k1);
     }
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
       const int j = 4 * jj + w;
       if (4 * jj + 3 > k && j > k && !(ahead && j == k1)) {
         const T ljk = laneValue(cl, j);
-        if (lane >= j) rA[jj] -= cl * (ljk * mult);
+        rA[jj] -= cd * ljk;
         rW[jj] -= ljk * rw;
       }
     }
@@ -73,10 +75,10 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
   for (int jj = 0; jj < J; jj++) {
     const int j = 4 * jj + w;
     if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = rA[jj];
-    Ws[j * P + lane] = (j < nb && lane < nb) ? rW[jj] : T(0);  // W[j][lane]
+    Ws[j * P + lane] = (j < nb && lane < nb) ? rW[jj] : T(0);
   }
   __syncthreads();
-  T* Wt = a.W + t.w_off;  // column-major: Wt[i + c E] = W[i][c]
+  T* Wt = a.W + t.w_off;
   for (int idx = threadIdx.x; idx < E * E; idx += 256) Wt[idx] = Ws[(idx & 63) * P + (idx >> 6)];
 }
 

@@ -205,7 +205,8 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
 // 128-tile. The matrix cores are fed the transposed product (operand roles of A and B swapped), so the accumulator
 // components of a lane hold 16 consecutive rows of one column of C and the epilogue reads and writes C straight from
 // registers along columns. The depth is staged through LDS in chunks of KC (A and B depth-major, one LDS read per
-// operand fragment); the next chunk is loaded into registers while the current one feeds the matrix cores.
+// operand fragment); the next chunk is loaded into registers while the current one feeds the matrix cores. 128-tiles
+// double-buffer the chunks in LDS (one barrier per chunk).
 // Workgroups are dispatched round-robin over the eight XCDs; each XCD is given a contiguous run of the task list,
 // whose neighbouring tiles share operand panels in that XCD's L2.
 template <typename T, int G>
@@ -217,8 +218,9 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   // depth chunk, LDS row pitch (FP64 128-tiles keep 16 so that two workgroups fit per CU)
   constexpr int KC = (G == 128 && sizeof(T) == 8) ? 16 : 32, PA = G + 4;
   constexpr int LS = NT / G, NL = KC / LS;            // loader: depth stride, depths per thread
-  __shared__ T As[KC * PA];
-  __shared__ T Bs[KC * PA];
+  constexpr int NBUF = G == 128 ? 2 : 1;
+  __shared__ T As[NBUF][KC * PA];
+  __shared__ T Bs[NBUF][KC * PA];
   const int nblk = gridDim.x, q8 = nblk / 8, r8 = nblk % 8, x8 = blockIdx.x % 8;
   const GemmTask t = tasks[x8 * q8 + min(x8, r8) + blockIdx.x / 8];
   T* const bases[3] = {Lx, U, W};
@@ -248,31 +250,51 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
       pb[r] = bv;
     }
   };
-  load(0);
-  for (int k0 = 0; k0 < t.k; k0 += KC) {
+  auto stash = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < NL; r++) {
-      As[(lk + LS * r) * PA + li] = pa[r];
-      Bs[(lk + LS * r) * PA + li] = pb[r];
+      As[buf][(lk + LS * r) * PA + li] = pa[r];
+      Bs[buf][(lk + LS * r) * PA + li] = pb[r];
     }
-    __syncthreads();
-    if (k0 + KC < t.k) load(k0 + KC);
-    if (active) {
+  };
+  auto compute = [&](int buf) {
+    if (!active) return;
 #pragma unroll
-      for (int k4 = 0; k4 < KC; k4 += 4) {
-        const int kk = (k4 + (lane >> 4)) * PA + (lane & 15);
-        T af[NBI], bf[NBJ];
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      const int kk = (k4 + (lane >> 4)) * PA + (lane & 15);
+      T af[NBI], bf[NBJ];
 #pragma unroll
-        for (int b = 0; b < NBI; b++) af[b] = As[kk + wr + 16 * b];
+      for (int b = 0; b < NBI; b++) af[b] = As[buf][kk + wr + 16 * b];
 #pragma unroll
-        for (int b = 0; b < NBJ; b++) bf[b] = Bs[kk + wc + 16 * b];
+      for (int b = 0; b < NBJ; b++) bf[b] = Bs[buf][kk + wc + 16 * b];
 #pragma unroll
-        for (int bi = 0; bi < NBI; bi++)
+      for (int bi = 0; bi < NBI; bi++)
 #pragma unroll
-          for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = mfma16x16x4(bf[bj], af[bi], acc[bi][bj]);
-      }
+        for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = mfma16x16x4(bf[bj], af[bi], acc[bi][bj]);
     }
+  };
+  load(0);
+  if constexpr (NBUF == 2) {
+    // double-buffered: chunk c is read from buffer c & 1 while chunk c + 1 is written to the other; one barrier
+    stash(0);
     __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < t.k; k0 += KC) {
+      const bool more = k0 + KC < t.k;
+      if (more) load(k0 + KC);
+      compute(buf);
+      if (more) stash(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  } else {
+    for (int k0 = 0; k0 < t.k; k0 += KC) {
+      stash(0);
+      __syncthreads();
+      if (k0 + KC < t.k) load(k0 + KC);
+      compute(0);
+      __syncthreads();
+    }
   }
   if (!active) return;
   // component r of acc[bi][bj]: C row wr + 16 bi + (lane & 15), C column wc + 16 bj + mfmaRow(lane, r)

@@ -20,6 +20,7 @@ sys.path.insert(0, os.path.join(ROOT, "applied-multiphysics_amd"))
 
 METRIC = "factor+solve s & TFLOP/s vs peak, 3D P1 heat 61³ nodes, FP64 vs mixed @1e-10 resid"
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (TFLOP/s): FP64, FP32
+PEAK_HBM = 8000.0  # MI355X HBM3E (GB/s)
 KERNEL_CLASSES = ("zero", "assemble", "small_front", "extend_add", "diag_block", "gemm_tile", "solve_small_fwd",
                   "solve_gather", "solve_fwd_flow", "solve_small_bwd", "solve_bwd_gather", "solve_bwd_init", "solve_bwd_flow", "solve_total")
 
@@ -143,6 +144,8 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir",
         kinds = {k: num.kernel_ms(k) for k in KERNEL_CLASSES}
         num.set_profiling(False)
         gm, gn = kinds["gemm_tile"]
+        st_ms, st_n = kinds["solve_total"]
+        res["solve_ms_per_pass"] = st_ms / st_n if st_n else 0.0
         res.update({"factor_ms_profiled": float(num.info["ms"]), "gemm_ms_per_step": gm, "gemm_launches_per_step": gn,
                     "kernel_ms_profiled_step": {k: v[0] for k, v in kinds.items()}})
         log(f"{log_prefix}: {1e3 * dt:.2f} ms/step; profiled step by class: " +
@@ -151,6 +154,27 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir",
         log(f"{log_prefix}: {1e3 * dt:.2f} ms/step (captured graph)")
     del g
     return res, num
+
+
+def hbm_bytes(sym, w, wv, q=1, small_max=96):
+    """Algorithmic bytes of one solve pass pair (forward + backward) and of the extend-adds into tiled fronts:
+    solve B = 2 w sum_s (nc m - nc (nc - 1) / 2) (L read once per direction) + 2 (4 sum_s mu) (row indices)
+    + q 4 wv n (right-hand side in, solution out, two sweeps); extend-add sum over children of tiled fronts
+    (m > small_max) of w 3 mu (mu + 1) / 2 (read the update, read and write the parent entries). w: factor bytes
+    per entry, wv: solve-vector bytes per entry."""
+    import numpy as np
+
+    ss = np.asarray(sym.superStart, np.int64)
+    Sp = np.asarray(sym.Sp, np.int64)
+    nc, m = np.diff(ss), np.diff(Sp)
+    mu = m - nc
+    solve = 2 * w * int(np.sum(nc * m - nc * (nc - 1) // 2)) + 2 * 4 * int(mu.sum()) + q * 4 * wv * int(sym.n)
+    par = np.asarray(sym.superParent, np.int64)
+    has = par >= 0
+    big_parent = np.zeros(len(m), bool)
+    big_parent[has] = m[par[has]] > small_max
+    ea = int(np.sum(w * 3 * mu[big_parent] * (mu[big_parent] + 1) // 2))
+    return solve, ea
 
 
 def main():
@@ -248,6 +272,19 @@ def main():
                      "launches_per_step": prof.get("gemm_launches_per_step"), "flops_per_step": sym.gemmFlops,
                      "measured_on": f"profiled step of the {prec} path (HIP events around each launch)"},
     }
+    # achieved HBM rates of the scatter (extend-add into tiled fronts) and of the solve pass, same profiled step
+    w = 8 if prec == "fp64" else 4
+    ks = prof.get("kernel_ms_profiled_step", {})
+    solve_b, ea_b = hbm_bytes(sym, w, w)
+    hbm = {"peak": PEAK_HBM, "unit": "GB/s", "measured_on": result["roofline"]["measured_on"]}
+    if ks.get("extend_add", 0) > 0:
+        gbs = ea_b / (ks["extend_add"] / 1e3) / 1e9
+        hbm["extend_add"] = {"bytes": ea_b, "ms": ks["extend_add"], "achieved": gbs, "frac": gbs / PEAK_HBM}
+    sm = prof.get("solve_ms_per_pass", 0.0)
+    if sm > 0:
+        gbs = solve_b / (sm / 1e3) / 1e9
+        hbm["solve"] = {"bytes": solve_b, "ms": sm, "achieved": gbs, "frac": gbs / PEAK_HBM}
+    result["hbm"] = hbm
     if args.cpu_flops > 0:
         log("cpu baseline sample ...")
         result["cpu_baseline"] = cpu_baseline(pat, values, args.cpu_flops)
