@@ -158,7 +158,8 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     // column is updated from it on the side stream (bulk), concurrently with the next super-panel's factorization.
     // Ordering: the bulk update of super-panel s waits for s to be factored (signal after its last step); the
     // lookahead update from s + 1 waits for the bulk update of s (the side stream's last signal at that point), which
-    // also wrote the columns of s + 2; the Schur complement waits for the last bulk update.
+    // also wrote the columns of s + 2; the update-matrix extend-add after the level's last super-panel joins the side
+    // stream (the bulk launches also accumulate the Schur complement, super-panel by super-panel).
     int32_t maxNc = 0;
     for (int32_t s : big) maxNc = std::max(maxNc, frontCols(S, s));
     bool sideUsed = false;
@@ -181,6 +182,30 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
           t.d_off = f + d0;
           t.a_buf = t.b_buf = t.c_buf = kBufLx;
           t.mode = 0;
+          out.push_back(t);
+        }
+    };
+    // U[i0.., j0..] (op)= -L21[., d0 : d0 + depth] D L21[., d0 : d0 + depth]^T over the lower tiles of the mu x mu
+    // update matrix of front s: written by the first super-panel (mode 1), accumulated by the later ones (mode 0)
+    auto schurUpdate = [&S, &P](std::vector<GemmTask>& out, int32_t edge, int32_t s, int32_t d0, int32_t depth) {
+      const int32_t nc = frontCols(S, s), m = frontRows(S, s), mu = m - nc, f = S.superStart[s];
+      if (mu == 0 || depth <= 0) return;
+      const int64_t base = S.Lxp[s];
+      for (int32_t j0 = 0; j0 < mu; j0 += edge)
+        for (int32_t i0 = j0; i0 < mu; i0 += edge) {
+          GemmTask t{};
+          t.a_off = base + nc + i0 + int64_t(d0) * m;
+          t.b_off = base + nc + j0 + int64_t(d0) * m;
+          t.c_off = P.Uoff[s] + i0 + int64_t(j0) * mu;
+          t.lda = t.ldb = m;
+          t.ldc = mu;
+          t.rows = std::min(edge, mu - i0);
+          t.cols = std::min(edge, mu - j0);
+          t.k = depth;
+          t.d_off = f + d0;
+          t.a_buf = t.b_buf = kBufLx;
+          t.c_buf = kBufU;
+          t.mode = d0 == 0 ? 1 : 0;
           out.push_back(t);
         }
     };
@@ -245,41 +270,18 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
           if (nc > c1) lowerUpdate(out, edge, s, c1, std::min(nc, c2), c0, SPW);
         }
       }, sideUsed ? kSyncWait : 0);
-      // bulk: every later column, on the side stream once the super-panel is factored
+      // bulk, on the side stream once the super-panel is factored: every later column, and the super-panel's share of
+      // the Schur complement U = -L21 D L21^T (so that only the last super-panel's share follows the panel chain)
       if (gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
             for (int32_t s : big) {
               const int32_t nc = frontCols(S, s);
               if (nc > c2) lowerUpdate(out, edge, s, c2, nc, c0, SPW);
+              if (nc > c0) schurUpdate(out, edge, s, c0, std::min(SPW, nc - c0));
             }
           }, kSyncSide | kSyncWait | kSyncSignal))
         sideUsed = true;
     }
-    // Schur complement U = -L21 D L21^T into the update matrix (write only), after the last bulk update
-    gemmLaunch(L, [&](int32_t edge, std::vector<GemmTask>& out) {
-      for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s), m = frontRows(S, s), mu = m - nc, f = S.superStart[s];
-        if (mu == 0) continue;
-        const int64_t base = S.Lxp[s];
-        for (int32_t j0 = 0; j0 < mu; j0 += edge)
-          for (int32_t i0 = j0; i0 < mu; i0 += edge) {
-            GemmTask t{};
-            t.a_off = base + nc + i0;
-            t.b_off = base + nc + j0;
-            t.c_off = P.Uoff[s] + i0 + int64_t(j0) * mu;
-            t.lda = t.ldb = m;
-            t.ldc = mu;
-            t.rows = std::min(edge, mu - i0);
-            t.cols = std::min(edge, mu - j0);
-            t.k = nc;
-            t.d_off = f;
-            t.a_buf = t.b_buf = kBufLx;
-            t.c_buf = kBufU;
-            t.mode = 1;
-            out.push_back(t);
-          }
-      }
-    }, sideUsed ? kSyncWait : 0);
-    // no Schur launch made (every front of the level is a root): the next main-stream launch joins the side stream
+    // the next main-stream launch (the update-matrix extend-add, or the next level) joins the side stream
     if (sideUsed && (P.launches.back().sync & kSyncSide)) joinPending = true;
     // children's contributions to the update matrix
     {

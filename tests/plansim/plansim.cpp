@@ -163,15 +163,28 @@ struct Handle {
 
 extern "C" {
 
-void* ps_create(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax) {
+// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel
+void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax,
+                       const int32_t* prm) {
   Handle* h = new Handle();
   if (!analyzePattern(n, rowptr, colidx, perm, relax != 0, h->S)) {
     delete h;
     return nullptr;
   }
-  h->F = buildFactorPlan(h->S);
+  PlanParams pp;
+  if (prm) {
+    pp.smallMax = prm[0];
+    pp.smallNarrow = prm[1];
+    pp.wideLaunch = prm[2];
+    pp.superPanel = prm[3];
+  }
+  h->F = buildFactorPlan(h->S, pp);
   h->SP = buildSolvePlan(h->S, h->F);
   return h;
+}
+
+void* ps_create(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax) {
+  return ps_create_params(n, rowptr, colidx, perm, relax, nullptr);
 }
 
 void ps_free(void* p) { delete static_cast<Handle*>(p); }
@@ -238,6 +251,168 @@ int32_t ps_factor(void* p, const double* values, int32_t ldlt, double pivotTol, 
   if (DOut) std::memcpy(DOut, c.D.data(), size_t(S.n) * 8);
   if (pertOut) std::memcpy(pertOut, c.pert.data(), size_t(S.n));
   return c.failCol;
+}
+
+// Two-stream ordering contract of the factor plan. Launches run on the main stream or, with kSyncSide, on the side
+// stream, in enqueue order per stream; kSyncWait makes a launch wait for the other stream's most recent kSyncSignal
+// launch (as the device executor does). Every element of Lx, U, W and D that one launch writes must not be read or
+// written by a launch of the other stream that is not ordered with it, nor by another task of the same launch.
+// Returns the number of conflicting accesses (0 for a race-free plan). dropWaits ignores every kSyncWait (a plan
+// with its waits removed must show conflicts, which pins the checker itself).
+int64_t ps_check_streams(void* p, int32_t dropWaits) {
+  Handle* h = static_cast<Handle*>(p);
+  const Symbolic& S = h->S;
+  const FactorPlan& P = h->F;
+  const int32_t N = int32_t(P.launches.size());
+  // vector clocks: vc[l][s] = sequence number (within stream s) of the last launch of stream s ordered before l
+  std::vector<int32_t> vc0(N), vc1(N), st(N), seq(N);
+  int32_t tail[2][2] = {{-1, -1}, {-1, -1}}, sig[2][2] = {{-1, -1}, {-1, -1}}, cnt[2] = {0, 0};
+  bool haveSig[2] = {false, false};
+  for (int32_t l = 0; l < N; l++) {
+    const Launch& L = P.launches[l];
+    const int s = (L.sync & kSyncSide) ? 1 : 0;
+    int32_t v[2] = {tail[s][0], tail[s][1]};
+    if (!dropWaits && (L.sync & kSyncWait) && haveSig[1 - s]) {
+      v[0] = std::max(v[0], sig[1 - s][0]);
+      v[1] = std::max(v[1], sig[1 - s][1]);
+    }
+    v[s] = cnt[s]++;
+    vc0[l] = v[0];
+    vc1[l] = v[1];
+    st[l] = s;
+    seq[l] = v[s];
+    tail[s][0] = v[0];
+    tail[s][1] = v[1];
+    if (L.sync & kSyncSignal) {
+      sig[s][0] = v[0];
+      sig[s][1] = v[1];
+      haveSig[s] = true;
+    }
+  }
+  auto before = [&](int32_t a, int32_t b) {  // launch a complete before launch b starts (a != b)
+    if (st[a] == st[b]) return a < b;
+    return (st[a] == 0 ? vc0[b] : vc1[b]) >= seq[a];
+  };
+  // shadow state per element: last writer and last reader per stream (launch, task)
+  struct Sh {
+    std::vector<int32_t> wl, wt, rl[2], rt[2];
+    void init(size_t n) {
+      wl.assign(n, -1);
+      wt.assign(n, -1);
+      for (int s = 0; s < 2; s++) {
+        rl[s].assign(n, -1);
+        rt[s].assign(n, -1);
+      }
+    }
+  };
+  Sh sh[4];
+  sh[0].init(size_t(std::max<int64_t>(P.nnzLStored, 1)));
+  sh[1].init(size_t(std::max<int64_t>(P.uEntries, 1)));
+  sh[2].init(size_t(std::max<int64_t>(P.wEntries, 1)));
+  sh[3].init(size_t(std::max(S.n, 1)));
+  int64_t bad = 0;
+  int32_t curL = 0, curT = 0;
+  auto conflict = [&](int32_t ol, int32_t ot) {
+    if (ol < 0) return false;
+    if (ol == curL) return ot != curT;
+    return !before(ol, curL);
+  };
+  auto acc = [&](int b, int64_t i, bool write) {
+    Sh& x = sh[b];
+    if (i < 0 || size_t(i) >= x.wl.size()) {
+      bad++;
+      return;
+    }
+    const int s = st[curL];
+    if (conflict(x.wl[i], x.wt[i])) bad++;
+    if (write) {
+      for (int o = 0; o < 2; o++)
+        if (conflict(x.rl[o][i], x.rt[o][i])) bad++;
+      x.wl[i] = curL;
+      x.wt[i] = curT;
+    } else {
+      x.rl[s][i] = curL;
+      x.rt[s][i] = curT;
+    }
+  };
+  for (int32_t l = 0; l < N; l++) {
+    const Launch& L = P.launches[l];
+    curL = l;
+    for (int64_t t = L.first; t < L.first + L.count; t++) {
+      curT = int32_t(t - L.first);
+      switch (L.kind) {
+        case kZero:
+          for (int64_t i = 0; i < P.zeros[t].len; i++) acc(0, P.zeros[t].off + i, true);
+          break;
+        case kAssemble:
+          if (P.adst[t] >= 0) acc(0, P.adst[t], true);
+          break;
+        case kSmall: {
+          const int32_t s = P.smallFronts[t];
+          const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s], mu = m - nc;
+          for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+            const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
+            for (int32_t b = 0; b < muc; b++)
+              for (int32_t a = b; a < muc; a++) acc(1, P.Uoff[ch] + a + int64_t(b) * muc, false);
+          }
+          for (int64_t i = 0; i < int64_t(m) * nc; i++) acc(0, S.Lxp[s] + i, true);
+          for (int64_t i = 0; i < int64_t(mu) * mu; i++) acc(1, P.Uoff[s] + i, true);
+          for (int32_t k = 0; k < nc; k++) acc(3, f + k, true);
+          break;
+        }
+        case kExtendAdd: {
+          const EaTask& e = P.ea[t];
+          const int32_t s = e.s, nc = S.superStart[s + 1] - S.superStart[s], m = S.Sp[s + 1] - S.Sp[s], mu = m - nc;
+          for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+            const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
+            const int32_t* rel = P.rel.data() + S.Sp[ch] + ncc;
+            for (int32_t b = 0; b < muc; b++) {
+              const int32_t j = rel[b];
+              if (j < e.j0 || j >= e.j1) continue;
+              for (int32_t a = b; a < muc; a++) {
+                acc(1, P.Uoff[ch] + a + int64_t(b) * muc, false);
+                const int32_t i = rel[a];
+                const int bb = j < nc ? 0 : 1;
+                const int64_t d = j < nc ? S.Lxp[s] + i + int64_t(j) * m : P.Uoff[s] + (i - nc) + int64_t(j - nc) * mu;
+                acc(bb, d, false);
+                acc(bb, d, true);
+              }
+            }
+          }
+          break;
+        }
+        case kDiag: {
+          const DiagTask& d = P.diag[t];
+          for (int32_t j = 0; j < d.nb; j++)
+            for (int32_t i = j; i < d.nb; i++) {
+              acc(0, d.f_off + i + int64_t(j) * d.ld, false);
+              acc(0, d.f_off + i + int64_t(j) * d.ld, true);
+            }
+          for (int64_t i = 0; i < int64_t(kTileEdge) * kTileEdge; i++) acc(2, d.w_off + i, true);
+          for (int32_t k = 0; k < d.nb; k++) acc(3, d.col0 + k, true);
+          break;
+        }
+        case kGemm: {
+          const GemmTask& g = P.gemm[t];
+          for (int32_t k = 0; k < g.k; k++) {
+            for (int32_t i = 0; i < g.rows; i++) acc(g.a_buf, g.a_off + i + int64_t(k) * g.lda, false);
+            for (int32_t j = 0; j < g.cols; j++) acc(g.b_buf, g.b_off + j + int64_t(k) * g.ldb, false);
+            if (g.mode != 2) acc(3, g.d_off + k, false);
+          }
+          if (g.mode == 2)
+            for (int32_t j = 0; j < g.cols; j++) acc(3, g.d_off + j, false);
+          for (int32_t j = 0; j < g.cols; j++)
+            for (int32_t i = 0; i < g.rows; i++) {
+              const int64_t d = g.c_off + i + int64_t(j) * g.ldc;
+              if (g.mode == 0) acc(g.c_buf, d, false);
+              acc(g.c_buf, d, true);
+            }
+          break;
+        }
+      }
+    }
+  }
+  return bad;
 }
 
 // Ordering contract of the tiled-supernode solve launches (their workgroups wait on flags raised by earlier tasks of

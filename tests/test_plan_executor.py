@@ -33,6 +33,10 @@ def sim():
     L.ps_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.ps_check_flow.argtypes = [C.c_void_p]
     L.ps_check_flow.restype = C.c_int64
+    L.ps_create_params.restype = C.c_void_p
+    L.ps_create_params.argtypes = [C.c_int32, i32p, i32p, C.c_void_p, C.c_int32, C.c_void_p]
+    L.ps_check_streams.argtypes = [C.c_void_p, C.c_int32]
+    L.ps_check_streams.restype = C.c_int64
     return L
 
 
@@ -112,3 +116,26 @@ def test_plan_failure_column(sim):
     assert 0 <= fc < n
     ok, _ = O.chol_factor(S, v)
     assert ok is False
+
+
+@pytest.mark.parametrize("name", ["heat3d_14", "heat3d_20", "grid27_11"])
+@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128), (96, 32, 2**30, 64), (64, 16, 256, 128)])
+def test_plan_two_stream_ordering(sim, name, prm):
+    """Every pair of factor launches on the two streams that the plan's waits leave unordered touches disjoint
+    memory, and no two tasks of one launch conflict; with the waits removed the checker must find conflicts (so it
+    is not vacuous)."""
+    pat, v = CASES[name]()
+    pa = None if prm is None else np.array(prm, np.int32)
+    h = sim.ps_create_params(pat["n"], pat["rowptr"], pat["colidx"], None, 1, None if pa is None else pa.ctypes.data)
+    try:
+        assert sim.ps_check_streams(h, 0) == 0
+        assert sim.ps_check_streams(h, 1) > 0
+        # the same plan replayed in launch order reproduces the oracle's factor
+        S = O.Sym(pat)
+        ok, Lo = O.chol_factor(S, v)
+        Lg, D = np.zeros(S.nnzLStored), np.zeros(pat["n"])
+        assert sim.ps_factor(h, v, 0, 0.0, Lg, D, None) == 2**31 - 1
+        m = lower_mask(S)
+        assert np.abs(Lg[m] - Lo[m]).max() / np.abs(Lo[m]).max() < 1e-12
+    finally:
+        sim.ps_free(h)
