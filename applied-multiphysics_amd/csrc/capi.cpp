@@ -61,6 +61,7 @@ int symToDevice(amp_sym* h) {
   HIP_TRY(upload(h->zeros, F.zeros));
   HIP_TRY(upload(h->smallFronts, F.smallFronts));
   HIP_TRY(upload(h->ea, F.ea));
+  HIP_TRY(upload(h->eaRanges, F.eaRanges));
   HIP_TRY(upload(h->diag, F.diag));
   HIP_TRY(upload(h->gemm, F.gemm));
   HIP_TRY(upload(h->sFronts, h->SP.fronts));
@@ -69,6 +70,7 @@ int symToDevice(amp_sym* h) {
   h->dfp.adst = h->adst.as<int64_t>();
   h->dfp.smallFronts = h->smallFronts.as<int32_t>();
   h->dfp.ea = h->ea.as<amp::EaTask>();
+  h->dfp.eaRanges = h->eaRanges.as<amp::EaRange>();
   h->dfp.diag = h->diag.as<amp::DiagTask>();
   h->dfp.gemm = h->gemm.p;
   h->dfp.launches = F.launches;

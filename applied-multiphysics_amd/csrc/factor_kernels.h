@@ -15,6 +15,7 @@ struct DeviceFactorPlan {
   const int64_t* adst = nullptr;
   const int32_t* smallFronts = nullptr;
   const EaTask* ea = nullptr;
+  const EaRange* eaRanges = nullptr;
   const DiagTask* diag = nullptr;
   const void* gemm = nullptr;
   std::vector<Launch> launches;

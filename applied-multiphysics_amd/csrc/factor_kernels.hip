@@ -150,49 +150,45 @@ __global__ __launch_bounds__(256) void k_small_front_wave(const int32_t* __restr
   }
 }
 
-// Extend-add into parent columns [j0, j1) of a tiled front. Each child column maps to one parent column through the
-// increasing relative map, so the child columns of the range are contiguous; children are added one after another
-// (reference order), each child's entries land on distinct parent entries, so the waves split a child's columns.
+// Extend-add into parent columns [j0, j1) of a tiled front. The plan lists, per work item, the column range of
+// every child that lands there (each child column maps to one parent column through the increasing relative map).
+// Children are added one after another (reference order); a child's entries land on distinct parent entries, so the
+// waves split a child's columns, each lane keeping eight rows in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks, FactorArgs<T> a) {
+__global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
+                                                    const EaRange* __restrict__ ranges, FactorArgs<T> a) {
+  constexpr int R = 8;
   const EaTask t = tasks[blockIdx.x];
   const int32_t s = t.s;
   const int32_t nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   T* L = a.Lx + a.Lxp[s];
   T* Us = mu > 0 ? a.U + a.Uoff[s] : nullptr;
-  for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+  const int lane = threadIdx.x & 63;
+  for (int32_t q = t.r0; q < t.r1; q++) {
+    const EaRange g = ranges[q];
+    const int32_t c = g.c;
     const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
     const int32_t* rel = a.rel + a.Sp[c] + ncc;
     const T* Uc = a.U + a.Uoff[c];
-    int32_t lo = 0, hi = muc;  // first b with rel[b] >= j0
-    while (lo < hi) {
-      const int32_t mid = (lo + hi) >> 1;
-      if (rel[mid] < t.j0) lo = mid + 1; else hi = mid;
-    }
-    const int32_t b0 = lo;
-    hi = muc;
-    while (lo < hi) {  // first b with rel[b] >= j1
-      const int32_t mid = (lo + hi) >> 1;
-      if (rel[mid] < t.j1) lo = mid + 1; else hi = mid;
-    }
-    const int32_t b1 = lo;
-    // child column b lands in parent column rel[b], rows rel[r] for r >= b; each wave takes its own columns (distinct
-    // parent columns) and keeps four rows per lane in flight
-    const int lane = threadIdx.x & 63;
-    for (int32_t b = b0 + int32_t(threadIdx.x >> 6); b < b1; b += int32_t(blockDim.x >> 6)) {
+    // child column b lands in parent column rel[b], rows rel[r] for r >= b
+    for (int32_t b = g.b0 + int32_t(threadIdx.x >> 6); b < g.b1; b += int32_t(blockDim.x >> 6)) {
       const int32_t j = rel[b];
       const T* src = Uc + int64_t(b) * muc;
       T* dst = j < nc ? L + int64_t(j) * m : Us + int64_t(j - nc) * mu;
       const int32_t sh = j < nc ? 0 : nc;  // parent row -> row of the column's storage
       int32_t r = b + lane;
-      for (; r + 192 < muc; r += 256) {
-        const int32_t d0 = rel[r] - sh, d1 = rel[r + 64] - sh, d2 = rel[r + 128] - sh, d3 = rel[r + 192] - sh;
-        const T s0 = src[r], s1 = src[r + 64], s2 = src[r + 128], s3 = src[r + 192];
-        const T v0 = dst[d0], v1 = dst[d1], v2 = dst[d2], v3 = dst[d3];
-        dst[d0] = v0 + s0;
-        dst[d1] = v1 + s1;
-        dst[d2] = v2 + s2;
-        dst[d3] = v3 + s3;
+      for (; r + 64 * (R - 1) < muc; r += 64 * R) {
+        int32_t d[R];
+        T v[R], w[R];
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          d[u] = rel[r + 64 * u] - sh;
+          v[u] = src[r + 64 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < R; u++) w[u] = dst[d[u]];
+#pragma unroll
+        for (int u = 0; u < R; u++) dst[d[u]] = w[u] + v[u];
       }
       for (; r < muc; r += 64) dst[rel[r] - sh] += src[r];
     }
@@ -400,7 +396,7 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         break;
       }
       case kExtendAdd:
-        hipLaunchKernelGGL(k_extend_add<T>, dim3(grid), dim3(256), 0, st, dp.ea + L.first, a);
+        hipLaunchKernelGGL(k_extend_add<T>, dim3(grid), dim3(256), 0, st, dp.ea + L.first, dp.eaRanges, a);
         break;
       case kDiag:
         launchDiagBlocks<T>(dp.diag + L.first, grid, a, st);

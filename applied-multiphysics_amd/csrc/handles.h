@@ -79,7 +79,7 @@ struct amp_sym {
   bool onDevice = false;
   // device copies of the structure and of the task arrays
   amp::DevBuf superStart, Sp, Si, Lxp, Uoff, Woff, childHead, childNext, rel, lowerP, lowerSrc, amap, adst, perm, dDiagSrc;
-  amp::DevBuf zeros, smallFronts, ea, diag, gemm, sFronts, sBlocks;
+  amp::DevBuf zeros, smallFronts, ea, eaRanges, diag, gemm, sFronts, sBlocks;
   amp::DeviceFactorPlan dfp;
   amp::DeviceSolvePlan dsp;
   amp::PlanParams params;  // the thresholds the plans were built with

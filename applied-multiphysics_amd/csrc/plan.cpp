@@ -1,6 +1,8 @@
 // Host construction of the factor and solve plans (see plan.h).
 #include "plan.h"
 
+#include <stdexcept>
+
 #include <algorithm>
 #include <functional>
 
@@ -80,6 +82,21 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     }
   }
 
+  // extend-add work item: parent columns [j0, j1) of front s, with the column range of every child that lands there
+  auto addEa = [&S, &P](int32_t s, int32_t j0, int32_t j1) {
+    EaTask t{s, j0, j1, int32_t(P.eaRanges.size()), 0, 0};
+    for (int32_t c = S.childHead[s]; c != -1; c = S.childNext[c]) {
+      const int32_t ncc = frontCols(S, c);
+      const int32_t* rel = P.rel.data() + S.Sp[c] + ncc;
+      const int32_t muc = frontRows(S, c) - ncc;
+      const int32_t b0 = int32_t(std::lower_bound(rel, rel + muc, j0) - rel);
+      const int32_t b1 = int32_t(std::lower_bound(rel, rel + muc, j1) - rel);
+      if (b1 > b0) P.eaRanges.push_back(EaRange{c, b0, b1, 0});
+    }
+    if (P.eaRanges.size() > size_t(INT32_MAX)) throw std::length_error("extend-add ranges exceed 2^31");
+    t.r1 = int32_t(P.eaRanges.size());
+    if (t.r1 > t.r0) P.ea.push_back(t);  // items with no contribution are left out
+  };
   bool joinPending = false;  // the next main-stream launch must wait for the side stream
   auto push = [&P, &joinPending](LaunchKind kind, int32_t level, int64_t first, int64_t count, int32_t maxM = 0,
                                  int32_t threads = 0, int32_t edge = 0, int32_t sync = 0) {
@@ -148,7 +165,7 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
       for (int32_t s : big) {
         if (S.childHead[s] == -1) continue;
         const int32_t nc = frontCols(S, s);
-        for (int32_t j = 0; j < nc; j += kEaWidth) P.ea.push_back(EaTask{s, j, std::min(nc, j + kEaWidth), 0});
+        for (int32_t j = 0; j < nc; j += kEaWidth) addEa(s, j, std::min(nc, j + kEaWidth));
       }
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }
@@ -289,7 +306,7 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
       for (int32_t s : big) {
         if (S.childHead[s] == -1) continue;
         const int32_t nc = frontCols(S, s), m = frontRows(S, s);
-        for (int32_t j = nc; j < m; j += kEaWidth) P.ea.push_back(EaTask{s, j, std::min(m, j + kEaWidth), 0});
+        for (int32_t j = nc; j < m; j += kEaWidth) addEa(s, j, std::min(m, j + kEaWidth));
       }
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }

@@ -42,9 +42,15 @@ struct DiagTask {
   int32_t ld, nb, col0, pad;
 };
 
-// Add every child's update matrix into columns [j0, j1) (front coordinates) of front s.
+// Add every child's update matrix into columns [j0, j1) (front coordinates) of front s. The children's columns that
+// land there are listed in EaRange entries [r0, r1), children in the reference's order (childHead chain).
 struct EaTask {
-  int32_t s, j0, j1, pad;
+  int32_t s, j0, j1, r0, r1, pad;
+};
+
+// Columns [b0, b1) of child c's update matrix (they map to increasing parent columns through the relative map).
+struct EaRange {
+  int32_t c, b0, b1, pad;
 };
 
 struct ZeroTask {
@@ -92,6 +98,7 @@ struct FactorPlan {
   std::vector<ZeroTask> zeros;
   std::vector<int32_t> smallFronts;
   std::vector<EaTask> ea;
+  std::vector<EaRange> eaRanges;
   std::vector<DiagTask> diag;
   std::vector<GemmTask> gemm;
   std::vector<Launch> launches;

@@ -90,17 +90,23 @@ void runSmall(Ctx& c, int32_t s, const double* values) {
       for (int32_t a = 0; a < mu; a++) c.U[P.Uoff[s] + a + int64_t(b) * mu] = F[nc + a + int64_t(nc + b) * m];
 }
 
+// the task's child column ranges, as the device does; every column a range lists must land in [j0, j1)
 void runEa(Ctx& c, const EaTask& t) {
   const Symbolic& S = *c.S;
   const FactorPlan& P = *c.P;
   const int32_t s = t.s, nc = S.superStart[s + 1] - S.superStart[s], m = S.Sp[s + 1] - S.Sp[s], mu = m - nc;
-  for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+  for (int32_t q = t.r0; q < t.r1; q++) {
+    const EaRange& g = P.eaRanges[q];
+    const int32_t ch = g.c;
     const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
     const int32_t* rel = P.rel.data() + S.Sp[ch] + ncc;
     const double* Uc = c.U.data() + P.Uoff[ch];
-    for (int32_t b = 0; b < muc; b++) {
+    for (int32_t b = g.b0; b < g.b1; b++) {
       const int32_t j = rel[b];
-      if (j < t.j0 || j >= t.j1) continue;
+      if (j < t.j0 || j >= t.j1) {
+        c.failCol = -2;  // a range outside its work item: reported as a plan error
+        continue;
+      }
       for (int32_t a = b; a < muc; a++) {
         const int32_t i = rel[a];
         const double v = Uc[a + int64_t(b) * muc];
@@ -363,12 +369,12 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
         case kExtendAdd: {
           const EaTask& e = P.ea[t];
           const int32_t s = e.s, nc = S.superStart[s + 1] - S.superStart[s], m = S.Sp[s + 1] - S.Sp[s], mu = m - nc;
-          for (int32_t ch = S.childHead[s]; ch != -1; ch = S.childNext[ch]) {
+          for (int32_t q = e.r0; q < e.r1; q++) {
+            const int32_t ch = P.eaRanges[q].c;
             const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
             const int32_t* rel = P.rel.data() + S.Sp[ch] + ncc;
-            for (int32_t b = 0; b < muc; b++) {
+            for (int32_t b = P.eaRanges[q].b0; b < P.eaRanges[q].b1; b++) {
               const int32_t j = rel[b];
-              if (j < e.j0 || j >= e.j1) continue;
               for (int32_t a = b; a < muc; a++) {
                 acc(1, P.Uoff[ch] + a + int64_t(b) * muc, false);
                 const int32_t i = rel[a];
