@@ -423,7 +423,7 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   const amp::FactorPlan& F = h->F;
   HIP_TRY(hipStreamCreateWithFlags(&num->stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&num->side, hipStreamNonBlocking));
-  num->syncEv.resize(size_t(F.signals) + 1, nullptr);
+  num->syncEv.resize(size_t(F.signals) + 2, nullptr);  // signals, the final join, the initial fork
   for (hipEvent_t& e : num->syncEv) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&num->inEv, hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&num->ev0));

@@ -29,8 +29,9 @@ struct ProfileSink {
   size_t used = 0;
 };
 
-// Enqueue the factor plan on `main`, with the launches marked kSyncSide on `side`; `sync` holds at least
-// FactorPlan::signals + 1 events. The side stream is joined into `main` before returning.
+// Enqueue the factor plan on `main`, with the launches marked kSyncSide on `side`; `sync` holds
+// FactorPlan::signals + 2 events (the signals, the final join, and the initial fork: the side stream first waits for
+// everything enqueued on `main` before the call). The side stream is joined into `main` before returning.
 template <typename T>
 void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t main,
                       hipStream_t side, const std::vector<hipEvent_t>& sync, ProfileSink* prof);

@@ -356,6 +356,9 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
   if (prof) prof->used = 0;
   hipEvent_t last[2] = {nullptr, nullptr};  // most recent signal of each stream
   size_t ev = 0;
+  // fork: the side stream starts after everything enqueued on `main` so far (and joins a capture of `main`)
+  (void)hipEventRecord(sync.back(), main);
+  (void)hipStreamWaitEvent(side, sync.back(), 0);
   for (const Launch& L : dp.launches) {
     const unsigned grid = unsigned(L.count);
     const int si = (L.sync & kSyncSide) ? 1 : 0;

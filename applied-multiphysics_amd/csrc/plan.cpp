@@ -132,8 +132,11 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     const int64_t len = S.Lxp[s + 1] - S.Lxp[s];
     for (int64_t o = 0; o < len; o += kZeroChunk) P.zeros.push_back(ZeroTask{S.Lxp[s] + o, std::min<int64_t>(kZeroChunk, len - o)});
   }
-  push(kZero, -1, 0, int64_t(P.zeros.size()));
-  push(kAssemble, -1, 0, S.nnzA);
+  // on the side stream, overlapping the whole-front levels (which touch neither these blocks nor A's entries in
+  // them); the first launch of the first level with tiled fronts waits for it
+  push(kZero, -1, 0, int64_t(P.zeros.size()), 0, 0, 0, kSyncSide);
+  push(kAssemble, -1, 0, S.nnzA, 0, 0, 0, kSyncSide | kSyncSignal);
+  bool assemblyPending = true;
 
   std::vector<std::vector<int32_t>> byLevel(P.nlevels);
   for (int32_t s = 0; s < ns; s++) byLevel[P.level[s]].push_back(s);
@@ -158,6 +161,10 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     for (int32_t s : fronts)
       if (P.isBig[s]) big.push_back(s);
     if (big.empty()) continue;
+    if (assemblyPending) {
+      joinPending = true;
+      assemblyPending = false;
+    }
 
     // children's contributions to the factor columns
     {
@@ -369,7 +376,9 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     if (big.empty()) continue;
     int64_t first = int64_t(P.fronts.size());
     P.fronts.insert(P.fronts.end(), big.begin(), big.end());
-    push(kSolveGather, L, first, int64_t(big.size()));
+    int32_t maxM = 0;
+    for (int32_t s : big) maxM = std::max(maxM, frontRows(S, s));
+    push(kSolveGather, L, first, int64_t(big.size()), maxM);
     // every task depends only on tasks listed before it: own tiles by block index across the level's supernodes,
     // then the below tiles (which consume every block of their supernode)
     const int64_t bf = int64_t(P.blocks.size());
@@ -397,7 +406,9 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     if (big.empty()) continue;
     int64_t first = int64_t(P.fronts.size());
     P.fronts.insert(P.fronts.end(), big.begin(), big.end());
-    push(kSolveBwdGather, L, first, int64_t(big.size()));
+    int32_t maxMu = 0;
+    for (int32_t s : big) maxMu = std::max(maxMu, frontRows(S, s) - frontCols(S, s));
+    push(kSolveBwdGather, L, first, int64_t(big.size()), maxMu);
     // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk)
     std::vector<int32_t> p0(ns, 0), np(ns, 0);
     int64_t bf = int64_t(P.blocks.size());

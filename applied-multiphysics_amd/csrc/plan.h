@@ -155,7 +155,8 @@ struct SolveLaunch {
   SolveKind kind;
   int32_t level;
   int64_t first, count;  // small / gather: range in `fronts`; flow kinds: range in `blocks`
-  int32_t maxM;
+  int32_t maxM;          // small: largest front height; gathers: largest m (forward) or mu (backward), which sets
+                         // the number of row slices per front
 };
 
 struct SolvePlan {

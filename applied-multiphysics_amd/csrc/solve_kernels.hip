@@ -167,12 +167,36 @@ __global__ __launch_bounds__(256) void k_solve_wave_fwd(const int32_t* __restric
   }
 }
 
+// Front vectors of the tiled supernodes of a level, rows [kGatherRows y, kGatherRows (y + 1)) of front fronts[x] per
+// workgroup (x, y), so that the large fronts near the root are gathered by many workgroups. Every workgroup walks each
+// child's whole relative map and keeps the rows that land in its slice (the map increases, so a lane stops at the first
+// row past the slice); per entry the order is the single-workgroup one: own value or zero, then the children in order.
+constexpr int kGatherRows = 512;
+
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y, S* fv,
                                                       int32_t q, int64_t fvN) {
   const int32_t s = fronts[blockIdx.x];
-  const int32_t nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s];
-  for (int32_t k = 0; k < q; k++) gatherFront<T, S>(a, s, y + int64_t(k) * a.n, fv + k * fvN, fv + k * fvN + a.Sp[s], nc, m);
+  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
+  const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(m, i0 + kGatherRows);
+  if (i0 >= m) return;  // whole workgroup
+  for (int32_t k = 0; k < q; k++) {
+    const S* yk = y + int64_t(k) * a.n;
+    const S* fk = fv + k * fvN;
+    S* v = fv + k * fvN + a.Sp[s];
+    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) v[i] = i < nc ? yk[f + i] : S(0);
+    __syncthreads();
+    for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+      const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+      const int32_t o = a.Sp[c] + ncc;
+      for (int32_t r = threadIdx.x; r < muc; r += blockDim.x) {
+        const int32_t i = a.rel[o + r];
+        if (i >= i1) break;
+        if (i >= i0) v[i] += fk[o + r];
+      }
+      __syncthreads();
+    }
+  }
 }
 
 namespace {
@@ -475,11 +499,12 @@ __global__ __launch_bounds__(256) void k_solve_bwd_gather(const int32_t* __restr
   const int32_t s = fronts[blockIdx.x];
   const int32_t nc = a.superStart[s + 1] - a.superStart[s], mu = a.Sp[s + 1] - a.Sp[s] - nc;
   const int32_t* rows = a.Si + a.Sp[s] + nc;
+  const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(mu, i0 + kGatherRows);
   for (int32_t jj = 0; jj < q; jj++) {
     S* dst = fv + jj * fvN + a.Sp[s] + nc;
     const S* yk = y + int64_t(jj) * a.n;
-#pragma unroll 4
-    for (int32_t i = threadIdx.x; i < mu; i += blockDim.x) dst[i] = yk[rows[i]];
+#pragma unroll 2
+    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) dst[i] = yk[rows[i]];
   }
 }
 
@@ -689,8 +714,8 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                              dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
       case kSolveGather:
-        hipLaunchKernelGGL((k_solve_gather<T, S>), dim3(grid), dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q,
-                           dp.fvEntries);
+        hipLaunchKernelGGL((k_solve_gather<T, S>), dim3(grid, unsigned((L.maxM + kGatherRows - 1) / kGatherRows)),
+                           dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
       case kSolveFwdFlow:
         err = q == 1 ? launchFlow((const void*)k_solve_fwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, a, y,
@@ -715,8 +740,9 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                              dp.fronts + L.first, a, y, q);
         break;
       case kSolveBwdGather:
-        hipLaunchKernelGGL((k_solve_bwd_gather<T, S>), dim3(grid), dim3(256), 0, st, dp.fronts + L.first, a, y, fv,
-                           q, dp.fvEntries);
+        hipLaunchKernelGGL((k_solve_bwd_gather<T, S>),
+                           dim3(grid, unsigned(std::max(1, (L.maxM + kGatherRows - 1) / kGatherRows))), dim3(256), 0,
+                           st, dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
       case kSolveBwdInit:
         if (q == 1)
