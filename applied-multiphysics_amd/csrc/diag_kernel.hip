@@ -17,9 +17,8 @@ namespace amp {
 // constant.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
-  constexpr int E = kTileEdge, P = kTileEdge + 1, J = kTileEdge / 4;
+  constexpr int E = kTileEdge, J = kTileEdge / 4;
   __shared__ T colL[2][E], colD[2][E], rowW[2][E];
-  __shared__ T Ws[E * P];
   const DiagTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63, nb = t.nb;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -71,15 +70,16 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
     }
     __syncthreads();
   }
+  // W (column-major, edge 64) straight from the registers: each store writes row j of W across the 64 columns, the
+  // L2 merges the rows into whole lines; no LDS transpose, so the kernel keeps a small LDS footprint and fits on a CU
+  // beside the trailing-update workgroups it runs with
+  T* Wt = a.W + t.w_off;
 #pragma unroll
   for (int jj = 0; jj < J; jj++) {
     const int j = 4 * jj + w;
     if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = rA[jj];
-    Ws[j * P + lane] = (j < nb && lane < nb) ? rW[jj] : T(0);
+    Wt[j + E * lane] = (j < nb && lane < nb) ? rW[jj] : T(0);
   }
-  __syncthreads();
-  T* Wt = a.W + t.w_off;
-  for (int idx = threadIdx.x; idx < E * E; idx += 256) Wt[idx] = Ws[(idx & 63) * P + (idx >> 6)];
 }
 
 template <typename T>

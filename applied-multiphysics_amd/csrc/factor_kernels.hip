@@ -4,6 +4,8 @@
 // The arithmetic per front is the reference's dense partial factorization (packages/core/src/la/sparse.mjs:992-1098,
 // ldlt.mjs:492-535, pivots as wasm/front-real.mjs:367-389), reorganised as a blocked right-looking factorization
 // whose trailing products run on the matrix cores.
+#include <type_traits>
+
 #include "factor_kernels.h"
 #include "pivot.h"
 
@@ -234,23 +236,27 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   for (int bi = 0; bi < NBI; bi++)
 #pragma unroll
     for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = V4{0, 0, 0, 0};
-  T pa[NL], pb[NL];
-  auto load = [&](int k0) {
+  // two register sets of loaded depth values: a chunk is loaded two chunks before its turn to feed the matrix
+  // cores, so its global-memory latency is covered by two chunks of products
+  T pa[2][NL], pb[2][NL];
+  auto load = [&](auto set, int k0) {
+    constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int r = 0; r < NL; r++) {
       const int k = k0 + lk + LS * r;
       const bool kin = k < t.k;
-      pa[r] = (kin && rowA) ? A[li + int64_t(k) * t.lda] : T(0);
+      pa[S][r] = (kin && rowA) ? A[li + int64_t(k) * t.lda] : T(0);
       T bv = (kin && rowB) ? B[li + int64_t(k) * t.ldb] : T(0);
       if (scaleDepth && kin) bv *= D[t.d_off + k];
-      pb[r] = bv;
+      pb[S][r] = bv;
     }
   };
-  auto stash = [&](int buf) {
+  auto stash = [&](auto set, int buf) {
+    constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int r = 0; r < NL; r++) {
-      As[buf][(lk + LS * r) * PA + li] = pa[r];
-      Bs[buf][(lk + LS * r) * PA + li] = pb[r];
+      As[buf][(lk + LS * r) * PA + li] = pa[S][r];
+      Bs[buf][(lk + LS * r) * PA + li] = pb[S][r];
     }
   };
   auto compute = [&](int buf) {
@@ -269,27 +275,46 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
         for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = mfma16x16x4(bf[bj], af[bi], acc[bi][bj]);
     }
   };
-  load(0);
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  load(S0{}, 0);
+  if (KC < t.k) load(S1{}, KC);
   if constexpr (NBUF == 2) {
-    // double-buffered: chunk c is read from buffer c & 1 while chunk c + 1 is written to the other; one barrier
-    stash(0);
+    // double-buffered LDS: chunk c is read from buffer c & 1 while chunk c + 1 (already in registers) is written to
+    // the other, and chunk c + 2 is loaded into the register set chunk c left; one barrier per chunk. Chunk c's
+    // register set is c & 1 (the loop is unrolled by two so that the sets are compile-time).
+    stash(S0{}, 0);
     __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < t.k; k0 += KC) {
-      const bool more = k0 + KC < t.k;
-      if (more) load(k0 + KC);
+    auto step = [&](auto set, int k0) {  // `set`: register set of chunk c + 1; buffer of chunk c is set ^ 1
+      constexpr int S = decltype(set)::value;
+      // the buffer index is made opaque, so that the LDS addresses of both buffers are not precomputed and kept live
+      // across the loop
+      int buf = S ^ 1;
+      asm volatile("" : "+s"(buf));
+      if (k0 + 2 * KC < t.k) load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
       compute(buf);
-      if (more) stash(buf ^ 1);
+      if (k0 + KC < t.k) stash(set, buf ^ 1);
       __syncthreads();
-      buf ^= 1;
+    };
+    for (int k0 = 0; k0 < t.k; k0 += 2 * KC) {
+      step(S1{}, k0);
+      if (k0 + KC < t.k) step(S0{}, k0 + KC);
     }
   } else {
-    for (int k0 = 0; k0 < t.k; k0 += KC) {
-      stash(0);
+    // one LDS buffer: chunk c (register set c & 1) is stashed, then chunk c + 2 is loaded into the same set while
+    // chunk c feeds the matrix cores
+    auto step = [&](auto set, int k0) {
+      int buf = 0;  // opaque, as above
+      asm volatile("" : "+s"(buf));
+      stash(set, buf);
       __syncthreads();
-      if (k0 + KC < t.k) load(k0 + KC);
-      compute(0);
+      if (k0 + 2 * KC < t.k) load(set, k0 + 2 * KC);
+      compute(buf);
       __syncthreads();
+    };
+    for (int k0 = 0; k0 < t.k; k0 += 2 * KC) {
+      step(S0{}, k0);
+      if (k0 + KC < t.k) step(S1{}, k0 + KC);
     }
   }
   if (!active) return;

@@ -233,9 +233,10 @@ __device__ inline int32_t flowReady(const int32_t* flags, int32_t k, int32_t lim
   return s_r;
 }
 
-// Every thread makes its stores visible device-wide, then thread 0 raises the flag (release, device scope).
+// The workgroup barrier orders every thread's stores before thread 0's flag store, whose device-scope release makes
+// them all visible to the device (one L2 write-back per publish; a device-scope fence in every thread would also
+// invalidate the L2 once per wave).
 __device__ inline void flowPublish(int32_t* flag, int32_t val) {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
