@@ -446,6 +446,12 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
     num->fc.bwdFlag = w + SP.flowFlags;
     num->fc.err = w + 2 * SP.flowFlags;
     num->fc.epoch = w + 2 * SP.flowFlags + 1;
+    // solved-value mailboxes of both directions (pass number 0 in every word: never current)
+    const size_t boxBytes = size_t(std::max<int64_t>(SP.flowFlags, 1)) * amp::kBoxWords * 8;
+    HIP_TRY(num->boxes.alloc(2 * boxBytes));
+    HIP_TRY(hipMemset(num->boxes.p, 0, 2 * boxBytes));
+    num->fc.fwdBox = num->boxes.as<uint64_t>();
+    num->fc.bwdBox = num->fc.fwdBox + boxBytes / 8;
   }
   rc = ensureVecs(num.get(), 1);
   if (rc) return rc;

@@ -69,12 +69,19 @@ struct SolveArgs {
 // clearing; the pass number lives in device memory (*epoch) and is raised by a kernel at the start of every pass
 // of at most kFlowRhs right-hand sides, so a captured graph raises it on every replay. err is set when a wait gives
 // up.
+// The solved values of a block also travel in mailboxes (fwdBox / bwdBox): per flag, kFlowRhs x 64 values, each in
+// 64-bit words that carry the pass number in their high half (one word per FP32 value, two per FP64 value: low and
+// high half of the bits). A consumer accepts a word only when it carries the current pass number, so the values need
+// no fence to be read safely; the flags only tell a consumer how many blocks are worth reading.
 struct FlowCtl {
   int32_t* fwdFlag;
   int32_t* bwdFlag;
   int32_t* err;
   int32_t* epoch;
+  uint64_t* fwdBox;
+  uint64_t* bwdBox;
 };
+constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 values)
 
 // Solve q systems in place on the device: b (q vectors of n, original order) -> x. y (n q), fv (fvEntries q) and pz
 // (initSlots x 64 x min(q, kFlowRhs)) are scratch of the working precision S; Lx / W / D are the factor's precision

@@ -97,7 +97,7 @@ struct amp_num {
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
   // tiled-supernode solve synchronisation: forward and backward block flags, the wait-failure flag
-  amp::DevBuf flow;
+  amp::DevBuf flow, boxes;
   amp::FlowCtl fc{};
   bool ok = false;
   bool profiling = false;

@@ -204,10 +204,10 @@ namespace {
 constexpr uint32_t kFlowSpinLimit = 1u << 22;  // about half a second of polling before a wait gives up
 constexpr int kFlowRun = 8;                    // blocks consumed per synchronisation of a flow task
 
-// Waits (thread 0 polls, then acquires at device scope) until block `k` is complete and returns the end of the run of
-// complete blocks that starts at k and stops at `lim` (exclusive; lim < k walks downwards). A block is published only
-// after every block it consumes, so when block lim -1 (lim + 1 downwards) is complete, so is the whole run; the two
-// flags are read together. A wait that exceeds the spin limit, or that finds another wait already failed, sets *err
+// Waits (thread 0 polls) until the flag of block `k` is raised and returns the end of the run of raised flags that
+// starts at k and stops at `lim` (exclusive; lim < k walks downwards). A block is published only after every block it
+// consumes, so when block lim - 1 (lim + 1 downwards) is raised, so is the whole run; the two flags are read together.
+// The flags are hints: the values themselves are read from the mailboxes (boxLoad), which validate every word. A wait that exceeds the spin limit, or that finds another wait already failed, sets *err
 // and proceeds, so the launch always completes; the host reports the error.
 __device__ inline int32_t flowReady(const int32_t* flags, int32_t k, int32_t lim, int32_t want, int32_t* err) {
   __shared__ int32_t s_r;
@@ -215,7 +215,7 @@ __device__ inline int32_t flowReady(const int32_t* flags, int32_t k, int32_t lim
     const int32_t step = lim > k ? 1 : -1, last = lim - step;
     uint32_t spins = 0;
     int32_t fk, fl;
-    // relaxed polling (no cache invalidation per poll), one acquire fence once the flag is seen
+    // relaxed polling: no cache invalidation (the mailbox words are read with device-coherent loads)
     for (;;) {
       fk = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       fl = __hip_atomic_load(flags + last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -226,19 +226,59 @@ __device__ inline int32_t flowReady(const int32_t* flags, int32_t k, int32_t lim
       }
       __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_r = fl == want ? lim : k + step;
   }
   __syncthreads();
   return s_r;
 }
 
-// The workgroup barrier orders every thread's stores before thread 0's flag store, whose device-scope release makes
-// them all visible to the device (one L2 write-back per publish; a device-scope fence in every thread would also
-// invalidate the L2 once per wave).
+// Raise a block's flag after the workgroup's mailbox stores (relaxed: consumers validate the words themselves).
 __device__ inline void flowPublish(int32_t* flag, int32_t val) {
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Mailbox word index of value r (0..63) of right-hand side jj of the block with flag `flag` (see FlowCtl).
+__device__ inline int64_t boxSlot(int32_t flag, int32_t jj, int32_t r) {
+  return (int64_t(flag) * kFlowRhs + jj) * 64 + r;
+}
+
+// Store / load one solved value with the pass number in every word (device-coherent 64-bit atomic accesses, each
+// single-copy atomic). A load spins until its words carry `want`; a wait that exceeds the spin limit sets *err and
+// returns what it read (the host reports the error).
+__device__ inline void boxStore(uint64_t* box, int64_t slot, int32_t want, float v) {
+  const uint64_t tag = uint64_t(uint32_t(want)) << 32;
+  __hip_atomic_store(box + 2 * slot, tag | __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void boxStore(uint64_t* box, int64_t slot, int32_t want, double v) {
+  const uint64_t tag = uint64_t(uint32_t(want)) << 32;
+  const uint64_t bits = uint64_t(__double_as_longlong(v));
+  __hip_atomic_store(box + 2 * slot, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(box + 2 * slot + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t boxWord(const uint64_t* p, int32_t want, int32_t* err) {
+  uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t spins = 0; uint32_t(w >> 32) != uint32_t(want);) {
+    if (++spins >= kFlowSpinLimit) {
+      atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return w;
+}
+template <typename S>
+__device__ inline S boxLoad(const uint64_t* box, int64_t slot, int32_t want, int32_t* err);
+template <>
+__device__ inline float boxLoad<float>(const uint64_t* box, int64_t slot, int32_t want, int32_t* err) {
+  return __uint_as_float(uint32_t(boxWord(box + 2 * slot, want, err)));
+}
+template <>
+__device__ inline double boxLoad<double>(const uint64_t* box, int64_t slot, int32_t want, int32_t* err) {
+  const uint64_t lo = boxWord(box + 2 * slot, want, err) & 0xffffffffull;
+  const uint64_t hi = boxWord(box + 2 * slot + 1, want, err) & 0xffffffffull;
+  return __longlong_as_double(int64_t((hi << 32) | lo));
 }
 
 }  // namespace
@@ -313,7 +353,9 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
       const int32_t col = k0 * E + idx;
 #pragma unroll
       for (int jj = 0; jj < Q; jj++)
-        if (jj < q) xs[jj][idx] = col < nc ? y[int64_t(jj) * a.n + f + col] : S(0);
+        if (jj < q)
+          xs[jj][idx] = col < nc ? boxLoad<S>(c.fwdBox, boxSlot(t.flag0 + k0 + idx / E, jj, idx % E), want, c.err)
+                                 : S(0);
     }
     __syncthreads();
     for (int32_t k = k0; k < k1; k++) {
@@ -362,8 +404,11 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
   }
   __syncthreads();
   if (tid < E && live)
-    for (int jj = 0; jj < q; jj++)
-      y[int64_t(jj) * a.n + f + r] = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
+    for (int jj = 0; jj < q; jj++) {
+      const S xv = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
+      y[int64_t(jj) * a.n + f + r] = xv;
+      if (nb > 1) boxStore(c.fwdBox, boxSlot(t.flag0 + t.blk, jj, rr), want, xv);
+    }
   if (nb > 1) flowPublish(c.fwdFlag + t.flag0 + t.blk, want);  // single blocks have no waiters
 }
 
@@ -590,7 +635,9 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
       const int32_t row = (k0 - idx / E) * E + (idx % E);
 #pragma unroll
       for (int jj = 0; jj < Q; jj++)
-        if (jj < q) xs[jj][idx] = row < nc ? y[int64_t(jj) * a.n + f + row] : S(0);
+        if (jj < q)
+          xs[jj][idx] = row < nc ? boxLoad<S>(c.bwdBox, boxSlot(t.flag0 + k0 - idx / E, jj, idx % E), want, c.err)
+                                 : S(0);
     }
     __syncthreads();
     for (int32_t k = k0; k > k1; k--) {
@@ -628,7 +675,10 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
     }
     sum += __shfl_xor(sum, 16, 64);
     sum += __shfl_xor(sum, 32, 64);
-    if (jj < q && rs == 0 && live) y[int64_t(jj) * a.n + f + col] = sum;
+    if (jj < q && rs == 0 && live) {
+      y[int64_t(jj) * a.n + f + col] = sum;
+      if (t.blk > 0) boxStore(c.bwdBox, boxSlot(t.flag0 + t.blk, jj, cj), want, sum);
+    }
   }
   if (t.blk > 0) flowPublish(c.bwdFlag + t.flag0 + t.blk, want);  // block 0 has no waiters
 }
