@@ -438,7 +438,10 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   HIP_TRY(num->values.alloc(size_t(std::max<int64_t>(h->S.nnzPattern, 1)) * 8));
   {
     const amp::SolvePlan& SP = h->SP;
-    const size_t bytes = size_t(2 * SP.flowFlags + 2) * 4;  // flags of both directions, error flag, pass number
+    int32_t nflow = 0;
+    for (const amp::SolveLaunch& L : SP.launches) nflow += L.kind == amp::kSolveFwdFlow || L.kind == amp::kSolveBwdFlow;
+    // flags of both directions, error flag, pass number, task counters of the flow launches
+    const size_t bytes = size_t(2 * SP.flowFlags + 2 + nflow) * 4;
     HIP_TRY(num->flow.alloc(bytes));
     HIP_TRY(hipMemset(num->flow.p, 0, bytes));
     int32_t* w = num->flow.as<int32_t>();
@@ -446,6 +449,8 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
     num->fc.bwdFlag = w + SP.flowFlags;
     num->fc.err = w + 2 * SP.flowFlags;
     num->fc.epoch = w + 2 * SP.flowFlags + 1;
+    num->fc.ctr = w + 2 * SP.flowFlags + 2;
+    num->fc.nctr = nflow;
     // solved-value mailboxes of both directions (pass number 0 in every word: never current)
     const size_t boxBytes = size_t(std::max<int64_t>(SP.flowFlags, 1)) * amp::kBoxWords * 8;
     HIP_TRY(num->boxes.alloc(2 * boxBytes));

@@ -80,6 +80,8 @@ struct FlowCtl {
   int32_t* epoch;
   uint64_t* fwdBox;
   uint64_t* bwdBox;
+  int32_t* ctr;  // task counters of the flow launches of a pass (cleared by the kernel that raises the pass number)
+  int32_t nctr;
 };
 constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 values)
 

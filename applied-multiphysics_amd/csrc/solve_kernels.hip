@@ -11,9 +11,22 @@
 
 namespace amp {
 
-// Raise the pass number of the tiled-supernode solve flags (wrapping to 1; flags start at 0).
-__global__ void k_epoch_bump(int32_t* epoch) {
+// Raise the pass number of the tiled-supernode solve flags (wrapping to 1; flags start at 0) and clear the task
+// counters of the pass's flow launches.
+__global__ void k_epoch_bump(int32_t* epoch, int32_t* ctr, int32_t nctr) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
+  for (int32_t i = threadIdx.x; i < nctr; i += blockDim.x) ctr[i] = 0;
+}
+
+// Next task of a flow launch for the calling workgroup: tasks are handed out in list order as workgroups ask for them,
+// so every task a running task waits on was taken earlier by a workgroup that is running or done, and the launch
+// drains whatever the number of co-resident workgroups (no cooperative launch needed).
+__device__ inline int32_t nextTask(int32_t* ctr) {
+  __shared__ int32_t s_i;
+  __syncthreads();  // the previous task's readers of s_i are done
+  if (threadIdx.x == 0) s_i = atomicAdd(ctr, 1);
+  __syncthreads();
+  return s_i;
 }
 
 template <typename S>
@@ -412,20 +425,16 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
   if (nb > 1) flowPublish(c.fwdFlag + t.flag0 + t.blk, want);  // single blocks have no waiters
 }
 
-// Persistent form: the grid is at most the number of co-resident workgroups (cooperative launch), and workgroup b
-// runs tasks b, b + grid, ... in order. Every task waits only on tasks listed before it, so the lowest unfinished task
-// never waits on an unfinished one and is being run: the launch always drains.
+// Persistent form: workgroups take tasks in list order (nextTask). Every task waits only on tasks listed before it, so
+// the lowest unfinished task never waits on an unfinished one and is being run: the launch always drains.
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_fwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
-                                                        SolveArgs<T> a, S* y, S* fv, int32_t q, int64_t fvN,
-                                                        FlowCtl c) {
+                                                        int32_t* ctr, SolveArgs<T> a, S* y, S* fv, int32_t q,
+                                                        int64_t fvN, FlowCtl c) {
   __shared__ S part[4][Q][kTileEdge];
   __shared__ S xs[Q][kFlowRun * kTileEdge];
   const int32_t want = *c.epoch;  // this pass's number (raised by the kernel that starts the pass)
-  for (int32_t i = blockIdx.x; i < ntasks; i += gridDim.x) {
-    fwdTask<T, S, Q>(tasks[i], a, y, fv, q, fvN, c, want, part, xs);
-    __syncthreads();
-  }
+  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr)) fwdTask<T, S, Q>(tasks[i], a, y, fv, q, fvN, c, want, part, xs);
 }
 
 template <typename T, typename S, int Q>
@@ -686,14 +695,12 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
 // Persistent form, as k_solve_fwd_flow.
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_bwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
-                                                        SolveArgs<T> a, S* y, const S* pz, int32_t q, FlowCtl c) {
+                                                        int32_t* ctr, SolveArgs<T> a, S* y, const S* pz, int32_t q,
+                                                        FlowCtl c) {
   __shared__ S xs[Q][kFlowRun * kTileEdge];
   __shared__ S zs[Q][kTileEdge];
   const int32_t want = *c.epoch;
-  for (int32_t i = blockIdx.x; i < ntasks; i += gridDim.x) {
-    bwdTask<T, S, Q>(tasks[i], a, y, pz, q, c, want, xs, zs);
-    __syncthreads();
-  }
+  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr)) bwdTask<T, S, Q>(tasks[i], a, y, pz, q, c, want, xs, zs);
 }
 
 namespace {
@@ -712,19 +719,19 @@ int residentGroups(const void* kernel, int dev) {
   return cache[{kernel, dev}] = cus * perCu;
 }
 
-// Cooperative launch of a persistent flow kernel over `ntasks` tasks with at most the co-resident number of
-// workgroups (the runtime refuses a cooperative grid that cannot be co-resident).
-template <typename... Args>
-hipError_t launchFlow(const void* kernel, int64_t ntasks, hipStream_t st, const BlockTask* tasks, Args... args) {
+// Launch of a persistent flow kernel over `ntasks` tasks with at most the co-resident number of workgroups (more
+// would only wait for a slot); an ordinary launch on the pass's stream, so it shares the hardware queue of the pass's
+// other kernels.
+template <typename K, typename... Args>
+hipError_t launchFlow(K kernel, int64_t ntasks, hipStream_t st, const BlockTask* tasks, int32_t* ctr, Args... args) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  const int cap = residentGroups(kernel, dev);
+  const int cap = residentGroups((const void*)kernel, dev);
   if (cap <= 0) return hipErrorInvalidConfiguration;
-  int32_t n = int32_t(ntasks);
   const unsigned grid = unsigned(std::min<int64_t>(ntasks, cap));
-  void* ptrs[] = {(void*)&tasks, (void*)&n, (void*)&args...};
-  return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(256), ptrs, 0, st);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, st, tasks, int32_t(ntasks), ctr, args...);
+  return hipGetLastError();
 }
 
 template <typename T, typename S>
@@ -735,6 +742,7 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
   hipLaunchKernelGGL(k_perm_in<S>, dim3(g), dim3(256), 0, st, b, y, a.perm, n, q);
   bool scaled = false;
   hipError_t err = hipSuccess;
+  int32_t nflow = 0;  // flow launches so far: index of the next one's task counter
   for (const SolveLaunch& L : dp.launches) {
     if (a.ldlt && !scaled && L.kind >= kSolveSmallBwd) {
       hipLaunchKernelGGL((k_diag_scale<T, S>), dim3(g), dim3(256), 0, st, y, a.D, n, q);
@@ -769,10 +777,10 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                            dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
       case kSolveFwdFlow:
-        err = q == 1 ? launchFlow((const void*)k_solve_fwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, a, y,
-                                  fv, q, dp.fvEntries, fc)
-                     : launchFlow((const void*)k_solve_fwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, a,
-                                  y, fv, q, dp.fvEntries, fc);
+        err = q == 1 ? launchFlow(k_solve_fwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++, a,
+                                  y, fv, q, dp.fvEntries, fc)
+                     : launchFlow(k_solve_fwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++,
+                                  a, y, fv, q, dp.fvEntries, fc);
         break;
       case kSolveSmallBwd:
         if (L.maxM <= kSolveWaveMax) {
@@ -804,10 +812,10 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
                              fv, pz, q, dp.fvEntries);
         break;
       case kSolveBwdFlow:
-        err = q == 1 ? launchFlow((const void*)k_solve_bwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, a, y, pz,
-                                  q, fc)
-                     : launchFlow((const void*)k_solve_bwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, a,
-                                  y, pz, q, fc);
+        err = q == 1 ? launchFlow(k_solve_bwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++, a,
+                                  y, pz, q, fc)
+                     : launchFlow(k_solve_bwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++,
+                                  a, y, pz, q, fc);
         break;
     }
     if (rec) {
@@ -830,7 +838,7 @@ hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, con
   if (n == 0 || q == 0) return hipSuccess;
   for (int32_t k0 = 0; k0 < q; k0 += kFlowRhs) {
     const int32_t qc = std::min(kFlowRhs, q - k0);
-    hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(64), 0, st, fc.epoch);
+    hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(256), 0, st, fc.epoch, fc.ctr, fc.nctr);
     const hipError_t e = solvePass<T, S>(dp, a, b + int64_t(k0) * n, x + int64_t(k0) * n, qc, y, fv, pz, fc, st, prof);
     if (e != hipSuccess) return e;
   }
