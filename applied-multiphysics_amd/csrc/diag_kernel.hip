@@ -3,21 +3,23 @@
 
 namespace amp {
 
-// Four waves per block. Wave w owns the columns j = 4 jj + w of the block (lane i keeps A[i][j] in rA[jj]) and the
-// rows i = 4 ii + w of the inverse W = L^-1 (lane c keeps W[i][c] in rW[ii]), so at every elimination step the
-// columns and rows still active are spread evenly over the waves. The pivot step of column k (by its owner wave):
-// read the pivot from lane k (every lane evaluates the pivot rule on the broadcast value, lane k records it), scale
-// column k, finish row k of W (Gauss-Jordan: W[k][:] *= 1 / l_kk) and publish both through LDS (double-buffered by
-// step parity, one barrier per step); the column is published twice, as l and as l d, both zero at and above the
-// pivot row. The update of step k: every wave applies the rank-1 update to its active columns
-// (A[i][j] -= (l_ik d) l_jk, l_jk read from the column's own lane through a scalar register; rows at or above the
-// pivot receive zero, and rows above the diagonal of column j are never read or stored) and rows of W
-// (W[i][:] -= l_jk W[k][:]). Lookahead: the owner of column k + 1 updates that column first and runs its pivot step
-// while the other waves finish the update of step k. The step loop is unrolled, so every register index is a
+// Eight waves per block. Wave w owns the columns j = 8 jj + w of the block (lane i keeps A[i][j] in rA[jj]) and the
+// rows i = 8 ii + w of the inverse W = L^-1 (lane c keeps W[i][c] in rW[ii]), so at every elimination step the
+// columns and rows still active are spread evenly over the waves and each wave has at most eight to update. The pivot
+// step of column k (by its owner wave): read the pivot from lane k (every lane evaluates the pivot rule on the
+// broadcast value, lane k records it), scale column k, finish row k of W (Gauss-Jordan: W[k][:] *= 1 / l_kk) and
+// publish both through LDS (double-buffered by step parity, one barrier per step); the column is published twice, as
+// l and as l d, both zero at and above the pivot row. The update of step k: every wave applies the rank-1 update to
+// its active columns (A[i][j] -= (l_ik d) l_jk, with l_jk read as an LDS broadcast of entry j of the published column;
+// rows at or above the pivot receive zero, and rows above the diagonal of column j are never read or stored) and rows
+// of W (W[i][:] -= l_jk W[k][:]). Lookahead: the owner of column k + 1 updates that column first and runs its pivot
+// step while the other waves finish the update of step k. The step loop is unrolled, so every register index is a
 // constant.
+constexpr int kDiagWaves = 8;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
-  constexpr int E = kTileEdge, J = kTileEdge / 4;
+__global__ __launch_bounds__(64 * kDiagWaves) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
+  constexpr int E = kTileEdge, NW = kDiagWaves, J = kTileEdge / NW;
   __shared__ T colL[2][E], colD[2][E], rowW[2][E];
   const DiagTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63, nb = t.nb;
@@ -26,12 +28,12 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
   T rA[J], rW[J];
 #pragma unroll
   for (int jj = 0; jj < J; jj++) {
-    const int j = 4 * jj + w;
+    const int j = NW * jj + w;
     rA[jj] = (j < nb && lane < nb && lane >= j) ? F[lane + int64_t(j) * t.ld] : T(0);
     rW[jj] = j == lane ? T(1) : T(0);
   }
   auto pivotStep = [&](const int k) {
-    const int kk = k >> 2, p = k & 1;
+    const int kk = k / NW, p = k & 1;
     T inv, mult;
     const T dk = laneValue(rA[kk], k);
     const T piv = pivotRule<T>(dk, t.col0 + k, a, &inv, &mult, lane == k);
@@ -50,20 +52,20 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
   for (int k = 0; k < E; k++) {
     if (k >= nb) continue;
     const int p = k & 1, k1 = k + 1;
-    const T cl = colL[p][lane], cd = colD[p][lane], rw = rowW[p][lane];
-    const bool ahead = k1 < nb && w == (k1 & 3);
+    const T cd = colD[p][lane], rw = rowW[p][lane];
+    const bool ahead = k1 < nb && w == (k1 % NW);
     if (k1 < E && ahead) {
-      const int jj = k1 >> 2;
-      const T ljk = laneValue(cl, k1);
+      const int jj = k1 / NW;
+      const T ljk = colL[p][k1];
       rA[jj] -= cd * ljk;
       rW[jj] -= ljk * rw;
       pivotStep(k1);
     }
 #pragma unroll
     for (int jj = 0; jj < J; jj++) {
-      const int j = 4 * jj + w;
-      if (4 * jj + 3 > k && j > k && !(ahead && j == k1)) {
-        const T ljk = laneValue(cl, j);
+      const int j = NW * jj + w;
+      if (NW * jj + NW - 1 > k && j > k && !(ahead && j == k1)) {
+        const T ljk = colL[p][j];
         rA[jj] -= cd * ljk;
         rW[jj] -= ljk * rw;
       }
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
   T* Wt = a.W + t.w_off;
 #pragma unroll
   for (int jj = 0; jj < J; jj++) {
-    const int j = 4 * jj + w;
+    const int j = NW * jj + w;
     if (j < nb && lane < nb && lane >= j) F[lane + int64_t(j) * t.ld] = rA[jj];
     Wt[j + E * lane] = (j < nb && lane < nb) ? rW[jj] : T(0);
   }
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(256) void k_diag_block(const DiagTask* __restrict__
 
 template <typename T>
 void launchDiagBlocks(const DiagTask* tasks, unsigned count, const FactorArgs<T>& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_block<T>, dim3(count), dim3(256), 0, st, tasks, a);
+  hipLaunchKernelGGL(k_diag_block<T>, dim3(count), dim3(64 * kDiagWaves), 0, st, tasks, a);
 }
 
 template void launchDiagBlocks<double>(const DiagTask*, unsigned, const FactorArgs<double>&, hipStream_t);
