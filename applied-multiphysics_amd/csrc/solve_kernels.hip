@@ -563,38 +563,51 @@ __global__ __launch_bounds__(256) void k_solve_bwd_gather(const int32_t* __restr
   }
 }
 
-// Partial products of the backward initialisation: for the task's column block and below rows [r0, r1),
-// pz[slot][jj][c] = sum_r L[nc + r, c] y[below r] (y[below] gathered into the front vector). Wave w handles columns
-// 16 w .. 16 w + 15 (lane & 15), lane >> 4 selects every fourth row, so each load instruction reads four consecutive
-// rows of 16 columns.
+// Partial products of the backward initialisation: for the task's column block and below rows [r0, r1) (at most
+// kInitRows), pz[slot][jj][c] = sum_r L[nc + r, c] y[below r] (y[below] gathered into the front vector). Wave w handles
+// columns 16 w .. 16 w + 15, four at a time; lane i takes rows r0 + i, r0 + 64 + i, ..., so every load instruction
+// reads 64 consecutive rows of one column, and the column sums are reduced across the wave at the end.
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a,
                                                         const S* fv, S* pz, int32_t q, int64_t fvN) {
+  constexpr int E = kTileEdge, CG = 4, NR = kInitRows / E;
   const BlockTask t = tasks[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int cj = 16 * w + (lane & 15), rs = lane >> 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t s = t.s, nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t col = t.blk * kTileEdge + cj;
-  const bool live = col < nc;
-  S acc[Q];
+  const T* Lb = a.Lx + a.Lxp[s] + nc;
+  const S* yb = fv + a.Sp[s] + nc;
+#pragma unroll 1
+  for (int g = 0; g < 16 / CG; g++) {
+    const int cj0 = 16 * w + CG * g;  // first column (in the block) of this group
+    S acc[CG][Q];
 #pragma unroll
-  for (int jj = 0; jj < Q; jj++) acc[jj] = S(0);
-  if (live) {
-    const T* Lb = a.Lx + a.Lxp[s] + int64_t(col) * m + nc;
-    const S* yb = fv + a.Sp[s] + nc;
-#pragma unroll 8
-    for (int32_t i = t.r0 + rs; i < t.r1; i += 4) {
-      const S lv = S(Lb[i]);
+    for (int c = 0; c < CG; c++)
 #pragma unroll
-      for (int jj = 0; jj < Q; jj++)
-        if (jj < q) acc[jj] += lv * yb[jj * fvN + i];
+      for (int jj = 0; jj < Q; jj++) acc[c][jj] = S(0);
+#pragma unroll
+    for (int u = 0; u < NR; u++) {
+      const int32_t i = t.r0 + E * u + lane;
+      const bool row = i < t.r1;
+      S yv[Q];
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) yv[jj] = (row && jj < q) ? yb[jj * fvN + i] : S(0);
+#pragma unroll
+      for (int c = 0; c < CG; c++) {
+        const int32_t col = t.blk * E + cj0 + c;
+        const S lv = (row && col < nc) ? S(Lb[int64_t(col) * m + i]) : S(0);
+#pragma unroll
+        for (int jj = 0; jj < Q; jj++) acc[c][jj] += lv * yv[jj];
+      }
     }
-  }
 #pragma unroll
-  for (int jj = 0; jj < Q; jj++) {
-    acc[jj] += __shfl_xor(acc[jj], 16, 64);
-    acc[jj] += __shfl_xor(acc[jj], 32, 64);
-    if (jj < q && rs == 0) pz[(int64_t(t.p0) * q + jj) * kTileEdge + cj] = acc[jj];
+    for (int c = 0; c < CG; c++)
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        S v = acc[c][jj];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (jj < q && lane == c) pz[(int64_t(t.p0) * q + jj) * E + cj0 + c] = v;
+      }
   }
 }
 
