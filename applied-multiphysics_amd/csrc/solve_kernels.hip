@@ -217,28 +217,16 @@ namespace {
 constexpr uint32_t kFlowSpinLimit = 1u << 22;  // about half a second of polling before a wait gives up
 constexpr int kFlowRun = 8;                    // blocks consumed per synchronisation of a flow task
 
-// Waits (thread 0 polls) until the flag of block `k` is raised and returns the end of the run of raised flags that
-// starts at k and stops at `lim` (exclusive; lim < k walks downwards). A block is published only after every block it
-// consumes, so when block lim - 1 (lim + 1 downwards) is raised, so is the whole run; the two flags are read together.
-// The flags are hints: the values themselves are read from the mailboxes (boxLoad), which validate every word. A wait that exceeds the spin limit, or that finds another wait already failed, sets *err
-// and proceeds, so the launch always completes; the host reports the error.
-__device__ inline int32_t flowReady(const int32_t* flags, int32_t k, int32_t lim, int32_t want, int32_t* err) {
+// The run of blocks a consumer stages at once: from block `k` up to `lim` (exclusive; lim < k walks downwards) when
+// the flag of block lim - 1 (lim + 1 downwards) is raised (a block is published only after every block it consumes,
+// so then the whole run is complete), else block k alone. Never waits: the consumer's mailbox loads wait for the
+// values themselves (one device round trip after they are stored), and the flags only let a consumer that starts late
+// take a run of finished blocks in one pass.
+__device__ inline int32_t flowRun(const int32_t* flags, int32_t k, int32_t lim, int32_t want) {
   __shared__ int32_t s_r;
   if (threadIdx.x == 0) {
     const int32_t step = lim > k ? 1 : -1, last = lim - step;
-    uint32_t spins = 0;
-    int32_t fk, fl;
-    // relaxed polling: no cache invalidation (the mailbox words are read with device-coherent loads)
-    for (;;) {
-      fk = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      fl = __hip_atomic_load(flags + last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (fk == want) break;
-      if (++spins >= kFlowSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-        atomicOr(err, 1);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
+    const int32_t fl = last == k ? want : __hip_atomic_load(flags + last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_r = fl == want ? lim : k + step;
   }
   __syncthreads();
@@ -272,7 +260,9 @@ __device__ inline void boxStore(uint64_t* box, int64_t slot, int32_t want, doubl
 __device__ inline uint64_t boxWord(const uint64_t* p, int32_t want, int32_t* err) {
   uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t spins = 0; uint32_t(w >> 32) != uint32_t(want);) {
-    if (++spins >= kFlowSpinLimit) {
+    // give up at the spin limit, or early once any other wait of the launch has given up
+    if (++spins >= kFlowSpinLimit ||
+        ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
       atomicOr(err, 1);
       break;
     }
@@ -361,7 +351,7 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
     __syncthreads();
   }
   for (int32_t k0 = 0; k0 < (local0 ? 0 : steps);) {
-    const int32_t k1 = flowReady(c.fwdFlag + t.flag0, k0, min(steps, k0 + kFlowRun), want, c.err);
+    const int32_t k1 = flowRun(c.fwdFlag + t.flag0, k0, min(steps, k0 + kFlowRun), want);
     for (int idx = tid; idx < (k1 - k0) * E; idx += 256) {
       const int32_t col = k0 * E + idx;
 #pragma unroll
@@ -651,7 +641,7 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
   const int32_t kEnd = t.blk;  // blocks nb - 1 down to blk + 1
   if (nb - 1 > kEnd) loadL(cur, nb - 1);
   for (int32_t k0 = nb - 1; k0 > kEnd;) {
-    const int32_t k1 = flowReady(c.bwdFlag + t.flag0, k0, max(kEnd, k0 - kFlowRun), want, c.err);
+    const int32_t k1 = flowRun(c.bwdFlag + t.flag0, k0, max(kEnd, k0 - kFlowRun), want);
     // blocks k0 down to k1 + 1 are complete; stage their x (block k at slot k0 - k)
     for (int idx = tid; idx < (k0 - k1) * E; idx += 256) {
       const int32_t row = (k0 - idx / E) * E + (idx % E);
