@@ -88,6 +88,33 @@ __device__ inline T pivotRule(T d, int32_t col, const FactorArgs<T>& a, T* inv, 
   return T(1);
 }
 
+// Branch-free form of pivotRule for a kernel built for one factorization kind (LDLT). Every lane evaluates the rule on
+// the broadcast pivot d; the lane with `record` keeps the column's results in its own registers (rec_d: D, rec_p:
+// perturbation flag, rec_fail: the column if the pivot fails, else unchanged) for a single store at the end of the
+// kernel, so the elimination step has no branch and no memory access.
+template <typename T, bool LDLT>
+__device__ inline T pivotRuleNB(T d, int32_t col, T tau, T* inv, T* mult, bool record, T& rec_d, uint8_t& rec_p,
+                                int32_t& rec_fail) {
+  if constexpr (!LDLT) {
+    rec_fail = (record && !(d > T(0))) ? min(rec_fail, col) : rec_fail;
+    const T r = invSqrt(d);
+    *inv = r;
+    *mult = T(1);
+    return d * r;
+  } else {
+    const bool small = !(fabs(d) >= tau);
+    const bool bad = small && !isfinite(d);
+    const bool pert = small && !bad;
+    d = pert ? (d < T(0) ? -tau : tau) : d;
+    rec_fail = (record && bad) ? min(rec_fail, col) : rec_fail;
+    rec_d = record ? d : rec_d;
+    rec_p = record ? uint8_t(pert) : rec_p;
+    *inv = recip(d);
+    *mult = d;
+    return T(1);
+  }
+}
+
 template <typename T>
 void launchDiagBlocks(const DiagTask* tasks, unsigned count, const FactorArgs<T>& a, hipStream_t st);
 
