@@ -155,12 +155,13 @@ __global__ __launch_bounds__(256) void k_small_front_wave(const int32_t* __restr
 // Extend-add into parent columns [j0, j1) of a tiled front. The plan lists, per work item, the column range of
 // every child that lands there (each child column maps to one parent column through the increasing relative map).
 // Children are added one after another (reference order); a child's entries land on distinct parent entries, so the
-// waves split a child's columns, each lane keeping eight rows in flight.
+// waves split a child's columns and their row chunks, each lane keeping eight rows in flight.
 template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
                                                     const EaRange* __restrict__ ranges, FactorArgs<T> a) {
-  constexpr int R = 8;
+  constexpr int R = 8, CH = 64 * R, NWV = 4;  // rows in flight per lane, rows per chunk, waves
   const EaTask t = tasks[blockIdx.x];
+  const int32_t wave = int32_t(threadIdx.x >> 6);
   const int32_t s = t.s;
   const int32_t nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   T* L = a.Lx + a.Lxp[s];
@@ -172,27 +173,37 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
     const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
     const int32_t* rel = a.rel + a.Sp[c] + ncc;
     const T* Uc = a.U + a.Uoff[c];
-    // child column b lands in parent column rel[b], rows rel[r] for r >= b
-    for (int32_t b = g.b0 + int32_t(threadIdx.x >> 6); b < g.b1; b += int32_t(blockDim.x >> 6)) {
+    // child column b lands in parent column rel[b], rows rel[r] for r >= b; its rows are cut into chunks of 64 R,
+    // and the (column, chunk) pairs of the range are dealt to the waves in turn, so that a few long columns still
+    // keep every wave busy
+    int32_t slot = 0;
+    for (int32_t b = g.b0; b < g.b1; b++) {
+      const int32_t nch = (muc - b + CH - 1) / CH;
+      const int32_t first = (wave - slot % NWV + NWV) % NWV;  // this wave's first chunk of the column
+      slot += nch;
+      if (first >= nch) continue;
       const int32_t j = rel[b];
       const T* src = Uc + int64_t(b) * muc;
       T* dst = j < nc ? L + int64_t(j) * m : Us + int64_t(j - nc) * mu;
       const int32_t sh = j < nc ? 0 : nc;  // parent row -> row of the column's storage
-      int32_t r = b + lane;
-      for (; r + 64 * (R - 1) < muc; r += 64 * R) {
-        int32_t d[R];
-        T v[R], w[R];
+      for (int32_t ch = first; ch < nch; ch += NWV) {
+        int32_t r = b + ch * CH + lane;
+        if (b + (ch + 1) * CH <= muc) {
+          int32_t d[R];
+          T v[R], w[R];
 #pragma unroll
-        for (int u = 0; u < R; u++) {
-          d[u] = rel[r + 64 * u] - sh;
-          v[u] = src[r + 64 * u];
+          for (int u = 0; u < R; u++) {
+            d[u] = rel[r + 64 * u] - sh;
+            v[u] = src[r + 64 * u];
+          }
+#pragma unroll
+          for (int u = 0; u < R; u++) w[u] = dst[d[u]];
+#pragma unroll
+          for (int u = 0; u < R; u++) dst[d[u]] = w[u] + v[u];
+        } else {
+          for (; r < muc; r += 64) dst[rel[r] - sh] += src[r];
         }
-#pragma unroll
-        for (int u = 0; u < R; u++) w[u] = dst[d[u]];
-#pragma unroll
-        for (int u = 0; u < R; u++) dst[d[u]] = w[u] + v[u];
       }
-      for (; r < muc; r += 64) dst[rel[r] - sh] += src[r];
     }
     __syncthreads();
   }

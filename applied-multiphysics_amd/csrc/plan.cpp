@@ -172,7 +172,7 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
       for (int32_t s : big) {
         if (S.childHead[s] == -1) continue;
         const int32_t nc = frontCols(S, s);
-        for (int32_t j = 0; j < nc; j += kEaWidth) addEa(s, j, std::min(nc, j + kEaWidth));
+        for (int32_t j = 0; j < nc; j += kEaWidthF) addEa(s, j, std::min(nc, j + kEaWidthF));
       }
       push(kExtendAdd, L, first, int64_t(P.ea.size()) - first);
     }

@@ -16,7 +16,8 @@ namespace amp {
 constexpr int kTileEdge = 64;   // panel width of tiled fronts (diagonal blocks); block size of their solves
 constexpr int kGemmEdge = 128;  // output-tile edge of update products in launches with enough tiles
 constexpr int kWideLaunch = 512;  // tiles of edge kGemmEdge a launch needs to use them (else kTileEdge)
-constexpr int kEaWidth = 16;    // parent columns per extend-add work item
+constexpr int kEaWidth = 16;    // parent columns per extend-add work item into an update matrix
+constexpr int kEaWidthF = 4;    // parent columns per extend-add work item into the factor columns (on the chain)
 constexpr int kSuperPanel = 256;  // columns brought up to date together (left-looking) in a tiled front
 constexpr int kSmallMax = 96;   // fronts with m <= kSmallMax are factored (and solved) whole by one workgroup
 constexpr int kWaveFrontMax = 32;  // fronts with m <= this are factored by one wave each (four per workgroup)
