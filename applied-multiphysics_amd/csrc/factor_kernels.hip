@@ -162,6 +162,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
   constexpr int R = 8, CH = 64 * R, NWV = 4;  // rows in flight per lane, rows per chunk, waves
   const EaTask t = tasks[blockIdx.x];
   const int32_t wave = int32_t(threadIdx.x >> 6);
+  const bool factorCols = t.j0 < a.superStart[t.s + 1] - a.superStart[t.s];
   const int32_t s = t.s;
   const int32_t nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   T* L = a.Lx + a.Lxp[s];
@@ -173,37 +174,41 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
     const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
     const int32_t* rel = a.rel + a.Sp[c] + ncc;
     const T* Uc = a.U + a.Uoff[c];
-    // child column b lands in parent column rel[b], rows rel[r] for r >= b; its rows are cut into chunks of 64 R,
-    // and the (column, chunk) pairs of the range are dealt to the waves in turn, so that a few long columns still
-    // keep every wave busy
-    int32_t slot = 0;
-    for (int32_t b = g.b0; b < g.b1; b++) {
-      const int32_t nch = (muc - b + CH - 1) / CH;
-      const int32_t first = (wave - slot % NWV + NWV) % NWV;  // this wave's first chunk of the column
-      slot += nch;
-      if (first >= nch) continue;
+    // child column b lands in parent column rel[b], rows rel[r] for r >= b (lane r - b mod 64, R rows in flight)
+    auto addRows = [&](int32_t b, int32_t r, int32_t rEnd) {
       const int32_t j = rel[b];
       const T* src = Uc + int64_t(b) * muc;
       T* dst = j < nc ? L + int64_t(j) * m : Us + int64_t(j - nc) * mu;
       const int32_t sh = j < nc ? 0 : nc;  // parent row -> row of the column's storage
-      for (int32_t ch = first; ch < nch; ch += NWV) {
-        int32_t r = b + ch * CH + lane;
-        if (b + (ch + 1) * CH <= muc) {
-          int32_t d[R];
-          T v[R], w[R];
+      for (; r + 64 * (R - 1) < rEnd; r += 64 * R) {
+        int32_t d[R];
+        T v[R], w[R];
 #pragma unroll
-          for (int u = 0; u < R; u++) {
-            d[u] = rel[r + 64 * u] - sh;
-            v[u] = src[r + 64 * u];
-          }
-#pragma unroll
-          for (int u = 0; u < R; u++) w[u] = dst[d[u]];
-#pragma unroll
-          for (int u = 0; u < R; u++) dst[d[u]] = w[u] + v[u];
-        } else {
-          for (; r < muc; r += 64) dst[rel[r] - sh] += src[r];
+        for (int u = 0; u < R; u++) {
+          d[u] = rel[r + 64 * u] - sh;
+          v[u] = src[r + 64 * u];
         }
+#pragma unroll
+        for (int u = 0; u < R; u++) w[u] = dst[d[u]];
+#pragma unroll
+        for (int u = 0; u < R; u++) dst[d[u]] = w[u] + v[u];
       }
+      for (; r < rEnd; r += 64) dst[rel[r] - sh] += src[r];
+    };
+    if (factorCols) {
+      // into the factor columns (on the panel chain, few work items): the rows of each child column are cut into
+      // chunks of 64 R and the (column, chunk) pairs are dealt to the waves in turn, so that a few long columns still
+      // keep every wave busy
+      int32_t slot = 0;
+      for (int32_t b = g.b0; b < g.b1; b++) {
+        const int32_t nch = (muc - b + CH - 1) / CH;
+        const int32_t first = (wave - slot % NWV + NWV) % NWV;  // this wave's first chunk of the column
+        slot += nch;
+        for (int32_t ch = first; ch < nch; ch += NWV) addRows(b, b + ch * CH + lane, min(muc, b + (ch + 1) * CH));
+      }
+    } else {
+      // into an update matrix (many work items): the waves split the child's columns
+      for (int32_t b = g.b0 + wave; b < g.b1; b += NWV) addRows(b, b + lane, muc);
     }
     __syncthreads();
   }
