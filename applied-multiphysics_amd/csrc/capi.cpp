@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -29,14 +30,25 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
-void finishSym(amp_sym* h) {
+// Factor and solve plans of the analysis; AMP_ERR_ARG when the system exceeds the plans' index ranges.
+int buildPlans(amp_sym* h, const amp::PlanParams& p) {
+  try {
+    h->F = amp::buildFactorPlan(h->S, p);
+    h->SP = amp::buildSolvePlan(h->S, h->F);
+  } catch (const std::length_error& e) {
+    return fail(AMP_ERR_ARG, std::string("analyze: ") + e.what());
+  }
+  return AMP_OK;
+}
+
+int finishSym(amp_sym* h) {
   const amp::Symbolic& S = h->S;
-  h->F = amp::buildFactorPlan(S, h->params);
-  h->SP = amp::buildSolvePlan(S, h->F);
+  if (int rc = buildPlans(h, h->params)) return rc;
   h->diagSrc.assign(S.n, -1);
   for (int32_t j = 0; j < S.n; j++)
     for (int32_t q = S.lowerP[j]; q < S.lowerP[j + 1]; q++)
       if (S.lowerI[q] == j) h->diagSrc[j] = S.lowerSrc[q];
+  return AMP_OK;
 }
 
 int symToDevice(amp_sym* h) {
@@ -289,7 +301,7 @@ int amp_analyze(int32_t n, const int32_t* rowptr, const int32_t* colidx, const i
     return fail(AMP_ERR_INTERNAL, "analyze: amdOrder internal error");
   h->rowptr.assign(rowptr, rowptr + n + 1);
   h->colidx.assign(colidx, colidx + rowptr[n]);
-  finishSym(h.get());
+  if (int rc = finishSym(h.get())) return rc;
   *out = h.release();
   return AMP_OK;
 }
@@ -360,7 +372,7 @@ int amp_sym_import(const amp_sym_desc* d, amp_sym** out) {
     }
   }
   S.fill = S.nnzA ? double(S.nnzL) / double(S.nnzA) : 1;
-  finishSym(h.get());
+  if (int rc = finishSym(h.get())) return rc;
   *out = h.release();
   return AMP_OK;
 }
@@ -689,9 +701,8 @@ namespace {
 
 int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
   h->params = p;
-  h->F = amp::buildFactorPlan(h->S, p);
-  h->SP = amp::buildSolvePlan(h->S, h->F);
   h->onDevice = false;
+  if (int rc = buildPlans(h, p)) return rc;
   return symToDevice(h);
 }
 

@@ -223,9 +223,14 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
 // double-buffer the chunks in LDS (one barrier per chunk).
 // Workgroups are dispatched round-robin over the eight XCDs; each XCD is given a contiguous run of the task list,
 // whose neighbouring tiles share operand panels in that XCD's L2.
-template <typename T, int G>
+// Loads are issued on every path (addresses clamped into the operands, chunks past the end re-read its last depth),
+// and the entries outside an operand are replaced by zeros when a chunk is written to LDS: the wait before a chunk is
+// written then counts only the loads issued before it, so the chunk loaded two steps ahead stays in flight.
+// LDLT: the kernel built for LDL^T products scales the B operand by the pivots of its depths (B D) when a chunk is
+// written to LDS (update tasks; the triangular-solve tasks are not scaled).
+template <typename T, int G, bool LDLT>
 __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U,
-                                                                   T* W, const T* __restrict__ D, int32_t ldlt) {
+                                                                   T* W, const T* __restrict__ D) {
   using V4 = typename Vec4<T>::type;
   constexpr int WRW = G == 128 ? 4 : 2, WCW = 2, NT = 64 * WRW * WCW;  // wave grid, threads
   constexpr int WTR = G / WRW, WTC = G / WCW, NBI = WTR / 16, NBJ = WTC / 16;
@@ -238,16 +243,21 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   __shared__ T Bs[NBUF][KC * PA];
   const int nblk = gridDim.x, q8 = nblk / 8, r8 = nblk % 8, x8 = blockIdx.x % 8;
   const GemmTask t = tasks[x8 * q8 + min(x8, r8) + blockIdx.x / 8];
-  T* const bases[3] = {Lx, U, W};
-  const T* A = bases[t.a_buf] + t.a_off;
-  const T* B = bases[t.b_buf] + t.b_off;
-  T* C = bases[t.c_buf] + t.c_off;
+  // buffers chosen by selects of the kernel arguments (not an indexed array), so that the compiler keeps them in the
+  // global address space (global loads and stores rather than flat ones, which LDS waits would also cover)
+  auto base = [&](int32_t b) { return b == 0 ? Lx : b == 1 ? U : W; };
+  const T* A = base(t.a_buf) + t.a_off;
+  const T* B = base(t.b_buf) + t.b_off;
+  T* C = base(t.c_buf) + t.c_off;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave % WRW) * WTR, wc = (wave / WRW) * WTC;
   const bool active = wr < t.rows && wc < t.cols;  // uniform per wave
-  const bool scaleDepth = ldlt && t.mode != 2;
+  const bool scaleDepth = LDLT && t.mode != 2;
   const int li = tid % G, lk = tid / G;  // loader: row li, depths lk + LS r
   const bool rowA = li < t.rows, rowB = li < t.cols;
+  // in-tile offsets are 32-bit (depth x leading dimension < 2^31, checked by the plan)
+  const T* Ar = A + min(li, t.rows - 1);
+  const T* Br = B + min(li, t.cols - 1);
   V4 acc[NBI][NBJ];
 #pragma unroll
   for (int bi = 0; bi < NBI; bi++)
@@ -256,24 +266,26 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   // two register sets of loaded depth values: a chunk is loaded two chunks before its turn to feed the matrix
   // cores, so its global-memory latency is covered by two chunks of products
   T pa[2][NL], pb[2][NL];
+  T pd[LDLT ? 2 : 1][LDLT ? NL : 1];  // LDLT: pivots of the loaded depths
   auto load = [&](auto set, int k0) {
     constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int r = 0; r < NL; r++) {
-      const int k = k0 + lk + LS * r;
-      const bool kin = k < t.k;
-      pa[S][r] = (kin && rowA) ? A[li + int64_t(k) * t.lda] : T(0);
-      T bv = (kin && rowB) ? B[li + int64_t(k) * t.ldb] : T(0);
-      if (scaleDepth && kin) bv *= D[t.d_off + k];
-      pb[S][r] = bv;
+      const int k = min(k0 + lk + LS * r, t.k - 1);
+      pa[S][r] = Ar[k * t.lda];
+      pb[S][r] = Br[k * t.ldb];
+      if constexpr (LDLT) pd[S][r] = D[t.d_off + k];
     }
   };
-  auto stash = [&](auto set, int buf) {
+  auto stash = [&](auto set, int buf, int k0) {
     constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int r = 0; r < NL; r++) {
-      As[buf][(lk + LS * r) * PA + li] = pa[S][r];
-      Bs[buf][(lk + LS * r) * PA + li] = pb[S][r];
+      const bool kin = k0 + lk + LS * r < t.k;
+      T bv = pb[S][r];
+      if constexpr (LDLT) bv = scaleDepth ? bv * pd[S][r] : bv;
+      As[buf][(lk + LS * r) * PA + li] = (kin && rowA) ? pa[S][r] : T(0);
+      Bs[buf][(lk + LS * r) * PA + li] = (kin && rowB) ? bv : T(0);
     }
   };
   auto compute = [&](int buf) {
@@ -295,12 +307,12 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   load(S0{}, 0);
-  if (KC < t.k) load(S1{}, KC);
+  load(S1{}, KC);
   if constexpr (NBUF == 2) {
     // double-buffered LDS: chunk c is read from buffer c & 1 while chunk c + 1 (already in registers) is written to
     // the other, and chunk c + 2 is loaded into the register set chunk c left; one barrier per chunk. Chunk c's
     // register set is c & 1 (the loop is unrolled by two so that the sets are compile-time).
-    stash(S0{}, 0);
+    stash(S0{}, 0, 0);
     __syncthreads();
     auto step = [&](auto set, int k0) {  // `set`: register set of chunk c + 1; buffer of chunk c is set ^ 1
       constexpr int S = decltype(set)::value;
@@ -308,9 +320,9 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
       // across the loop
       int buf = S ^ 1;
       asm volatile("" : "+s"(buf));
-      if (k0 + 2 * KC < t.k) load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
+      load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
       compute(buf);
-      if (k0 + KC < t.k) stash(set, buf ^ 1);
+      if (k0 + KC < t.k) stash(set, buf ^ 1, k0 + KC);
       __syncthreads();
     };
     for (int k0 = 0; k0 < t.k; k0 += 2 * KC) {
@@ -323,9 +335,9 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
     auto step = [&](auto set, int k0) {
       int buf = 0;  // opaque, as above
       asm volatile("" : "+s"(buf));
-      stash(set, buf);
+      stash(set, buf, k0);
       __syncthreads();
-      if (k0 + 2 * KC < t.k) load(set, k0 + 2 * KC);
+      load(set, k0 + 2 * KC);
       compute(buf);
       __syncthreads();
     };
@@ -343,7 +355,7 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
     for (int r = 0; r < 4; r++) {
       const int col = wc + 16 * bj + mfmaRow<T>(lane, r);
       if (col >= t.cols) continue;
-      const T dinv = (ldlt && t.mode == 2) ? T(1) / D[t.d_off + col] : T(1);
+      const T dinv = (LDLT && t.mode == 2) ? T(1) / D[t.d_off + col] : T(1);
       T* Cc = C + int64_t(col) * t.ldc;
 #pragma unroll
       for (int bi = 0; bi < NBI; bi++) {
@@ -355,7 +367,7 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
         else if (t.mode == 1)
           Cc[row] = -v;
         else
-          Cc[row] = ldlt ? v * dinv : v;
+          Cc[row] = LDLT ? v * dinv : v;
       }
     }
 }
@@ -446,14 +458,20 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
       case kDiag:
         launchDiagBlocks<T>(dp.diag + L.first, grid, a, st);
         break;
-      case kGemm:
-        if (L.edge == kGemmEdge)
-          hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge>), dim3(grid), dim3(512), 0, st,
-                             reinterpret_cast<const GemmTask*>(dp.gemm) + L.first, a.Lx, a.U, a.W, a.D, a.ldlt);
-        else
-          hipLaunchKernelGGL((k_gemm_tile<T, kTileEdge>), dim3(grid), dim3(256), 0, st,
-                             reinterpret_cast<const GemmTask*>(dp.gemm) + L.first, a.Lx, a.U, a.W, a.D, a.ldlt);
+      case kGemm: {
+        const GemmTask* gt = reinterpret_cast<const GemmTask*>(dp.gemm) + L.first;
+        if (L.edge == kGemmEdge) {
+          if (a.ldlt)
+            hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge, true>), dim3(grid), dim3(512), 0, st, gt, a.Lx, a.U, a.W, a.D);
+          else
+            hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge, false>), dim3(grid), dim3(512), 0, st, gt, a.Lx, a.U, a.W, a.D);
+        } else if (a.ldlt) {
+          hipLaunchKernelGGL((k_gemm_tile<T, kTileEdge, true>), dim3(grid), dim3(256), 0, st, gt, a.Lx, a.U, a.W, a.D);
+        } else {
+          hipLaunchKernelGGL((k_gemm_tile<T, kTileEdge, false>), dim3(grid), dim3(256), 0, st, gt, a.Lx, a.U, a.W, a.D);
+        }
         break;
+      }
     }
     if (prof && 2 * prof->used + 1 < prof->events.size()) {
       (void)hipEventRecord(prof->events[2 * prof->used + 1], st);

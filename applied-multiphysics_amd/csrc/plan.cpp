@@ -42,6 +42,9 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
   for (int32_t s = 0; s < ns; s++) {
     const int32_t m = frontRows(S, s), nc = frontCols(S, s), mu = m - nc;
     P.isBig[s] = m > std::min(prm.smallMax, kSmallMax);
+    // the product kernels address a tile's operands with 32-bit offsets: depth x leading dimension < 2^31
+    if (P.isBig[s] && int64_t(m) * std::max(SPW, kGemmEdge) >= (int64_t(1) << 31))
+      throw std::length_error("a tiled front is too tall for 32-bit tile offsets");
     if (mu > 0) {
       P.Uoff[s] = uTotal;
       uTotal += int64_t(mu) * mu;
