@@ -308,8 +308,10 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
           }, kSyncSide | kSyncWait | kSyncSignal))
         sideUsed = true;
     }
-    // the next main-stream launch (the update-matrix extend-add, or the next level) joins the side stream
-    if (sideUsed && (P.launches.back().sync & kSyncSide)) joinPending = true;
+    // the next main-stream launch (the update-matrix extend-add, or the next level) joins the side stream: also when
+    // the level's last super-panel had no bulk launch (no front with a Schur share or columns left there), since an
+    // earlier bulk launch may still be writing update matrices that nothing on the main stream has waited for
+    if (sideUsed) joinPending = true;
     // children's contributions to the update matrix
     {
       const int64_t first = int64_t(P.ea.size());

@@ -193,22 +193,27 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
   const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
   const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(m, i0 + kGatherRows);
   if (i0 >= m) return;  // whole workgroup
+  // the slice of v is accumulated in LDS (children in order, one barrier each) and stored once
+  __shared__ S vs[kGatherRows];
   for (int32_t k = 0; k < q; k++) {
     const S* yk = y + int64_t(k) * a.n;
     const S* fk = fv + k * fvN;
     S* v = fv + k * fvN + a.Sp[s];
-    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) v[i] = i < nc ? yk[f + i] : S(0);
+    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) vs[i - i0] = i < nc ? yk[f + i] : S(0);
     __syncthreads();
     for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
       const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
       const int32_t o = a.Sp[c] + ncc;
       for (int32_t r = threadIdx.x; r < muc; r += blockDim.x) {
         const int32_t i = a.rel[o + r];
+        const S val = fk[o + r];  // loaded with the index, not after it
         if (i >= i1) break;
-        if (i >= i0) v[i] += fk[o + r];
+        if (i >= i0) vs[i - i0] += val;
       }
       __syncthreads();
     }
+    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) v[i] = vs[i - i0];
+    __syncthreads();
   }
 }
 

@@ -55,6 +55,8 @@ CASES = {
     "heat3d_20": lambda: W.heat3d(20)[:2],
     "grid27_11": lambda: grid27(11, 11, 11)[:2],
     "hex": lambda: hex_matrix(),
+    # a forest: levels that hold roots beside fronts of deeper trees
+    "batch2d_8": lambda: W.batch_heat2d(nsys=8, lo=30, hi=90, nrhs=1)[:2],
 }
 
 
@@ -118,8 +120,8 @@ def test_plan_failure_column(sim):
     assert ok is False
 
 
-@pytest.mark.parametrize("name", ["heat3d_14", "heat3d_20", "grid27_11"])
-@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128), (96, 32, 2**30, 64), (64, 16, 256, 128)])
+@pytest.mark.parametrize("name", ["heat3d_14", "heat3d_20", "grid27_11", "batch2d_8"])
+@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128), (96, 32, 2**30, 64), (64, 16, 256, 128), (64, 32, 256, 128)])
 def test_plan_two_stream_ordering(sim, name, prm):
     """Every pair of factor launches on the two streams that the plan's waits leave unordered touches disjoint
     memory, and no two tasks of one launch conflict; with the waits removed the checker must find conflicts (so it
