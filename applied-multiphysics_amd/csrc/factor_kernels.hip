@@ -234,9 +234,9 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
   using V4 = typename Vec4<T>::type;
   constexpr int WRW = G == 128 ? 4 : 2, WCW = 2, NT = 64 * WRW * WCW;  // wave grid, threads
   constexpr int WTR = G / WRW, WTC = G / WCW, NBI = WTR / 16, NBJ = WTC / 16;
-  // depth chunk, LDS row pitch (128-tiles take 16: two FP64 workgroups fit per CU, and the FP32 ones keep fewer
-  // loads in flight per thread, so more waves stay resident)
-  constexpr int KC = G == 128 ? 16 : 32, PA = G + 4;
+  // depth chunk, LDS row pitch (16 for 128-tiles and FP64: two FP64 128-workgroups fit per CU, fewer loads in flight
+  // per thread keep more waves resident; FP32 64-tiles take 32)
+  constexpr int KC = (G == 128 || sizeof(T) == 8) ? 16 : 32, PA = G + 4;
   constexpr int LS = NT / G, NL = KC / LS;            // loader: depth stride, depths per thread
   constexpr int NBUF = G == 128 ? 2 : 1;
   __shared__ T As[NBUF][KC * PA];
