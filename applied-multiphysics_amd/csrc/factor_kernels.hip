@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
 // LDLT: the kernel built for LDL^T products scales the B operand by the pivots of its depths (B D) when a chunk is
 // written to LDS (update tasks; the triangular-solve tasks are not scaled).
 template <typename T, int G, bool LDLT>
-__global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U,
+__global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U,
                                                                    T* W, const T* __restrict__ D) {
   using V4 = typename Vec4<T>::type;
   constexpr int WRW = G == 128 ? 4 : 2, WCW = 2, NT = 64 * WRW * WCW;  // wave grid, threads
@@ -314,37 +314,46 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256) void k_gemm_tile(const GemmTa
     // register set is c & 1 (the loop is unrolled by two so that the sets are compile-time).
     stash(S0{}, 0, 0);
     __syncthreads();
-    auto step = [&](auto set, int k0) {  // `set`: register set of chunk c + 1; buffer of chunk c is set ^ 1
+    // `set`: register set of chunk c + 1; buffer of chunk c is set ^ 1; `ld`: whether chunk c + 2 is loaded
+    auto step = [&](auto set, int k0, auto ld) {
       constexpr int S = decltype(set)::value;
       // the buffer index is made opaque, so that the LDS addresses of both buffers are not precomputed and kept live
       // across the loop
       int buf = S ^ 1;
       asm volatile("" : "+s"(buf));
-      load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
+      if constexpr (decltype(ld)::value) load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
       compute(buf);
       if (k0 + KC < t.k) stash(set, buf ^ 1, k0 + KC);
       __syncthreads();
     };
-    for (int k0 = 0; k0 < t.k; k0 += 2 * KC) {
-      step(S1{}, k0);
-      if (k0 + KC < t.k) step(S0{}, k0 + KC);
+    // pairs of chunks while a chunk two ahead exists (the pair's second load may re-read the last depth), then the
+    // last one or two chunks without loads: at most one chunk is loaded in vain per tile
+    int k0 = 0;
+    for (; k0 + 2 * KC < t.k; k0 += 2 * KC) {
+      step(S1{}, k0, std::true_type{});
+      step(S0{}, k0 + KC, std::true_type{});
     }
+    step(S1{}, k0, std::false_type{});
+    if (k0 + KC < t.k) step(S0{}, k0 + KC, std::false_type{});
   } else {
     // one LDS buffer: chunk c (register set c & 1) is stashed, then chunk c + 2 is loaded into the same set while
     // chunk c feeds the matrix cores
-    auto step = [&](auto set, int k0) {
+    auto step = [&](auto set, int k0, auto ld) {
       int buf = 0;  // opaque, as above
       asm volatile("" : "+s"(buf));
       stash(set, buf, k0);
       __syncthreads();
-      load(set, k0 + 2 * KC);
+      if constexpr (decltype(ld)::value) load(set, k0 + 2 * KC);
       compute(buf);
       __syncthreads();
     };
-    for (int k0 = 0; k0 < t.k; k0 += 2 * KC) {
-      step(S0{}, k0);
-      if (k0 + KC < t.k) step(S1{}, k0 + KC);
+    int k0 = 0;  // pairs while a chunk two ahead exists, then the rest without loads, as above
+    for (; k0 + 2 * KC < t.k; k0 += 2 * KC) {
+      step(S0{}, k0, std::true_type{});
+      step(S1{}, k0 + KC, std::true_type{});
     }
+    step(S0{}, k0, std::false_type{});
+    if (k0 + KC < t.k) step(S1{}, k0 + KC, std::false_type{});
   }
   if (!active) return;
   // component r of acc[bi][bj]: C row wr + 16 bi + (lane & 15), C column wc + 16 bj + mfmaRow(lane, r)
