@@ -358,6 +358,37 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
   if (!active) return;
   // component r of acc[bi][bj]: C row wr + 16 bi + (lane & 15), C column wc + 16 bj + mfmaRow(lane, r)
   const int row0 = wr + (lane & 15);
+  if (t.mode == 0) {
+    // C -= A B^T, one 16-column block at a time: its C entries are all read (addresses clamped into the tile) before
+    // any is written, so a block costs one memory round trip (the compiler keeps a store before every later load of
+    // C; reading and writing entry by entry cost a round trip per entry)
+    constexpr int BJB = 1;  // column blocks per batch (more cost registers and were not faster)
+#pragma unroll
+    for (int bj0 = 0; bj0 < NBJ; bj0 += BJB) {
+      T cv[BJB][4][NBI];
+#pragma unroll
+      for (int b = 0; b < BJB; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const T* Cc = C + int64_t(min(wc + 16 * (bj0 + b) + mfmaRow<T>(lane, r), t.cols - 1)) * t.ldc;
+#pragma unroll
+          for (int bi = 0; bi < NBI; bi++) cv[b][r][bi] = Cc[min(row0 + 16 * bi, t.rows - 1)];
+        }
+#pragma unroll
+      for (int b = 0; b < BJB; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int col = wc + 16 * (bj0 + b) + mfmaRow<T>(lane, r);
+          T* Cc = C + int64_t(col) * t.ldc;
+#pragma unroll
+          for (int bi = 0; bi < NBI; bi++) {
+            const int row = row0 + 16 * bi;
+            if (col < t.cols && row < t.rows) Cc[row] = cv[b][r][bi] - acc[bi][bj0 + b][r];
+          }
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int bj = 0; bj < NBJ; bj++)
 #pragma unroll
@@ -371,9 +402,7 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
         const int row = row0 + 16 * bi;
         if (row >= t.rows) continue;
         const T v = acc[bi][bj][r];
-        if (t.mode == 0)
-          Cc[row] = Cc[row] - v;
-        else if (t.mode == 1)
+        if (t.mode == 1)
           Cc[row] = -v;
         else
           Cc[row] = LDLT ? v * dinv : v;
