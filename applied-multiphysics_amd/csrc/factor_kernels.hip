@@ -306,8 +306,37 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
   };
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
+  // component r of acc[bi][bj]: C row wr + 16 bi + (lane & 15), C column wc + 16 bj + mfmaRow(lane, r)
+  const int row0 = wr + (lane & 15);
+  // C -= A B^T (mode 0): the last one or two steps load no chunk, so their register sets read the C entries of column
+  // blocks 0 and 1 instead (entry r NBI + bi of a block in pa[S] then pb[S]), which arrive while the last chunks feed
+  // the matrix cores
+  static_assert(4 * NBI <= 2 * NL, "a column block of C fits in one register set");
+  auto prefetchC = [&](auto set, int bj) {
+    constexpr int S = decltype(set)::value;
+    if (t.mode != 0 || !active) return;
+    const T* Cc = C;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      Cc = C + int64_t(min(wc + 16 * bj + mfmaRow<T>(lane, r), t.cols - 1)) * t.ldc;
+#pragma unroll
+      for (int bi = 0; bi < NBI; bi++) {
+        const int u = r * NBI + bi;
+        const T v = Cc[min(row0 + 16 * bi, t.rows - 1)];
+        if (u < NL)
+          pa[S][u < NL ? u : 0] = v;
+        else
+          pb[S][u < NL ? 0 : u - NL] = v;
+      }
+    }
+  };
+  auto prefetched = [&](auto set, int u) -> T {
+    constexpr int S = decltype(set)::value;
+    return u < NL ? pa[S][u < NL ? u : 0] : pb[S][u < NL ? 0 : u - NL];
+  };
   load(S0{}, 0);
   load(S1{}, KC);
+  int k0 = 0;
   if constexpr (NBUF == 2) {
     // double-buffered LDS: chunk c is read from buffer c & 1 while chunk c + 1 (already in registers) is written to
     // the other, and chunk c + 2 is loaded into the register set chunk c left; one barrier per chunk. Chunk c's
@@ -321,14 +350,16 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
       // across the loop
       int buf = S ^ 1;
       asm volatile("" : "+s"(buf));
-      if constexpr (decltype(ld)::value) load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
+      if constexpr (decltype(ld)::value)
+        load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * KC);
+      else
+        prefetchC(std::integral_constant<int, S ^ 1>{}, S ^ 1);  // blocks 0, 1 into sets 0, 1
       compute(buf);
       if (k0 + KC < t.k) stash(set, buf ^ 1, k0 + KC);
       __syncthreads();
     };
     // pairs of chunks while a chunk two ahead exists (the pair's second load may re-read the last depth), then the
     // last one or two chunks without loads: at most one chunk is loaded in vain per tile
-    int k0 = 0;
     for (; k0 + 2 * KC < t.k; k0 += 2 * KC) {
       step(S1{}, k0, std::true_type{});
       step(S0{}, k0 + KC, std::true_type{});
@@ -343,11 +374,14 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
       asm volatile("" : "+s"(buf));
       stash(set, buf, k0);
       __syncthreads();
-      if constexpr (decltype(ld)::value) load(set, k0 + 2 * KC);
+      if constexpr (decltype(ld)::value)
+        load(set, k0 + 2 * KC);
+      else
+        prefetchC(set, decltype(set)::value);  // blocks 0, 1 into sets 0, 1
       compute(buf);
       __syncthreads();
     };
-    int k0 = 0;  // pairs while a chunk two ahead exists, then the rest without loads, as above
+    // pairs while a chunk two ahead exists, then the rest without loads, as above
     for (; k0 + 2 * KC < t.k; k0 += 2 * KC) {
       step(S0{}, k0, std::true_type{});
       step(S1{}, k0 + KC, std::true_type{});
@@ -356,9 +390,8 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
     if (k0 + KC < t.k) step(S1{}, k0 + KC, std::false_type{});
   }
   if (!active) return;
-  // component r of acc[bi][bj]: C row wr + 16 bi + (lane & 15), C column wc + 16 bj + mfmaRow(lane, r)
-  const int row0 = wr + (lane & 15);
   if (t.mode == 0) {
+    const bool pre1 = k0 + KC < t.k;  // the second drain step ran: block 1 is in register set 1
     // C -= A B^T, one 16-column block at a time: its C entries are all read (addresses clamped into the tile) before
     // any is written, so a block costs one memory round trip (the compiler keeps a store before every later load of
     // C; reading and writing entry by entry cost a round trip per entry)
@@ -367,13 +400,23 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
     for (int bj0 = 0; bj0 < NBJ; bj0 += BJB) {
       T cv[BJB][4][NBI];
 #pragma unroll
-      for (int b = 0; b < BJB; b++)
+      for (int b = 0; b < BJB; b++) {
+        const int bj = bj0 + b;
+        if (bj == 0 || (bj == 1 && pre1)) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const T* Cc = C + int64_t(min(wc + 16 * (bj0 + b) + mfmaRow<T>(lane, r), t.cols - 1)) * t.ldc;
+          for (int r = 0; r < 4; r++)
 #pragma unroll
-          for (int bi = 0; bi < NBI; bi++) cv[b][r][bi] = Cc[min(row0 + 16 * bi, t.rows - 1)];
+            for (int bi = 0; bi < NBI; bi++)
+              cv[b][r][bi] = bj == 0 ? prefetched(S0{}, r * NBI + bi) : prefetched(S1{}, r * NBI + bi);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const T* Cc = C + int64_t(min(wc + 16 * bj + mfmaRow<T>(lane, r), t.cols - 1)) * t.ldc;
+#pragma unroll
+            for (int bi = 0; bi < NBI; bi++) cv[b][r][bi] = Cc[min(row0 + 16 * bi, t.rows - 1)];
+          }
         }
+      }
 #pragma unroll
       for (int b = 0; b < BJB; b++)
 #pragma unroll
