@@ -29,15 +29,17 @@ __global__ void k_assemble(const int64_t* __restrict__ adst, const int32_t* __re
 }
 
 // Whole front in LDS: assemble A's columns, extend-add the children (in the reference's child order), factor the nc
-// pivot columns right-looking, store the m x nc block and the mu x mu update matrix.
+// pivot columns right-looking, store the m x nc block and the mu x mu update matrix. The triangular loops give each
+// wave whole columns and its lanes the rows of a column (no index division per entry).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ fronts, FactorArgs<T> a,
                                                     const double* __restrict__ values, int32_t maxM) {
+  constexpr int NW = NT / 64;
   extern __shared__ unsigned char smem[];
   T* F = reinterpret_cast<T*>(smem);
   T* wcol = F + maxM * maxM;
   __shared__ T s_inv, s_mult;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int32_t s = fronts[blockIdx.x];
   const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   for (int idx = tid; idx < m * m; idx += NT) F[idx] = T(0);
@@ -48,9 +50,10 @@ __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ 
     const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
     const int32_t* rel = a.rel + a.Sp[c] + ncc;
     const T* Uc = a.U + a.Uoff[c];
-    for (int idx = tid; idx < muc * muc; idx += NT) {
-      const int r = idx % muc, b = idx / muc;
-      if (r >= b) F[rel[r] + rel[b] * m] += Uc[idx];
+    for (int b = wv; b < muc; b += NW) {  // child column b, its rows r >= b
+      T* Fb = F + rel[b] * m;
+      const T* Ub = Uc + b * muc;
+      for (int r = b + lane; r < muc; r += 64) Fb[rel[r]] += Ub[r];
     }
     __syncthreads();
   }
@@ -69,13 +72,11 @@ __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ 
       wcol[i] = l * mult;
     }
     __syncthreads();
-    const int w = m - k - 1;
-    for (int idx = tid; idx < w * w; idx += NT) {
-      const int ii = idx % w, jj = idx / w;
-      if (ii >= jj) {
-        const int i = k + 1 + ii, j = k + 1 + jj;
-        F[i + j * m] -= F[i + k * m] * wcol[j];
-      }
+    const T* Fk = F + k * m;
+    for (int j = k + 1 + wv; j < m; j += NW) {  // column j, rows i >= j
+      T* Fj = F + j * m;
+      const T wj = wcol[j];
+      for (int i = j + lane; i < m; i += 64) Fj[i] -= Fk[i] * wj;
     }
     __syncthreads();
   }
@@ -83,10 +84,8 @@ __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ 
   for (int idx = tid; idx < m * nc; idx += NT) L[idx] = F[idx];
   if (mu > 0) {
     T* Us = a.U + a.Uoff[s];
-    for (int idx = tid; idx < mu * mu; idx += NT) {
-      const int r = idx % mu, b = idx / mu;
-      Us[idx] = F[(nc + r) + (nc + b) * m];
-    }
+    for (int b = wv; b < mu; b += NW)
+      for (int r = lane; r < mu; r += 64) Us[r + b * mu] = F[(nc + r) + (nc + b) * m];
   }
 }
 
