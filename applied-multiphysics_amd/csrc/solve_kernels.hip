@@ -204,11 +204,25 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
     for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
       const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
       const int32_t o = a.Sp[c] + ncc;
-      for (int32_t r = threadIdx.x; r < muc; r += blockDim.x) {
-        const int32_t i = a.rel[o + r];
-        const S val = fk[o + r];  // loaded with the index, not after it
-        if (i >= i1) break;
-        if (i >= i0) vs[i - i0] += val;
+      // eight entries per lane are read before any is used, so walking a long map to the slice costs a round trip
+      // per 8 x 256 entries, not per 256
+      constexpr int U = 8;
+      for (int32_t r0 = threadIdx.x; r0 < muc; r0 += U * int32_t(blockDim.x)) {
+        int32_t iv[U];
+        S vv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int32_t r = r0 + u * int32_t(blockDim.x);
+          iv[u] = r < muc ? a.rel[o + r] : INT32_MAX;
+          vv[u] = r < muc ? fk[o + r] : S(0);
+        }
+        bool past = false;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          past = past || iv[u] >= i1;
+          if (!past && iv[u] >= i0) vs[iv[u] - i0] += vv[u];
+        }
+        if (past) break;
       }
       __syncthreads();
     }
