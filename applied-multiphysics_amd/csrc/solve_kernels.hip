@@ -71,38 +71,55 @@ __device__ inline void gatherFront(const SolveArgs<T>& a, int32_t s, const S* yk
   }
 }
 
-// Supernodes with m <= kSmallMax, one workgroup each, all right-hand sides of the pass together: per column the q
-// solution values are formed by q threads and broadcast through LDS, then every thread updates its rows for all q.
+// threads of the whole-front solve workgroups: one per row of the largest whole front
+constexpr int kSmallSolveThreads = 128;
+static_assert(kSmallMax <= kSmallSolveThreads, "a whole-front solve keeps one row per thread");
+
+// Supernodes with m <= kSmallMax, one workgroup each (at least m threads), all right-hand sides of the pass together.
+// Thread r keeps row r of the front vector in registers and loads its entry of the next column of L while the current
+// one is applied; per column the pivot's thread forms the q solution values and broadcasts them through LDS
+// (double-buffered, so one barrier per column).
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_small_fwd(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y,
                                                          S* fv, int32_t q, int64_t fvN) {
   __shared__ S v[Q][kSmallMax];
-  __shared__ S xc[Q];
+  __shared__ S xc[2][Q];
   const int32_t s = fronts[blockIdx.x];
   const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
   const T* L = a.Lx + a.Lxp[s];
+  const int r = threadIdx.x;
+  const bool own = r < m;
+  T lcur = (own && nc > 0) ? L[r] : T(0);  // L[r][col] for col = 0
   for (int32_t k = 0; k < q; k++) gatherFront<T, S>(a, s, y + int64_t(k) * a.n, fv + k * fvN, v[k], nc, m);
+  S vr[Q];
+#pragma unroll
+  for (int k = 0; k < Q; k++) vr[k] = (own && k < q) ? v[k][r] : S(0);
   for (int col = 0; col < nc; col++) {
-    if (threadIdx.x < q) {
-      const S x = a.ldlt ? v[threadIdx.x][col] : v[threadIdx.x][col] / S(L[col + col * m]);
-      v[threadIdx.x][col] = x;
-      xc[threadIdx.x] = x;
+    const T lnext = (own && col + 1 < nc) ? L[r + int64_t(col + 1) * m] : T(0);
+    const int b = col & 1;
+    if (r == col) {
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        if (k >= q) continue;
+        const S x = a.ldlt ? vr[k] : vr[k] / S(lcur);
+        vr[k] = x;
+        xc[b][k] = x;
+      }
     }
     __syncthreads();
-    for (int r = col + 1 + threadIdx.x; r < m; r += blockDim.x) {
-      const S l = S(L[r + col * m]);
+    if (own && r > col) {
 #pragma unroll
       for (int k = 0; k < Q; k++)
-        if (k < q) v[k][r] -= l * xc[k];
+        if (k < q) vr[k] -= S(lcur) * xc[b][k];
     }
-    __syncthreads();
+    lcur = lnext;
   }
-  for (int32_t k = 0; k < q; k++) {
-    S* yk = y + int64_t(k) * a.n;
-    S* fk = fv + k * fvN;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      if (i < nc) yk[f + i] = v[k][i];
-      fk[a.Sp[s] + i] = v[k][i];
+  if (own) {
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      if (k >= q) continue;
+      if (r < nc) y[int64_t(k) * a.n + f + r] = vr[k];
+      fv[k * fvN + a.Sp[s] + r] = vr[k];
     }
   }
 }
@@ -449,8 +466,11 @@ __global__ __launch_bounds__(256) void k_solve_fwd_flow(const BlockTask* __restr
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y,
                                                          int32_t q) {
-  __shared__ S z[Q][kSmallMax], xb[Q][kSmallMax];
-  __shared__ S xc[Q];
+  // thread i keeps solution row i (i < nc, at most one row per thread) in registers: z_i = y_i - sum_r L[nc + r][i]
+  // y[below r], then the backward substitution loads its entry of the next row of L while the current one is applied,
+  // one barrier per column (the broadcast of the pivot's values is double-buffered)
+  __shared__ S xb[Q][kSmallMax];
+  __shared__ S xc[2][Q];
   const int32_t s = fronts[blockIdx.x];
   const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s], mu = m - nc;
   const T* L = a.Lx + a.Lxp[s];
@@ -458,38 +478,46 @@ __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restri
   for (int32_t k = 0; k < q; k++)
     for (int r = threadIdx.x; r < mu; r += blockDim.x) xb[k][r] = y[int64_t(k) * a.n + rows[r]];
   __syncthreads();
-  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-    S sum[Q];
+  const int i = threadIdx.x;
+  const bool own = i < nc;
+  const T* Li = L + int64_t(own ? i : 0) * m;
+  S zr[Q];
 #pragma unroll
-    for (int k = 0; k < Q; k++) sum[k] = k < q ? y[int64_t(k) * a.n + f + i] : S(0);
+  for (int k = 0; k < Q; k++) zr[k] = (own && k < q) ? y[int64_t(k) * a.n + f + i] : S(0);
+  if (own) {
     for (int r = 0; r < mu; r++) {
-      const S l = S(L[nc + r + i * m]);
+      const S l = S(Li[nc + r]);
 #pragma unroll
       for (int k = 0; k < Q; k++)
-        if (k < q) sum[k] -= l * xb[k][r];
+        if (k < q) zr[k] -= l * xb[k][r];
     }
+  }
+  T lcur = (own && nc > 0) ? Li[nc - 1] : T(0);  // L[col][i] for col = nc - 1
+  for (int col = nc - 1; col >= 0; col--) {
+    const T lnext = (own && col > 0) ? Li[col - 1] : T(0);
+    const int b = col & 1;
+    if (i == col) {
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        if (k >= q) continue;
+        const S x = a.ldlt ? zr[k] : zr[k] / S(lcur);
+        zr[k] = x;
+        xc[b][k] = x;
+      }
+    }
+    __syncthreads();
+    if (own && i < col) {
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) zr[k] -= S(lcur) * xc[b][k];
+    }
+    lcur = lnext;
+  }
+  if (own) {
 #pragma unroll
     for (int k = 0; k < Q; k++)
-      if (k < q) z[k][i] = sum[k];
+      if (k < q) y[int64_t(k) * a.n + f + i] = zr[k];
   }
-  __syncthreads();
-  for (int col = nc - 1; col >= 0; col--) {
-    if (threadIdx.x < q) {
-      const S x = a.ldlt ? z[threadIdx.x][col] : z[threadIdx.x][col] / S(L[col + col * m]);
-      z[threadIdx.x][col] = x;
-      xc[threadIdx.x] = x;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < col; i += blockDim.x) {
-      const S l = S(L[col + i * m]);
-#pragma unroll
-      for (int k = 0; k < Q; k++)
-        if (k < q) z[k][i] -= l * xc[k];
-    }
-    __syncthreads();
-  }
-  for (int32_t k = 0; k < q; k++)
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) y[int64_t(k) * a.n + f + i] = z[k][i];
 }
 
 // Backward counterpart of k_solve_wave_fwd: lane i keeps solution row i of the supernode; the below rows' values are
@@ -788,10 +816,10 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
             hipLaunchKernelGGL((k_solve_wave_fwd<T, S, kFlowRhs>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
                                int32_t(L.count), a, y, fv, q, dp.fvEntries);
         } else if (q == 1)
-          hipLaunchKernelGGL((k_solve_small_fwd<T, S, 1>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+          hipLaunchKernelGGL((k_solve_small_fwd<T, S, 1>), dim3(grid), dim3(kSmallSolveThreads), 0, st,
                              dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         else
-          hipLaunchKernelGGL((k_solve_small_fwd<T, S, kFlowRhs>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+          hipLaunchKernelGGL((k_solve_small_fwd<T, S, kFlowRhs>), dim3(grid), dim3(kSmallSolveThreads), 0, st,
                              dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
       case kSolveGather:
@@ -814,10 +842,10 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
             hipLaunchKernelGGL((k_solve_wave_bwd<T, S, kFlowRhs>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
                                int32_t(L.count), a, y, q);
         } else if (q == 1)
-          hipLaunchKernelGGL((k_solve_small_bwd<T, S, 1>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+          hipLaunchKernelGGL((k_solve_small_bwd<T, S, 1>), dim3(grid), dim3(kSmallSolveThreads), 0, st,
                              dp.fronts + L.first, a, y, q);
         else
-          hipLaunchKernelGGL((k_solve_small_bwd<T, S, kFlowRhs>), dim3(grid), dim3(L.maxM <= 64 ? 64 : 128), 0, st,
+          hipLaunchKernelGGL((k_solve_small_bwd<T, S, kFlowRhs>), dim3(grid), dim3(kSmallSolveThreads), 0, st,
                              dp.fronts + L.first, a, y, q);
         break;
       case kSolveBwdGather:
