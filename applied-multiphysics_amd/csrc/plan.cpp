@@ -409,12 +409,8 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
       if (F.isBig[s]) big.push_back(s);
     pushSmall(kSolveSmallBwd, L);
     if (big.empty()) continue;
-    int64_t first = int64_t(P.fronts.size());
-    P.fronts.insert(P.fronts.end(), big.begin(), big.end());
-    int32_t maxMu = 0;
-    for (int32_t s : big) maxMu = std::max(maxMu, frontRows(S, s) - frontCols(S, s));
-    push(kSolveBwdGather, L, first, int64_t(big.size()), maxMu);
-    // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk)
+    // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk); y[below] is read through
+    // the row structure, so no gather launch precedes them
     std::vector<int32_t> p0(ns, 0), np(ns, 0);
     int64_t bf = int64_t(P.blocks.size());
     for (int32_t s : big) {

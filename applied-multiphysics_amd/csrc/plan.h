@@ -148,7 +148,8 @@ enum SolveKind : int32_t {
   kSolveFwdFlow = 2,
   kSolveSmallBwd = 3,
   kSolveBwdFlow = 4,
-  kSolveBwdGather = 5,
+  kSolveBwdGather = 5,  // not emitted (the backward partial products read y[below] through the row structure); the
+                        // number stays reserved so that the kinds' numbers in profiles are unchanged
   kSolveBwdInit = 6,
 };
 
@@ -156,8 +157,8 @@ struct SolveLaunch {
   SolveKind kind;
   int32_t level;
   int64_t first, count;  // small / gather: range in `fronts`; flow kinds: range in `blocks`
-  int32_t maxM;          // small: largest front height; gathers: largest m (forward) or mu (backward), which sets
-                         // the number of row slices per front
+  int32_t maxM;          // small: largest front height; gather: largest m, which sets the number of row slices per
+                         // front
 };
 
 struct SolvePlan {

@@ -583,36 +583,19 @@ __global__ __launch_bounds__(256) void k_solve_wave_bwd(const int32_t* __restric
     if (k < q && lane < nc) y[int64_t(k) * a.n + f + lane] = z[k];
 }
 
-// y[below] of each tiled supernode of a level into the below part of its front vector (contiguous for the backward
-// flow).
-template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_solve_bwd_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a,
-                                                          const S* y, S* fv, int32_t q, int64_t fvN) {
-  const int32_t s = fronts[blockIdx.x];
-  const int32_t nc = a.superStart[s + 1] - a.superStart[s], mu = a.Sp[s + 1] - a.Sp[s] - nc;
-  const int32_t* rows = a.Si + a.Sp[s] + nc;
-  const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(mu, i0 + kGatherRows);
-  for (int32_t jj = 0; jj < q; jj++) {
-    S* dst = fv + jj * fvN + a.Sp[s] + nc;
-    const S* yk = y + int64_t(jj) * a.n;
-#pragma unroll 2
-    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) dst[i] = yk[rows[i]];
-  }
-}
-
 // Partial products of the backward initialisation: for the task's column block and below rows [r0, r1) (at most
 // kInitRows), pz[slot][jj][c] = sum_r L[nc + r, c] y[below r] (y[below] gathered into the front vector). Wave w handles
 // columns 16 w .. 16 w + 15, four at a time; lane i takes rows r0 + i, r0 + 64 + i, ..., so every load instruction
 // reads 64 consecutive rows of one column, and the column sums are reduced across the wave at the end.
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a,
-                                                        const S* fv, S* pz, int32_t q, int64_t fvN) {
+                                                        const S* y, S* pz, int32_t q) {
   constexpr int E = kTileEdge, CG = 4, NR = kInitRows / E;
   const BlockTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t s = t.s, nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s];
   const T* Lb = a.Lx + a.Lxp[s] + nc;
-  const S* yb = fv + a.Sp[s] + nc;
+  const int32_t* rows = a.Si + a.Sp[s] + nc;  // y[below] read through the row structure
 #pragma unroll 1
   for (int g = 0; g < 16 / CG; g++) {
     const int cj0 = 16 * w + CG * g;  // first column (in the block) of this group
@@ -625,9 +608,10 @@ __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restr
     for (int u = 0; u < NR; u++) {
       const int32_t i = t.r0 + E * u + lane;
       const bool row = i < t.r1;
+      const int32_t yi = row ? rows[i] : 0;
       S yv[Q];
 #pragma unroll
-      for (int jj = 0; jj < Q; jj++) yv[jj] = (row && jj < q) ? yb[jj * fvN + i] : S(0);
+      for (int jj = 0; jj < Q; jj++) yv[jj] = (row && jj < q) ? y[int64_t(jj) * a.n + yi] : S(0);
 #pragma unroll
       for (int c = 0; c < CG; c++) {
         const int32_t col = t.blk * E + cj0 + c;
@@ -848,18 +832,15 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
           hipLaunchKernelGGL((k_solve_small_bwd<T, S, kFlowRhs>), dim3(grid), dim3(kSmallSolveThreads), 0, st,
                              dp.fronts + L.first, a, y, q);
         break;
-      case kSolveBwdGather:
-        hipLaunchKernelGGL((k_solve_bwd_gather<T, S>),
-                           dim3(grid, unsigned(std::max(1, (L.maxM + kGatherRows - 1) / kGatherRows))), dim3(256), 0,
-                           st, dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
-        break;
+      case kSolveBwdGather:  // never emitted by the plan
+        return hipErrorInvalidValue;
       case kSolveBwdInit:
         if (q == 1)
-          hipLaunchKernelGGL((k_solve_bwd_init<T, S, 1>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a, fv,
-                             pz, q, dp.fvEntries);
+          hipLaunchKernelGGL((k_solve_bwd_init<T, S, 1>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a, y,
+                             pz, q);
         else
           hipLaunchKernelGGL((k_solve_bwd_init<T, S, kFlowRhs>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a,
-                             fv, pz, q, dp.fvEntries);
+                             y, pz, q);
         break;
       case kSolveBwdFlow:
         err = q == 1 ? launchFlow(k_solve_bwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++, a,

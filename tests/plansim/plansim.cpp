@@ -493,7 +493,8 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
     for (int32_t i = 0; i < n; i++) y[size_t(k) * n + i] = b[size_t(k) * n + S.perm[i]];
   bool scaled = false;
   for (const SolveLaunch& L : P.launches) {
-    if (c.ldlt && !scaled && (L.kind == kSolveSmallBwd || L.kind == kSolveBwdGather || L.kind == kSolveBwdFlow)) {
+    if (c.ldlt && !scaled &&
+        (L.kind == kSolveSmallBwd || L.kind == kSolveBwdGather || L.kind == kSolveBwdInit || L.kind == kSolveBwdFlow)) {
       for (int32_t k = 0; k < q; k++)
         for (int32_t j = 0; j < n; j++) y[size_t(k) * n + j] /= c.D[j];
       scaled = true;
@@ -510,7 +511,7 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
             const int32_t col = bt.blk * kTileEdge + j;
             double sum = 0;
             if (col < nc)
-              for (int32_t r = bt.r0; r < bt.r1; r++) sum += Ls[nc + r + int64_t(col) * m] * fk[S.Sp[s] + nc + r];
+              for (int32_t r = bt.r0; r < bt.r1; r++) sum += Ls[nc + r + int64_t(col) * m] * yk[S.Si[S.Sp[s] + nc + r]];
             pz[(size_t(bt.p0) * q + k) * kTileEdge + j] = sum;
           }
         } else if (L.kind == kSolveBwdGather) {
