@@ -86,7 +86,9 @@ typedef struct {
 int amp_amd_order(int32_t n, const int32_t *rowptr, const int32_t *colidx, double denseFactor, int32_t *perm);
 
 /* analyze (sparse.mjs:666-902): ordering (AMD when perm is NULL), etree, postorder, column counts, relaxed supernodes
- * (relax != 0), row structures and block offsets, plus the device execution plan. */
+ * (relax != 0), row structures and block offsets, plus the device execution plan. AMP_ERR_ARG when the plan's index
+ * ranges cannot hold the system (a tiled front with rows x super-panel width >= 2^31, or more than 2^31 extend-add
+ * ranges). */
 int amp_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const int32_t *perm, int32_t relax,
                 amp_sym **out);
 
