@@ -260,9 +260,10 @@ constexpr int kFlowRun = 8;                    // blocks consumed per synchronis
 // take a run of finished blocks in one pass.
 __device__ inline int32_t flowRun(const int32_t* flags, int32_t k, int32_t lim, int32_t want) {
   __shared__ int32_t s_r;
+  const int32_t step = lim > k ? 1 : -1, last = lim - step;
+  if (last == k) return lim;  // a run of one block: nothing to look up (uniform, no barrier)
   if (threadIdx.x == 0) {
-    const int32_t step = lim > k ? 1 : -1, last = lim - step;
-    const int32_t fl = last == k ? want : __hip_atomic_load(flags + last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t fl = __hip_atomic_load(flags + last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_r = fl == want ? lim : k + step;
   }
   __syncthreads();
@@ -347,9 +348,12 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
     for (int j = 0; j < 16; j++) wv[j] = Wt[rr + (16 * g + j) * E];
   }
-  S acc[Q];
+  S acc[Q], vown[Q];  // vown: this row's front-vector value, read before any wait (the gather wrote it earlier)
 #pragma unroll
-  for (int jj = 0; jj < Q; jj++) acc[jj] = S(0);
+  for (int jj = 0; jj < Q; jj++) {
+    acc[jj] = S(0);
+    vown[jj] = (tid < E && live && jj < q) ? fv[jj * fvN + a.Sp[s] + r] : S(0);
+  }
   const int32_t steps = t.blk;
   T cur[16], nxt[16];
   auto loadL = [&](T* dst, int32_t k) {
@@ -412,20 +416,21 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
       for (int j = 0; j < 16; j++) cur[j] = nxt[j];
     }
-    __syncthreads();
     k0 = k1;
+    if (k0 < steps) __syncthreads();  // the next run is staged into xs (after the last run, part is written first)
   }
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) part[g][jj][rr] = acc[jj];
   __syncthreads();
   S* vs = &xs[0][0];  // reused: v of the own block, Q x 64
   if (tid < E) {
-    for (int jj = 0; jj < q; jj++) {
+#pragma unroll
+    for (int jj = 0; jj < Q; jj++) {
+      if (jj >= q) continue;
       S v = S(0);
       if (live) {
-        S* fvk = fv + jj * fvN + a.Sp[s];
-        v = fvk[r] - ((part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]));
-        if (!t.writer) fvk[r] = v;
+        v = vown[jj] - ((part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]));
+        if (!t.writer) fv[jj * fvN + a.Sp[s] + r] = v;
       }
       vs[jj * E + rr] = v;
     }
@@ -654,14 +659,15 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
     for (int i = 0; i < 16; i++) wv[i] = Wt[(rs + 4 * i) + cj * E];
   }
-  // the partial products of L21^T y[below] (slot order), summed before any wait
-  S acc[Q];
+  // the partial products of L21^T y[below] (slot order), summed before any wait, and this column's y value
+  S acc[Q], yown[Q];
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) {
     S below = S(0);
     if (jj < q && rs == 0)
       for (int32_t p = 0; p < t.np; p++) below += pz[(int64_t(t.p0 + p) * q + jj) * E + cj];
     acc[jj] = below;
+    yown[jj] = (jj < q && rs == 0 && live) ? y[int64_t(jj) * a.n + f + col] : S(0);
   }
   T cur[16], nxt[16];
   auto loadL = [&](T* dst, int32_t k) {
@@ -698,16 +704,19 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
       for (int i = 0; i < 16; i++) cur[i] = nxt[i];
     }
-    __syncthreads();
     k0 = k1;
+    if (k0 > kEnd) __syncthreads();  // the next run is staged into xs (the tail uses zs only)
   }
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) {
     acc[jj] += __shfl_xor(acc[jj], 16, 64);
     acc[jj] += __shfl_xor(acc[jj], 32, 64);
   }
-  if (rs == 0)
-    for (int jj = 0; jj < q; jj++) zs[jj][cj] = live ? y[int64_t(jj) * a.n + f + col] - acc[jj] : S(0);
+  if (rs == 0) {
+#pragma unroll
+    for (int jj = 0; jj < Q; jj++)
+      if (jj < q) zs[jj][cj] = live ? yown[jj] - acc[jj] : S(0);
+  }
   __syncthreads();
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) {
