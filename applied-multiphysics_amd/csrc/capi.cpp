@@ -30,11 +30,14 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
-// Factor and solve plans of the analysis; AMP_ERR_ARG when the system exceeds the plans' index ranges.
+// Factor and solve plans of the analysis; AMP_ERR_ARG when the system exceeds the plans' index ranges (the handle's
+// plans are then unchanged).
 int buildPlans(amp_sym* h, const amp::PlanParams& p) {
   try {
-    h->F = amp::buildFactorPlan(h->S, p);
-    h->SP = amp::buildSolvePlan(h->S, h->F);
+    amp::FactorPlan F = amp::buildFactorPlan(h->S, p);
+    amp::SolvePlan SP = amp::buildSolvePlan(h->S, F);
+    h->F = std::move(F);
+    h->SP = std::move(SP);
   } catch (const std::length_error& e) {
     return fail(AMP_ERR_ARG, std::string("analyze: ") + e.what());
   }
@@ -700,9 +703,9 @@ void amp_graph_free(amp_graph* g) { delete g; }
 namespace {
 
 int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
+  if (int rc = buildPlans(h, p)) return rc;
   h->params = p;
   h->onDevice = false;
-  if (int rc = buildPlans(h, p)) return rc;
   return symToDevice(h);
 }
 

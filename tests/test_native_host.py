@@ -75,6 +75,15 @@ def test_argument_errors_without_device():
         amp.analyze(pat, {"perm": np.arange(3)})
 
 
+def test_plan_rejects_fronts_beyond_tile_offsets():
+    """The product kernels address a tile's operands with 32-bit offsets, so a super-panel so wide that a tiled
+    front's rows x width reaches 2^31 is refused with an argument error (before any device work)."""
+    pat = W.heat3d(12)[0]
+    sym = amp.analyze(pat)
+    with pytest.raises(Exception, match="too tall for 32-bit tile offsets"):
+        sym.set_params({"smallMax": 96, "smallNarrow": 32, "wideLaunch": 512, "superPanel": 1 << 24})
+
+
 def test_sym_import_roundtrip():
     """The analysis handed over by a host symbolic phase gives the same structure as the native one."""
     pat = grid27(8, 7, 6)[0]
