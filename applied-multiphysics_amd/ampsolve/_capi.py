@@ -152,13 +152,13 @@ def check(rc):
 
 
 def ptr(a):
-    """Address of a numpy array or a torch tensor (device pointers pass through unchanged). The library's device
-    entry points order their work after the null stream; for a device tensor whose producer may still be queued on
-    another current torch stream, that stream is synchronised here."""
+    """Address of a numpy array, a device array (ampsolve.device.DeviceArray) or a framework device tensor. The
+    library's device entry points order their work after the null stream; for a framework tensor whose producer may
+    still be queued on another current stream of that framework, that stream is synchronised here."""
     if a is None:
         return None
     if hasattr(a, "data_ptr"):
-        if getattr(a, "is_cuda", False):
+        if getattr(a, "is_cuda", False) and type(a).__module__.startswith("torch"):
             import torch
 
             s = torch.cuda.current_stream(a.device)

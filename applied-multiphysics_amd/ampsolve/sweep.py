@@ -20,6 +20,7 @@ import scipy.sparse as sp
 
 from . import _capi as A
 from . import workloads as W
+from .device import DeviceArray, synchronize
 from .sparse import NumericFactor, analyze
 
 MODES = np.array([(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1), (2, 0, 0)])
@@ -111,19 +112,18 @@ class WarmStartNet:
         A.check(A.lib().amp_mlp_adam(self._h, float(lr), float(b1), float(b2), float(eps)))
 
     def train(self, X, T, epochs, batch, lr, seed=3, decay=0.3):
-        """Minibatch Adam over device tensors X (N x nIn), T (N x nOut); the learning rate steps down by `decay` at
-        each third of the epochs. Returns the mean loss of each epoch."""
-        import torch
-
+        """Minibatch Adam over device arrays X (N x nIn), T (N x nOut) whose rows the caller has put in random order:
+        each epoch visits the contiguous minibatches in an order of its own; the learning rate steps down by `decay`
+        at each third of the epochs. Returns the mean loss of each epoch."""
         N = X.shape[0]
+        starts = np.arange(0, N, batch)
         hist = []
         for ep in range(epochs):
             rate = lr * decay ** (3 * ep // epochs)
-            order = torch.from_numpy(np.argsort(W.lcg_uniform(seed + ep, N))).to(X.device)
             tot = 0.0
-            for s in range(0, N, batch):
-                idx = order[s:s + batch]
-                tot += self.gradient(X[idx].contiguous(), T[idx].contiguous()) * len(idx)
+            for s in starts[np.argsort(W.lcg_uniform(seed + ep, len(starts)))]:
+                e = min(s + batch, N)
+                tot += self.gradient(X[s:e], T[s:e]) * (e - s)
                 self.adam(rate)
             hist.append(tot / N)
         return hist
@@ -150,18 +150,15 @@ def pcg(num, values, b, x, tol=1e-10, maxIter=50):
 class LearnedSweep:
     """The sweep of the module docstring on one device."""
 
-    def __init__(self, ncell=30, hidden=256, device="cuda:0", maxBatch=512):
-        import torch
-
-        self.torch, self.dev = torch, torch.device(device)
+    def __init__(self, ncell=30, hidden=256, maxBatch=512):
         self.prob = ParamHeat3D(ncell)
         self.sym = analyze(self.prob.pat)
-        self.db = torch.from_numpy(self.prob.b).to(self.dev)
+        self.db = DeviceArray.from_numpy(self.prob.b)
         self.net = WarmStartNet(MODES.shape[0], hidden, self.prob.nnodes, maxBatch)
         self.direct = NumericFactor(self.sym, 0, "hip")
 
     def device_values(self, theta):
-        return self.torch.from_numpy(self.prob.values(theta)).to(self.dev)
+        return DeviceArray.from_numpy(self.prob.values(theta))
 
     def solve_direct(self, theta):
         """Fresh FP64 factor + solve (the refactoring method): (x, device ms of factor + solve)."""
@@ -171,9 +168,8 @@ class LearnedSweep:
         x = self.direct.solve(self.db)
         return x, fms + self.direct.kernel_ms("solve_total")[0]
 
-    def fit(self, thetas, epochs=150, batch=64, lr=3e-3):
+    def fit(self, thetas, epochs=150, batch=64, lr=3e-3, seed=5):
         """Training solves (FP32 factor + FP64 refinement, factors cached), network training. Returns statistics."""
-        torch = self.torch
         t0 = time.perf_counter()
         self.cache_theta = np.asarray(thetas, np.float64)
         self.cache, xs = [], []
@@ -185,32 +181,30 @@ class LearnedSweep:
             assert info["converged"], info
             num.release_workspace()
             self.cache.append(num)
-            xs.append(x)
-        torch.cuda.synchronize()
+            xs.append(x.numpy())
+        synchronize()
         t_data = time.perf_counter() - t0
-        Xs = torch.stack(xs)  # (N, n) FP64
-        full = torch.zeros((len(thetas), self.prob.nnodes), dtype=torch.float64, device=self.dev)
-        full[:, torch.from_numpy(self.prob.free).to(self.dev)] = Xs
+        full = self.prob.to_nodes(np.stack(xs))  # (N, nnodes) FP64
         # targets scaled per node (mean and spread over the training points; Dirichlet nodes stay at zero)
         self.mu = full.mean(0)
         sd = (full - self.mu).std(0)
-        self.sigma = torch.clamp(sd, min=1e-3 * float(sd.max()) if float(sd.max()) > 0 else 1.0)
-        T = ((full - self.mu) / self.sigma).float().contiguous()
-        Xin = torch.from_numpy(np.asarray(thetas, np.float32)).to(self.dev)
+        self.sigma = np.maximum(sd, 1e-3 * sd.max()) if sd.max() > 0 else np.ones_like(sd)
+        order = np.argsort(W.lcg_uniform(seed, len(thetas)))  # rows in random order once; minibatches are slices
+        T = DeviceArray.from_numpy(((full[order] - self.mu) / self.sigma).astype(np.float32))
+        Xin = DeviceArray.from_numpy(np.asarray(thetas, np.float32)[order])
         t0 = time.perf_counter()
         hist = self.net.train(Xin, T, epochs, batch, lr)
-        torch.cuda.synchronize()
+        synchronize()
         t_train = time.perf_counter() - t0
         return {"data_s": t_data, "train_s": t_train, "loss": hist}
 
     def predict(self, thetas):
-        """Network warm starts for a batch of points: (B, n) FP64 device tensor on the free DOFs."""
-        torch = self.torch
-        Xin = torch.from_numpy(np.asarray(thetas, np.float32)).to(self.dev)
-        Y = torch.empty((len(thetas), self.prob.nnodes), dtype=torch.float32, device=self.dev)
+        """Network warm starts for a batch of points: (B, n) FP64 device array on the free DOFs."""
+        Xin = DeviceArray.from_numpy(np.asarray(thetas, np.float32))
+        Y = DeviceArray((len(thetas), self.prob.nnodes), np.float32)
         self.net.forward(Xin, Y)
-        full = Y.double() * self.sigma + self.mu
-        return full[:, torch.from_numpy(self.prob.free).to(self.dev)].contiguous()
+        full = Y.numpy().astype(np.float64) * self.sigma + self.mu
+        return DeviceArray.from_numpy(full[:, self.prob.free])
 
     def nearest(self, theta):
         return int(np.argmin(np.linalg.norm(self.cache_theta - np.asarray(theta), axis=1)))

@@ -214,6 +214,17 @@ int ensureVecs(amp_num* num, int32_t nrhs) {
   const amp_sym* h = num->sym;
   const size_t n = size_t(std::max(h->S.n, 1)) * nrhs;
   const size_t es = num->prec == AMP_F64 ? 8 : 4;
+  if (num->liveGraphs > 0) {  // captured graphs hold the current addresses: no buffer may move
+    const size_t fvn = size_t(std::max<int64_t>(h->SP.fvEntries, 1)) * nrhs * es;
+    const size_t pzn =
+        size_t(std::max<int64_t>(h->SP.initSlots, 1)) * amp::kTileEdge * std::min(nrhs, amp::kFlowRhs) * es;
+    if (num->b.bytes < n * 8 || num->y.bytes < std::max(n * es, size_t(h->S.n) * 8) || num->fv.bytes < fvn ||
+        num->pz.bytes < pzn)
+      return fail(AMP_ERR_STATE, "solve: " + std::to_string(nrhs) +
+                                     " right-hand sides need larger solve buffers, which captured graphs of this "
+                                     "factor still use (free them first)");
+    return AMP_OK;
+  }
   if (num->b.bytes < n * 8) {
     HIP_TRY(num->b.alloc(n * 8));
     HIP_TRY(num->x.alloc(n * 8));
@@ -232,6 +243,7 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   if (rc) return rc;
   amp::ProfileSink* prof = num->profiling ? &num->solveProf : nullptr;
   if (prof) prof->used = 0;
+  HIP_TRY(hipMemsetAsync(num->fc.err, 0, 4, num->stream));  // the wait-failure flag reports this solve only
   HIP_TRY(hipEventRecord(num->ev0, num->stream));
   if (num->prec == AMP_F64)
     HIP_TRY((amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs,
@@ -327,15 +339,56 @@ int amp_sym_import(const amp_sym_desc* d, amp_sym** out) {
     if (S.perm[i] < 0 || S.perm[i] >= n) return fail(AMP_ERR_ARG, "amp_sym_import: perm out of range");
     S.pinv[S.perm[i]] = i;
   }
+  // every index the plans and kernels will follow is checked here, so that a corrupted descriptor is an argument
+  // error rather than an out-of-range access later
+  auto bad = [](const char* what) { return fail(AMP_ERR_ARG, std::string("amp_sym_import: ") + what); };
+  if ((n > 0) != (ns > 0) || ns > n) return bad("nsuper does not fit n");
+  if (d->nnzPattern < 0) return bad("nnzPattern");
+  {
+    std::vector<char> seen(n, 0);
+    for (int32_t i = 0; i < n; i++) {
+      if (seen[S.perm[i]]) return bad("perm is not a permutation");
+      seen[S.perm[i]] = 1;
+    }
+  }
   S.parent.assign(d->parent, d->parent + n);
+  for (int32_t j = 0; j < n; j++)
+    if (S.parent[j] != -1 && (S.parent[j] <= j || S.parent[j] >= n)) return bad("parent is not a postordered tree");
   S.superStart.assign(d->superStart, d->superStart + ns + 1);
+  if (ns > 0 && (S.superStart[ns] != n || S.superStart[0] != 0)) return bad("superStart");
+  for (int32_t s = 0; s < ns; s++)
+    if (S.superStart[s + 1] <= S.superStart[s]) return bad("superStart does not increase");
   S.Sp.assign(d->Sp, d->Sp + ns + 1);
+  if (ns > 0 && S.Sp[0] != 0) return bad("Sp[0]");
+  for (int32_t s = 0; s < ns; s++)
+    if (int64_t(S.Sp[s + 1]) - S.Sp[s] < int64_t(S.superStart[s + 1]) - S.superStart[s])
+      return bad("Sp shorter than the supernode");
   S.Si.assign(d->Si, d->Si + (ns ? d->Sp[ns] : 0));
+  for (int32_t s = 0; s < ns; s++) {  // own columns first, then strictly increasing rows below them
+    const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f;
+    for (int32_t r = 0; r < S.Sp[s + 1] - S.Sp[s]; r++) {
+      const int32_t i = S.Si[S.Sp[s] + r];
+      if (r < nc ? i != f + r : (i >= n || i <= S.Si[S.Sp[s] + r - 1])) return bad("Si row structure");
+    }
+  }
   S.lowerP.assign(d->lowerP, d->lowerP + n + 1);
+  if (n > 0 && S.lowerP[0] != 0) return bad("lowerP[0]");
+  for (int32_t j = 0; j < n; j++)
+    if (S.lowerP[j + 1] < S.lowerP[j]) return bad("lowerP does not increase");
   S.nnzA = n ? d->lowerP[n] : 0;
   S.lowerI.assign(d->lowerI, d->lowerI + S.nnzA);
   S.lowerSrc.assign(d->lowerSrc, d->lowerSrc + S.nnzA);
-  if (ns > 0 && (S.superStart[ns] != n || S.superStart[0] != 0)) return fail(AMP_ERR_ARG, "amp_sym_import: superStart");
+  for (int64_t q = 0; q < S.nnzA; q++)
+    if (S.lowerSrc[q] < 0 || S.lowerSrc[q] >= d->nnzPattern) return bad("lowerSrc out of the pattern's range");
+  {  // every lower entry of a column lies in its supernode's row structure
+    std::vector<int32_t> mark(n, -1);
+    for (int32_t s = 0; s < ns; s++) {
+      for (int32_t r = S.Sp[s]; r < S.Sp[s + 1]; r++) mark[S.Si[r]] = s;
+      for (int32_t j = S.superStart[s]; j < S.superStart[s + 1]; j++)
+        for (int32_t q = S.lowerP[j]; q < S.lowerP[j + 1]; q++)
+          if (S.lowerI[q] < j || S.lowerI[q] >= n || mark[S.lowerI[q]] != s) return bad("lowerI outside the row structure");
+    }
+  }
   S.colSuper.resize(n);
   for (int32_t s = 0; s < ns; s++)
     for (int32_t j = S.superStart[s]; j < S.superStart[s + 1]; j++) S.colSuper[j] = s;
@@ -353,6 +406,15 @@ int amp_sym_import(const amp_sym_desc* d, amp_sym** out) {
     S.childNext[s] = S.childHead[P];
     S.childHead[P] = s;
   }
+  {  // a child's rows below its columns all appear in its parent's structure (the relative maps need them)
+    std::vector<int32_t> mark(n, -1);
+    for (int32_t P = 0; P < ns; P++) {
+      for (int32_t r = S.Sp[P]; r < S.Sp[P + 1]; r++) mark[S.Si[r]] = P;
+      for (int32_t c = S.childHead[P]; c != -1; c = S.childNext[c])
+        for (int32_t r = S.Sp[c] + (S.superStart[c + 1] - S.superStart[c]); r < S.Sp[c + 1]; r++)
+          if (mark[S.Si[r]] != P) return bad("a child's rows are missing from its parent's structure");
+    }
+  }
   S.Lxp.assign(ns + 1, 0);
   S.maxFront = 0;
   for (int32_t s = 0; s < ns; s++) {
@@ -369,6 +431,7 @@ int amp_sym_import(const amp_sym_desc* d, amp_sym** out) {
     const int32_t m = S.Sp[s + 1] - S.Sp[s];
     for (int32_t j = S.superStart[s]; j < S.superStart[s + 1]; j++) {
       const int32_t cc = d->colCount ? d->colCount[j] : m - (j - S.superStart[s]);
+      if (cc < 1 || cc > n - j) return bad("colCount out of range");
       S.colCount[j] = cc;
       S.nnzL += cc;
       S.flops += double(cc) * cc;
@@ -568,6 +631,8 @@ int amp_num_get_kernel_ms(amp_num* num, const char* kernel, double* total_ms, in
 
 int amp_num_release_workspace(amp_num* num) {
   if (!num) return fail(AMP_ERR_ARG, "amp_num_release_workspace: bad arguments");
+  if (num->liveGraphs > 0)
+    return fail(AMP_ERR_STATE, "amp_num_release_workspace: captured graphs of this factor still use the workspace");
   HIP_TRY(hipStreamSynchronize(num->stream));
   num->U.reset();
   return AMP_OK;
@@ -583,7 +648,9 @@ struct amp_graph {
   hipGraphExec_t exec = nullptr;
   DevBuf r, d;  // residual and correction, n x nrhs FP64
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool counted = false;  // counted in num->liveGraphs
   ~amp_graph() {
+    if (counted) num->liveGraphs--;
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (e0) (void)hipEventDestroy(e0);
@@ -662,6 +729,8 @@ int amp_graph_create(amp_num* num, const double* d_values, const double* d_b, do
   if (rc) return rc;
   if (ce != hipSuccess) return fail(AMP_ERR_HIP, std::string("amp_graph_create: capture: ") + hipGetErrorString(ce));
   HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  num->liveGraphs++;
+  g->counted = true;
   *out = g.release();
   return AMP_OK;
 }
@@ -669,6 +738,7 @@ int amp_graph_create(amp_num* num, const double* d_values, const double* d_b, do
 int amp_graph_launch(amp_graph* g) {
   if (!g) return fail(AMP_ERR_ARG, "amp_graph_launch: bad arguments");
   HIP_TRY(amp::afterCaller(g->num->stream, g->num->inEv));
+  HIP_TRY(hipMemsetAsync(g->num->fc.err, 0, 4, g->num->stream));  // the wait-failure flag reports this replay only
   HIP_TRY(hipEventRecord(g->e0, g->num->stream));
   HIP_TRY(hipGraphLaunch(g->exec, g->num->stream));
   HIP_TRY(hipEventRecord(g->e1, g->num->stream));

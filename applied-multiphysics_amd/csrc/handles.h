@@ -101,6 +101,7 @@ struct amp_num {
   amp::FlowCtl fc{};
   bool ok = false;
   bool profiling = false;
+  int32_t liveGraphs = 0;  // captured graphs holding this handle's buffer addresses (amp_graph_create / _free)
   amp::ProfileSink prof, solveProf;
   float solveMs = 0;
   std::map<std::string, std::pair<double, int32_t>> kernelMs;

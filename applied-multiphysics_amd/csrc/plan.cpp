@@ -68,7 +68,8 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
         for (int32_t r = S.Sp[c] + frontCols(S, c); r < S.Sp[c + 1]; r++) P.rel[r] = pos[S.Si[r]];
     }
   }
-  // assembly maps of A's lower entries
+  // assembly maps of A's lower entries: in-block offsets (int32) for whole fronts, whose blocks hold at most
+  // kSmallMax^2 entries; absolute Lx indices (int64) for tiled fronts, whose m x nc blocks can pass 2^31 entries
   P.amap.assign(S.nnzA, 0);
   P.adst.assign(S.nnzA, -1);
   {
@@ -78,9 +79,11 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
       for (int32_t i = S.Sp[s]; i < S.Sp[s + 1]; i++) pos[S.Si[i]] = i - S.Sp[s];
       for (int32_t c = 0; c < nc; c++)
         for (int32_t q = S.lowerP[f + c]; q < S.lowerP[f + c + 1]; q++) {
-          const int32_t off = pos[S.lowerI[q]] + c * m;
-          P.amap[q] = off;
-          if (P.isBig[s]) P.adst[q] = S.Lxp[s] + off;
+          const int64_t off = int64_t(pos[S.lowerI[q]]) + int64_t(c) * m;
+          if (P.isBig[s])
+            P.adst[q] = S.Lxp[s] + off;
+          else
+            P.amap[q] = int32_t(off);
         }
     }
   }

@@ -32,8 +32,9 @@ static napi_value throw_amp(napi_env env, int rc) {
     if (rc_ != AMP_OK) return throw_amp(env, rc_); \
   } while (0)
 
-/* typed array -> data pointer and element count (NULL / 0 for null or undefined) */
-static int typed(napi_env env, napi_value v, void **data, size_t *len) {
+/* typed array of element type `want` -> data pointer and element count (NULL / 0 for null or undefined); 0 when the
+ * value is neither, or a typed array of another element type */
+static int typed(napi_env env, napi_value v, napi_typedarray_type want, void **data, size_t *len) {
   napi_valuetype t;
   *data = NULL;
   *len = 0;
@@ -44,7 +45,15 @@ static int typed(napi_env env, napi_value v, void **data, size_t *len) {
   napi_typedarray_type tt;
   napi_value ab;
   size_t off;
-  return napi_get_typedarray_info(env, v, &tt, len, data, &ab, &off) == napi_ok;
+  if (napi_get_typedarray_info(env, v, &tt, len, data, &ab, &off) != napi_ok) return 0;
+  return tt == want;
+}
+#define I32 napi_int32_array
+#define F64 napi_float64_array
+
+static napi_value throw_len(napi_env env, const char *what) {
+  napi_throw_range_error(env, NULL, what);
+  return NULL;
 }
 
 static void sym_finalize(napi_env env, void *data, void *hint) {
@@ -53,10 +62,18 @@ static void sym_finalize(napi_env env, void *data, void *hint) {
   amp_sym_free((amp_sym *)data);
 }
 
+/* a numeric handle with the sizes its arrays are checked against */
+typedef struct {
+  amp_num *num;
+  int64_t n, nnzPattern, nnzLStored;
+} num_ref;
+
 static void num_finalize(napi_env env, void *data, void *hint) {
   (void)env;
   (void)hint;
-  amp_num_free((amp_num *)data);
+  num_ref *r = (num_ref *)data;
+  amp_num_free(r->num);
+  free(r);
 }
 
 static napi_value copy_i32(napi_env env, const int32_t *src, size_t n) {
@@ -137,8 +154,12 @@ static napi_value js_analyze(napi_env env, napi_callback_info cb) {
   CHECK_NAPI(napi_get_value_int32(env, argv[0], &n));
   void *rp, *ci, *perm;
   size_t lrp, lci, lperm;
-  if (!typed(env, argv[1], &rp, &lrp) || !typed(env, argv[2], &ci, &lci) || !typed(env, argv[3], &perm, &lperm))
+  if (!typed(env, argv[1], I32, &rp, &lrp) || !typed(env, argv[2], I32, &ci, &lci) ||
+      !typed(env, argv[3], I32, &perm, &lperm))
     return throw_amp(env, AMP_ERR_ARG);
+  if (n < 0 || !rp || lrp < (size_t)n + 1 || ((const int32_t *)rp)[n] < 0 || lci < (size_t)((const int32_t *)rp)[n])
+    return throw_len(env, "analyze: rowptr / colidx shorter than the pattern");
+  if (perm && lperm != (size_t)n) return throw_len(env, "analyze: perm length differs from n");
   napi_get_value_bool(env, argv[4], &relax);
   amp_sym *h = NULL;
   int32_t dummy = 0;
@@ -161,16 +182,27 @@ static napi_value js_import(napi_env env, napi_callback_info cb) {
   const char *names[] = {"perm", "parent", "colCount", "superStart", "Sp", "Si", "lowerP", "lowerI", "lowerSrc"};
   const int32_t **slots[] = {&d.perm, &d.parent, &d.colCount, &d.superStart, &d.Sp, &d.Si, &d.lowerP, &d.lowerI,
                              &d.lowerSrc};
+  size_t lens[9];
   for (int i = 0; i < 9; i++) {
     void *p;
-    size_t len;
     CHECK_NAPI(napi_get_named_property(env, s, names[i], &v));
-    if (!typed(env, v, &p, &len)) return throw_amp(env, AMP_ERR_ARG);
+    if (!typed(env, v, I32, &p, &lens[i])) return throw_amp(env, AMP_ERR_ARG);
     *slots[i] = (const int32_t *)p;
   }
+  /* lengths the library reads: n for perm / parent / colCount, nsuper + 1 for superStart / Sp, Sp[nsuper] for Si,
+   * n + 1 for lowerP, lowerP[n] for lowerI / lowerSrc */
+  const size_t n = d.n < 0 ? 0 : (size_t)d.n, ns = d.nsuper < 0 ? 0 : (size_t)d.nsuper;
+  if (d.n < 0 || d.nsuper < 0 || lens[0] < n || lens[1] < n || (d.colCount && lens[2] < n) || lens[3] < ns + 1 ||
+      lens[4] < ns + 1 || lens[6] < n + 1)
+    return throw_len(env, "importSymbolic: analysis arrays shorter than n / nsuper");
+  if ((ns && (d.Sp[ns] < 0 || lens[5] < (size_t)d.Sp[ns])) || d.lowerP[n] < 0 || lens[7] < (size_t)d.lowerP[n] ||
+      lens[8] < (size_t)d.lowerP[n])
+    return throw_len(env, "importSymbolic: Si / lowerI / lowerSrc shorter than Sp / lowerP say");
   void *rp, *ci;
   size_t lrp, lci;
-  if (!typed(env, argv[1], &rp, &lrp) || !typed(env, argv[2], &ci, &lci)) return throw_amp(env, AMP_ERR_ARG);
+  if (!typed(env, argv[1], I32, &rp, &lrp) || !typed(env, argv[2], I32, &ci, &lci)) return throw_amp(env, AMP_ERR_ARG);
+  if (rp && (lrp < n + 1 || ((const int32_t *)rp)[n] < 0 || lci < (size_t)((const int32_t *)rp)[n]))
+    return throw_len(env, "importSymbolic: rowptr / colidx shorter than the pattern");
   double nnz;
   CHECK_NAPI(napi_get_value_double(env, argv[3], &nnz));
   d.nnzPattern = (int64_t)nnz;
@@ -203,10 +235,25 @@ static napi_value js_num_create(napi_env env, napi_callback_info cb) {
   CHECK_NAPI(napi_get_value_external(env, argv[0], &h));
   CHECK_NAPI(napi_get_value_int32(env, argv[1], &mode));
   CHECK_NAPI(napi_get_value_int32(env, argv[2], &prec));
-  amp_num *num = NULL;
-  CHECK_AMP(amp_num_create((amp_sym *)h, mode, prec, &num));
+  amp_sym_info info;
+  CHECK_AMP(amp_sym_get_info((amp_sym *)h, &info));
+  num_ref *r = (num_ref *)calloc(1, sizeof *r);
+  if (!r) return throw_len(env, "numCreate: out of host memory");
+  int rc = amp_num_create((amp_sym *)h, mode, prec, &r->num);
+  if (rc != AMP_OK) {
+    free(r);
+    return throw_amp(env, rc);
+  }
+  r->n = info.n;
+  r->nnzPattern = info.nnzPattern;
+  r->nnzLStored = info.nnzLStored;
   napi_value ext;
-  CHECK_NAPI(napi_create_external(env, num, num_finalize, NULL, &ext));
+  if (napi_create_external(env, r, num_finalize, NULL, &ext) != napi_ok) {
+    amp_num_free(r->num);
+    free(r);
+    napi_throw_error(env, NULL, "amp_native: napi_create_external");
+    return NULL;
+  }
   return ext;
 }
 
@@ -219,10 +266,11 @@ static napi_value js_factor(napi_env env, napi_callback_info cb) {
   size_t nv;
   double tol;
   CHECK_NAPI(napi_get_value_external(env, argv[0], &h));
-  if (!typed(env, argv[1], &vals, &nv)) return throw_amp(env, AMP_ERR_ARG);
+  if (!typed(env, argv[1], F64, &vals, &nv)) return throw_amp(env, AMP_ERR_ARG);
   CHECK_NAPI(napi_get_value_double(env, argv[2], &tol));
   amp_factor_info info;
-  CHECK_AMP(amp_factor((amp_num *)h, (const double *)vals, (int64_t)nv, tol, &info));
+  /* the library compares the count with the pattern's (AMP_ERR_PATTERN) and reads exactly nv values */
+  CHECK_AMP(amp_factor(((num_ref *)h)->num, (const double *)vals, (int64_t)nv, tol, &info));
   napi_value o, ok;
   CHECK_NAPI(napi_create_object(env, &o));
   CHECK_NAPI(napi_get_boolean(env, info.ok != 0, &ok));
@@ -243,10 +291,12 @@ static napi_value js_solve(napi_env env, napi_callback_info cb) {
   size_t lb, lx;
   int32_t q;
   CHECK_NAPI(napi_get_value_external(env, argv[0], &h));
-  if (!typed(env, argv[1], &b, &lb) || !typed(env, argv[2], &x, &lx)) return throw_amp(env, AMP_ERR_ARG);
+  if (!typed(env, argv[1], F64, &b, &lb) || !typed(env, argv[2], F64, &x, &lx)) return throw_amp(env, AMP_ERR_ARG);
   CHECK_NAPI(napi_get_value_int32(env, argv[3], &q));
-  if (lx < lb) return throw_amp(env, AMP_ERR_ARG);
-  CHECK_AMP(amp_solve((amp_num *)h, (const double *)b, (double *)x, q));
+  const num_ref *r = (const num_ref *)h;
+  if (q < 0 || lb < (size_t)(r->n * q) || lx < (size_t)(r->n * q))
+    return throw_len(env, "solve: b / out shorter than n x nrhs");
+  CHECK_AMP(amp_solve(r->num, (const double *)b, (double *)x, q));
   return argv[2];
 }
 
@@ -259,22 +309,25 @@ static napi_value js_refine(napi_env env, napi_callback_info cb) {
   size_t lv, lb, lx;
   amp_refine_opts o;
   CHECK_NAPI(napi_get_value_external(env, argv[0], &h));
-  if (!typed(env, argv[1], &vals, &lv) || !typed(env, argv[2], &b, &lb) || !typed(env, argv[3], &x, &lx))
+  if (!typed(env, argv[1], F64, &vals, &lv) || !typed(env, argv[2], F64, &b, &lb) || !typed(env, argv[3], F64, &x, &lx))
     return throw_amp(env, AMP_ERR_ARG);
+  const num_ref *r = (const num_ref *)h;
+  if (lv != (size_t)r->nnzPattern) return throw_len(env, "refine: values length differs from the pattern's");
+  if (lb < (size_t)r->n || lx < (size_t)r->n) return throw_len(env, "refine: b / out shorter than n");
   CHECK_NAPI(napi_get_value_int32(env, argv[4], &o.method));
   CHECK_NAPI(napi_get_value_int32(env, argv[5], &o.maxSteps));
   CHECK_NAPI(napi_get_value_int32(env, argv[6], &o.restart));
   CHECK_NAPI(napi_get_value_double(env, argv[7], &o.tol));
   amp_refine_info info;
-  CHECK_AMP(amp_refine((amp_num *)h, (const double *)vals, (const double *)b, (double *)x, &o, &info));
-  napi_value r, conv;
-  CHECK_NAPI(napi_create_object(env, &r));
-  set_num(env, r, "steps", info.steps);
+  CHECK_AMP(amp_refine(r->num, (const double *)vals, (const double *)b, (double *)x, &o, &info));
+  napi_value res, conv;
+  CHECK_NAPI(napi_create_object(env, &res));
+  set_num(env, res, "steps", info.steps);
   CHECK_NAPI(napi_get_boolean(env, info.converged != 0, &conv));
-  set_val(env, r, "converged", conv);
-  set_num(env, r, "relResidual", info.relResidual);
-  set_num(env, r, "ms", info.ms);
-  return r;
+  set_val(env, res, "converged", conv);
+  set_num(env, res, "relResidual", info.relResidual);
+  set_num(env, res, "ms", info.ms);
+  return res;
 }
 
 /* downloadFactor(numHandle, Lx, D | null) */
@@ -285,8 +338,11 @@ static napi_value js_download(napi_env env, napi_callback_info cb) {
   void *h, *Lx, *D;
   size_t l1, l2;
   CHECK_NAPI(napi_get_value_external(env, argv[0], &h));
-  if (!typed(env, argv[1], &Lx, &l1) || !typed(env, argv[2], &D, &l2)) return throw_amp(env, AMP_ERR_ARG);
-  CHECK_AMP(amp_download_factor((amp_num *)h, (double *)Lx, (double *)D));
+  if (!typed(env, argv[1], F64, &Lx, &l1) || !typed(env, argv[2], F64, &D, &l2)) return throw_amp(env, AMP_ERR_ARG);
+  const num_ref *r = (const num_ref *)h;
+  if (!Lx || l1 < (size_t)r->nnzLStored || (D && l2 < (size_t)r->n))
+    return throw_len(env, "downloadFactor: Lx / D shorter than the factor");
+  CHECK_AMP(amp_download_factor(r->num, (double *)Lx, (double *)D));
   return argv[1];
 }
 
@@ -298,7 +354,13 @@ static napi_value js_set_pattern(napi_env env, napi_callback_info cb) {
   void *h, *rp, *ci;
   size_t l1, l2;
   CHECK_NAPI(napi_get_value_external(env, argv[0], &h));
-  if (!typed(env, argv[1], &rp, &l1) || !typed(env, argv[2], &ci, &l2) || !rp) return throw_amp(env, AMP_ERR_ARG);
+  if (!typed(env, argv[1], I32, &rp, &l1) || !typed(env, argv[2], I32, &ci, &l2) || !rp)
+    return throw_amp(env, AMP_ERR_ARG);
+  amp_sym_info info;
+  CHECK_AMP(amp_sym_get_info((amp_sym *)h, &info));
+  const size_t n = (size_t)info.n;
+  if (l1 < n + 1 || ((const int32_t *)rp)[n] < 0 || l2 < (size_t)((const int32_t *)rp)[n])
+    return throw_len(env, "setPattern: rowptr / colidx shorter than the pattern");
   CHECK_AMP(amp_sym_set_pattern((amp_sym *)h, (const int32_t *)rp, (const int32_t *)ci));
   return argv[0];
 }
@@ -312,7 +374,9 @@ static napi_value js_fingerprint(napi_env env, napi_callback_info cb) {
   void *rp, *ci;
   size_t l1, l2;
   CHECK_NAPI(napi_get_value_int32(env, argv[0], &n));
-  if (!typed(env, argv[1], &rp, &l1) || !typed(env, argv[2], &ci, &l2)) return throw_amp(env, AMP_ERR_ARG);
+  if (!typed(env, argv[1], I32, &rp, &l1) || !typed(env, argv[2], I32, &ci, &l2)) return throw_amp(env, AMP_ERR_ARG);
+  if (n < 0 || !rp || l1 < (size_t)n + 1 || ((const int32_t *)rp)[n] < 0 || l2 < (size_t)((const int32_t *)rp)[n])
+    return throw_len(env, "fingerprint: rowptr / colidx shorter than the pattern");
   char out[64];
   CHECK_AMP(amp_pattern_fingerprint(n, (const int32_t *)rp, ci ? (const int32_t *)ci : &dummy, out));
   napi_value s;

@@ -50,6 +50,12 @@ const other = { ...pat, colidx: pat.colidx.slice() };
 other.colidx[1] = (other.colidx[1] + 1) % pat.n;
 try { new SparseCholesky(other, null, { sym }); } catch (e) { errors.push(e.message); }
 out.errors = errors;
+// the addon checks element types and lengths against the sizes the library will read
+const addonErrors = [];
+try { patternFingerprint({ n: pat.n, rowptr: Float64Array.from(pat.rowptr), colidx: pat.colidx }); } catch (e) { addonErrors.push(e.message); }
+try { patternFingerprint({ n: pat.n, rowptr: pat.rowptr, colidx: pat.colidx.subarray(0, 10) }); } catch (e) { addonErrors.push(e.message); }
+try { analyze({ n: pat.n + 5, rowptr: pat.rowptr, colidx: pat.colidx }); } catch (e) { addonErrors.push(e.message); }
+out.addon_errors = addonErrors;
 if (mode === '--gpu') {
   const b = Float64Array.from({ length: pat.n }, (_, i) => Math.sin(0.3 * i) + 0.1);
   const ch = new SparseCholesky(pat);
@@ -58,6 +64,7 @@ if (mode === '--gpu') {
   out.relres = residual(pat, val, x, b);
   const xs = solveNumericMulti(ch.num, [b, b.map((v) => 2 * v)]);
   out.multi = Math.max(...xs[1].map((v, i) => Math.abs(v - 2 * x[i])));
+  try { ch.solve(b.subarray(0, pat.n - 1)); } catch (e) { out.short_b = e.message; }
   const bad = new SparseCholesky(pat);
   out.indefinite_ok = bad.factor(laplace(30, 6).val);
   try { bad.solve(b); } catch (e) { out.solve_error = e.message; }

@@ -29,14 +29,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pat, values, target_flops):
-    """The oracle (single-threaded C restatement of the reference factor) on one complete subtree of the same
-    elimination tree, chosen with about target_flops of work."""
+def cpu_baseline(pat, values, b, target_flops):
+    """The oracle (the C restatement of the reference's factor and solve, bitwise the same for any thread count) on
+    the host cores: the complete factor + solve of the same system (timed as real-front-gate.mjs:68-114 times its
+    legs: ordering + symbolic, factor, solve; one run, all cores), and a single-thread sample (the reference's factor
+    is single-threaded) on one complete subtree of about target_flops. Returns (record, oracle solution)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
 
+    model, ncpu = cpu_info()
+    t0 = time.perf_counter()
     S = O.Sym(pat)
+    t_sym = time.perf_counter() - t0
+    O.set_threads(ncpu)
+    t0 = time.perf_counter()
+    ok, Lx = O.chol_factor(S, values)
+    t_fac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x = O.chol_solve(S, Lx, b) if ok else None
+    t_sol = time.perf_counter() - t0
+    del Lx
+    log(f"cpu baseline: oracle analyze {t_sym:.2f} s, factor {t_fac:.1f} s ({S.flops / t_fac / 1e9:.1f} GFLOP/s on "
+        f"{ncpu} threads), solve {t_sol:.2f} s")
+    # single-thread sample: one complete subtree of the same elimination tree
+    O.set_threads(1)
     colflops = np.concatenate([[0.0], np.cumsum(S.colCount.astype(np.float64) ** 2)])
     first = np.arange(S.nsuper)
     for s in range(S.nsuper):  # subtree of s = postorder range [first[s], s]
@@ -46,15 +63,21 @@ def cpu_baseline(pat, values, target_flops):
     sub = colflops[S.superStart[1:]] - colflops[S.superStart[first]]
     root = int(np.argmin(np.abs(sub - target_flops)))
     t0 = time.perf_counter()
-    ok, fl, _ = O.chol_factor_subtree(S, values, root)
+    ok1, fl, _ = O.chol_factor_subtree(S, values, root)
     dt = time.perf_counter() - t0
     m = int(S.Sp[root + 1] - S.Sp[root])
-    model, ncpu = cpu_info()
-    return {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port", "cpu": model, "nproc": ncpu,
-            "sample": f"oracle FP64 factor of the complete subtree of supernode {root} (root front m={m}, supernodes "
-                      f"{first[root]}..{root}) of the same 61^3 elimination tree: {fl:.3e} flops in {dt:.1f} s; the "
-                      f"full factor ({S.flops:.3e} flops) at that rate takes {S.flops / (fl / dt):.0f} s",
-            "ok": bool(ok), "est_full_factor_s": S.flops / (fl / dt)}
+    rec = {"value": S.flops / (t_fac + t_sol) / 1e12, "unit": "TFLOP/s", "cores": ncpu, "kind": "port",
+           "cpu": model, "nproc": ncpu, "ok": bool(ok),
+           "factor_s": t_fac, "solve_s": t_sol, "e2e_s": t_sym + t_fac + t_sol, "ordering_symbolic_s": t_sym,
+           "runs": 1,
+           "sample": f"oracle FP64 factor + solve of the complete 61^3 system ({S.flops:.3e} factor flops) on {ncpu} "
+                     f"threads, one run: factor {t_fac:.1f} s, solve {t_sol:.2f} s; value = factor flops / (factor + "
+                     f"solve)",
+           "single_thread": {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "ok": bool(ok1),
+                             "sample": f"oracle FP64 factor of the complete subtree of supernode {root} (root front "
+                                       f"m={m}, supernodes {first[root]}..{root}): {fl:.3e} flops in {dt:.1f} s",
+                             "full_factor_at_that_rate_s": S.flops / (fl / dt)}}
+    return rec, x
 
 
 def gemm_traffic(path):
@@ -88,9 +111,9 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir",
     solve (+ ir_steps refinement steps for the mixed path) sequence (amp_graph_*); the residual of its result is
     checked by the caller against the tolerance like every path's."""
     import numpy as np
-    import torch
 
     import ampsolve as amp
+    from ampsolve import device as D
 
     kind = "hip" if name.startswith("fp64") else "hip-f32"
     mixed = kind == "hip-f32"
@@ -119,14 +142,14 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir",
     for i in range(warmup):
         info = step()
         log(f"{log_prefix} warmup {i}" + (f": refine {info}" if info else ""))
-    torch.cuda.synchronize()
+    D.synchronize()
     rinfo = []
     t0 = time.perf_counter()
     for _ in range(steps):
         info = step()
         if info:
             rinfo.append(info)
-    torch.cuda.synchronize()
+    D.synchronize()
     dt = (time.perf_counter() - t0) / steps
     res = {"ms_per_step": 1e3 * dt, "tflops": sym.flops / dt / 1e12, "graph": bool(graph)}
     if rinfo:
@@ -140,7 +163,7 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir",
         # profiled step: HIP events around every launch, on the stream each launch runs on
         num.set_profiling(True)
         step()
-        torch.cuda.synchronize()
+        D.synchronize()
         kinds = {k: num.kernel_ms(k) for k in KERNEL_CLASSES}
         num.set_profiling(False)
         gm, gn = kinds["gemm_tile"]
@@ -185,7 +208,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=60)
     ap.add_argument("--paths", default="fp64,mixed,fp64-graph,mixed-graph")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-flops", type=float, default=6e10, help="work of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-flops", type=float, default=6e10,
+                    help="work of the single-thread CPU sample (0: skip the CPU baseline)")
     ap.add_argument("--refine", default="ir", choices=("ir", "gmres"), help="refinement of the mixed path")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 3 plain device copies of this many MiB (calibrates HBM byte counters)")
@@ -197,20 +221,19 @@ def main():
 
     import numpy as np
     import scipy.sparse as sp
-    import torch
 
     import ampsolve as amp
+    from ampsolve import device as D
     from ampsolve import workloads as W
 
-    if not torch.cuda.is_available():
+    if not D.available():
         raise SystemExit("bench.py needs a HIP device")
-    dev = torch.device("cuda:0")
     if args.copy_mb > 0:
-        src = torch.ones(args.copy_mb << 17, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
+        src = D.DeviceArray.from_numpy(np.ones(args.copy_mb << 17))
+        dst = D.DeviceArray(src.shape)
         for _ in range(3):
             dst.copy_(src)
-        torch.cuda.synchronize()
+        D.synchronize()
         log(f"copy calibration: 3 copies of {args.copy_mb} MiB")
         del src, dst
     t0 = time.perf_counter()
@@ -222,19 +245,18 @@ def main():
         f"flops={sym.flops:.3e} gemm flops={sym.gemmFlops:.3e} maxFront={sym.maxFront} "
         f"({time.perf_counter() - t0:.1f} s)")
     A = sp.csr_matrix((values, pat["colidx"], pat["rowptr"]), shape=(pat["n"], pat["n"]))
-    dv = torch.from_numpy(values).to(dev)
-    db = torch.from_numpy(b).to(dev)
+    dv = D.DeviceArray.from_numpy(values)
+    db = D.DeviceArray.from_numpy(b)
     paths = {}
     for name in args.paths.split(","):
-        dx = torch.empty_like(db)
+        dx = D.DeviceArray(db.shape)
         res, num = run_path(name, sym, dv, db, dx, args.steps, args.warmup, args.tol, name, args.refine,
                             graph=name.endswith("-graph"))
-        x = dx.cpu().numpy()
+        x = dx.numpy()
         res["relres"] = float(np.linalg.norm(A @ x - b) / np.linalg.norm(b))
         paths[name] = (res, x)
         log(f"{name}: relative residual {res['relres']:.3e}")
         del num
-        torch.cuda.empty_cache()
     if "fp64" in paths:
         xf = paths["fp64"][1]
         for k in paths:
@@ -286,8 +308,14 @@ def main():
         hbm["solve"] = {"bytes": solve_b, "ms": sm, "achieved": gbs, "frac": gbs / PEAK_HBM}
     result["hbm"] = hbm
     if args.cpu_flops > 0:
-        log("cpu baseline sample ...")
-        result["cpu_baseline"] = cpu_baseline(pat, values, args.cpu_flops)
+        log("cpu baseline: oracle factor + solve of the same system on the host cores ...")
+        rec, xo = cpu_baseline(pat, values, b, args.cpu_flops)
+        result["cpu_baseline"] = rec
+        if xo is not None:  # the error of every path's solution against the FP64 oracle
+            for k, (res, x) in paths.items():
+                res["rel_diff_vs_oracle"] = float(np.abs(x - xo).max() / np.abs(xo).max())
+            result["rel_diff_vs_oracle"] = r["rel_diff_vs_oracle"]
+            rec["gpu_speedup_factor_plus_solve"] = (rec["factor_s"] + rec["solve_s"]) / (r["ms_per_step"] / 1e3)
     print(json.dumps(result), flush=True)
 
 

@@ -166,7 +166,8 @@ int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, in
 
 void amp_num_free(amp_num *num);
 /* Free the factor's update-matrix workspace (needed only while factoring; the next factor allocates it again), so
- * that many factors can stay resident for solves only, as the parametric sweep's factor cache keeps them. */
+ * that many factors can stay resident for solves only, as the parametric sweep's factor cache keeps them.
+ * AMP_ERR_STATE while captured graphs of the handle exist. */
 int amp_num_release_workspace(amp_num *num);
 
 /* Shape thresholds of the device plans (no counterpart in the reference, whose kernel choice is per front,
@@ -191,7 +192,10 @@ int amp_sym_set_params(amp_sym *sym, const amp_tune_info *params);
  * for callers that refactor and resolve the same pattern many times, such as a batch of independent systems
  * assembled into one block-diagonal pattern. The device buffers are read at every replay (new values / right-hand
  * sides in place). Cholesky handles only (AMP_ERR_ARG otherwise); the pattern must be known for irSteps > 0.
- * amp_graph_sync waits for the last replay and reports its factor status (info->ms is the replay's device time). */
+ * amp_graph_sync waits for the last replay and reports its factor status (info->ms is the replay's device time).
+ * A graph holds the addresses of its factor's device buffers: while graphs of an amp_num exist, calls that would move
+ * those buffers are refused with AMP_ERR_STATE (amp_num_release_workspace, and solves with more right-hand sides than
+ * the buffers hold), and every graph must be freed before its amp_num. */
 int amp_graph_create(amp_num *num, const double *d_values, const double *d_b, double *d_x, int32_t nrhs,
                      int32_t irSteps, amp_graph **out);
 int amp_graph_launch(amp_graph *g);

@@ -6,6 +6,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -36,41 +37,83 @@ static inline double weight(const double *F, int64_t m, int32_t t, int32_t c, co
   return x->ldlt ? v * x->D[c] : v;
 }
 
-/* Apply groups g < ng to target columns t0..t0+nt-1 (nt <= 4); target t updates rows [t, m). */
-static void apply_groups(double *F, int64_t m, int32_t t0, int32_t nt, int32_t ng, const front_ctx *x) {
+/* Rows per parallel chunk of a left-looking group update, and the work (rows x groups) below which it stays on one
+ * thread. Splitting the rows of a target column among threads leaves every entry's operation sequence unchanged (the
+ * groups are applied to each row in increasing order), so the result is bitwise the same for any thread count. */
+#define ROW_CHUNK 512
+#define PAR_WORK 65536
+
+/* Groups g < ng into target columns t0 .. t0 + 3, rows [r0, r1) with r0 >= t0 + 3 (rows below every target's
+ * diagonal); w[jj][4 g + q] are the weights of target t0 + jj. */
+static void groups4_rows(double *F, int64_t m, int32_t t0, int32_t ng, const double *w, int64_t r0, int64_t r1) {
+  double *T0 = F + (int64_t)t0 * m, *T1 = T0 + m, *T2 = T1 + m, *T3 = T2 + m;
+  const int64_t W = 4 * (int64_t)ng;
   for (int32_t g = 0; g < ng; g++) {
     const int32_t c = 4 * g;
     const double *S0 = F + c * m, *S1 = S0 + m, *S2 = S1 + m, *S3 = S2 + m;
-    double w[4][4];
-    for (int32_t jj = 0; jj < nt; jj++)
-      for (int32_t q = 0; q < 4; q++) w[jj][q] = weight(F, m, t0 + jj, c + q, x);
-    if (nt == 4) {
-      /* head rows t0 .. t0+2 (targets whose diagonal is at or above the row) */
-      for (int32_t jj = 0; jj < 3; jj++) {
-        double *T = F + (int64_t)(t0 + jj) * m;
-        for (int32_t i = t0 + jj; i < t0 + 3; i++)
-          T[i] = T[i] - (((S0[i] * w[jj][0] + S1[i] * w[jj][1]) + S2[i] * w[jj][2]) + S3[i] * w[jj][3]);
-      }
-      double *T0 = F + (int64_t)t0 * m, *T1 = T0 + m, *T2 = T1 + m, *T3 = T2 + m;
-      const double a0 = w[0][0], a1 = w[0][1], a2 = w[0][2], a3 = w[0][3];
-      const double b0 = w[1][0], b1 = w[1][1], b2 = w[1][2], b3 = w[1][3];
-      const double c0 = w[2][0], c1 = w[2][1], c2 = w[2][2], c3 = w[2][3];
-      const double d0 = w[3][0], d1 = w[3][1], d2 = w[3][2], d3 = w[3][3];
-      for (int64_t i = t0 + 3; i < m; i++) {
-        const double s0 = S0[i], s1 = S1[i], s2 = S2[i], s3 = S3[i];
-        T0[i] = T0[i] - (((s0 * a0 + s1 * a1) + s2 * a2) + s3 * a3);
-        T1[i] = T1[i] - (((s0 * b0 + s1 * b1) + s2 * b2) + s3 * b3);
-        T2[i] = T2[i] - (((s0 * c0 + s1 * c1) + s2 * c2) + s3 * c3);
-        T3[i] = T3[i] - (((s0 * d0 + s1 * d1) + s2 * d2) + s3 * d3);
-      }
-    } else {
-      for (int32_t jj = 0; jj < nt; jj++) {
-        double *T = F + (int64_t)(t0 + jj) * m;
-        const double w0 = w[jj][0], w1 = w[jj][1], w2 = w[jj][2], w3 = w[jj][3];
-        for (int64_t i = t0 + jj; i < m; i++) T[i] = T[i] - (((S0[i] * w0 + S1[i] * w1) + S2[i] * w2) + S3[i] * w3);
-      }
+    const double a0 = w[c], a1 = w[c + 1], a2 = w[c + 2], a3 = w[c + 3];
+    const double b0 = w[W + c], b1 = w[W + c + 1], b2 = w[W + c + 2], b3 = w[W + c + 3];
+    const double c0 = w[2 * W + c], c1 = w[2 * W + c + 1], c2 = w[2 * W + c + 2], c3 = w[2 * W + c + 3];
+    const double d0 = w[3 * W + c], d1 = w[3 * W + c + 1], d2 = w[3 * W + c + 2], d3 = w[3 * W + c + 3];
+    for (int64_t i = r0; i < r1; i++) {
+      const double s0 = S0[i], s1 = S1[i], s2 = S2[i], s3 = S3[i];
+      T0[i] = T0[i] - (((s0 * a0 + s1 * a1) + s2 * a2) + s3 * a3);
+      T1[i] = T1[i] - (((s0 * b0 + s1 * b1) + s2 * b2) + s3 * b3);
+      T2[i] = T2[i] - (((s0 * c0 + s1 * c1) + s2 * c2) + s3 * c3);
+      T3[i] = T3[i] - (((s0 * d0 + s1 * d1) + s2 * d2) + s3 * d3);
     }
   }
+}
+
+/* Groups g < ng into one target column t, rows [r0, r1) (r0 >= t). */
+static void groups1_rows(double *F, int64_t m, int32_t t, int32_t ng, const double *w, int64_t r0, int64_t r1) {
+  double *T = F + (int64_t)t * m;
+  for (int32_t g = 0; g < ng; g++) {
+    const int32_t c = 4 * g;
+    const double *S0 = F + c * m, *S1 = S0 + m, *S2 = S1 + m, *S3 = S2 + m;
+    const double w0 = w[c], w1 = w[c + 1], w2 = w[c + 2], w3 = w[c + 3];
+    for (int64_t i = r0; i < r1; i++) T[i] = T[i] - (((S0[i] * w0 + S1[i] * w1) + S2[i] * w2) + S3[i] * w3);
+  }
+}
+
+/* Apply groups g < ng to target columns t0..t0+nt-1 (nt <= 4); target t updates rows [t, m). The weights are read
+ * from the source columns' rows t0 .. t0 + nt - 1, which no target update writes. par: rows may be split among the
+ * threads (not inside an enclosing parallel loop). */
+static void apply_groups(double *F, int64_t m, int32_t t0, int32_t nt, int32_t ng, const front_ctx *x, int par) {
+  if (ng == 0) return;
+  double wbuf[4 * 4 * 64];
+  double *w = 4 * ng <= 4 * 64 ? wbuf : malloc(sizeof(double) * 16 * (size_t)ng);
+  const int64_t W = 4 * (int64_t)ng;
+  for (int32_t jj = 0; jj < nt; jj++)
+    for (int32_t c = 0; c < 4 * ng; c++) w[jj * W + c] = weight(F, m, t0 + jj, c, x);
+  if (nt == 4) {
+    /* head rows t0 .. t0+2 (targets whose diagonal is at or above the row) */
+    for (int32_t jj = 0; jj < 3; jj++) groups1_rows(F, m, t0 + jj, ng, w + jj * W, t0 + jj, t0 + 3);
+    const int64_t r0 = t0 + 3, rows = m - r0;
+    if (par && rows * ng >= PAR_WORK && rows > ROW_CHUNK) {
+      const int64_t nchunk = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < nchunk; k++) {
+        const int64_t a = r0 + k * ROW_CHUNK, b = a + ROW_CHUNK < m ? a + ROW_CHUNK : m;
+        groups4_rows(F, m, t0, ng, w, a, b);
+      }
+    } else if (rows > 0)
+      groups4_rows(F, m, t0, ng, w, r0, m);
+  } else {
+    for (int32_t jj = 0; jj < nt; jj++) {
+      const int64_t r0 = t0 + jj, rows = m - r0;
+      if (par && rows * ng >= PAR_WORK && rows > ROW_CHUNK) {
+        const int64_t nchunk = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < nchunk; k++) {
+          const int64_t a = r0 + k * ROW_CHUNK, b = a + ROW_CHUNK < m ? a + ROW_CHUNK : m;
+          groups1_rows(F, m, t0 + jj, ng, w + jj * W, a, b);
+        }
+      } else
+        groups1_rows(F, m, t0 + jj, ng, w + jj * W, r0, m);
+    }
+  }
+  if (w != wbuf) free(w);
 }
 
 /* Single source column c into target t, rows [t, m), skipped when the weight is zero. */
@@ -110,7 +153,7 @@ static int front_factor(double *F, int32_t m, int32_t nc, front_ctx *x, double *
   const int64_t M = m;
   int32_t k = 0;
   for (; k + 3 < nc; k += 4) {
-    apply_groups(F, M, k, 4, k >> 2, x);
+    apply_groups(F, M, k, 4, k >> 2, x, 1);
     for (int32_t jj = 0; jj < 4; jj++) {
       const int32_t t = k + jj;
       for (int32_t c = k; c < t; c++) apply_single(F, M, t, c, x);
@@ -119,15 +162,19 @@ static int front_factor(double *F, int32_t m, int32_t nc, front_ctx *x, double *
   }
   for (; k < nc; k++) {
     const int32_t ng = k >> 2;
-    apply_groups(F, M, k, 1, ng, x);
+    apply_groups(F, M, k, 1, ng, x, 1);
     for (int32_t c = 4 * ng; c < k; c++) apply_single(F, M, k, c, x);
     if (!pivot(F, M, k, x, Dout)) return 0;
   }
+  /* Schur block: target column groups are independent (each reads only the factored columns) */
   const int32_t ng = nc >> 2;
-  int32_t j = nc;
-  for (; j < m; j += 4) {
+  const int32_t nj = (m - nc + 3) / 4;
+  const int par = (int64_t)(m - nc) * (m - nc) / 2 * (ng + 1) >= 4 * PAR_WORK;
+#pragma omp parallel for schedule(dynamic, 4) if (par)
+  for (int32_t q = 0; q < nj; q++) {
+    const int32_t j = nc + 4 * q;
     const int32_t nt = m - j < 4 ? m - j : 4;
-    apply_groups(F, M, j, nt, ng, x);
+    apply_groups(F, M, j, nt, ng, x, 0);
     for (int32_t c = 4 * ng; c < nc; c++)
       for (int32_t jj = 0; jj < nt; jj++) apply_single(F, M, j + jj, c, x);
   }
@@ -152,6 +199,7 @@ static int multifrontal(const orc_sym *S, const double *values, double *Lx, fron
     const int32_t r0 = S->Sp[s], m = S->Sp[s + 1] - r0;
     const int64_t M = m;
     for (int32_t r = 0; r < m; r++) rel[S->Si[r0 + r]] = r;
+#pragma omp parallel for schedule(dynamic, 16) if (M * M >= 8 * PAR_WORK)
     for (int64_t b = 0; b < M; b++) memset(F + b * M + b, 0, (size_t)(M - b) * 8); /* zero the lower part */
     for (int32_t c = 0; c < nc; c++) { /* assemble A's own columns (+= onto +0) */
       const int32_t j = f + c;
@@ -162,6 +210,9 @@ static int multifrontal(const orc_sym *S, const double *values, double *Lx, fron
       const int64_t off = stackPos[--nPos];
       const int64_t mu = stackDim[ch];
       const int32_t cr0 = S->Sp[ch] + (S->superStart[ch + 1] - S->superStart[ch]);
+      /* the relative map is injective: each front entry receives at most one add from this child, so columns of
+       * the child may go to different threads */
+#pragma omp parallel for schedule(dynamic, 16) if (mu * mu >= 8 * PAR_WORK)
       for (int64_t b = 0; b < mu; b++) {
         double *Fb = F + (int64_t)rel[S->Si[cr0 + b]] * M;
         const double *ub = stack + off + b * mu;
@@ -184,6 +235,7 @@ static int multifrontal(const orc_sym *S, const double *values, double *Lx, fron
         stack = realloc(stack, (size_t)bigger * 8);
         cap = bigger;
       }
+#pragma omp parallel for schedule(dynamic, 16) if (mu * mu >= 8 * PAR_WORK)
       for (int64_t b = 0; b < mu; b++) {
         const double *src = F + (nc + b) * M + nc;
         double *dst = stack + top + b * mu;
@@ -201,6 +253,9 @@ static int multifrontal(const orc_sym *S, const double *values, double *Lx, fron
   free(stack);
   return ok;
 }
+
+void orc_set_threads(int32_t k) { omp_set_num_threads(k > 0 ? k : 1); }
+int32_t orc_get_threads(void) { return omp_get_max_threads(); }
 
 int orc_chol_factor(const orc_sym *S, const double *values, double *Lx) {
   front_ctx x = {0, NULL, 0, 0, 0};

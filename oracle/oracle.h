@@ -54,6 +54,11 @@ void orc_sym_free(orc_sym *s);
 /* patternFingerprint (sparse.mjs:908-925): writes "n:nnz:hex" into out (>= 48 bytes). */
 void orc_fingerprint(int32_t n, const int32_t *rowptr, const int32_t *colidx, char *out);
 
+/* Threads of the numeric factorization (OpenMP; rows and target columns of a front are split among them without
+ * changing any entry's operation order, so the factor is bitwise the same for every thread count). */
+void orc_set_threads(int32_t k);
+int32_t orc_get_threads(void);
+
 /* factorNumeric (sparse.mjs:941-1125): Lx has nnzLStored entries. Returns 1 when positive definite, else 0. */
 int orc_chol_factor(const orc_sym *s, const double *values, double *Lx);
 /* factorNumeric restricted to the complete subtree of supernode `root` (a contiguous postorder range); flops of its

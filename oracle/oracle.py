@@ -90,6 +90,8 @@ def lib():
         L.orc_uplook_factor.restype = C.c_int
         L.orc_uplook_solve.argtypes = [C.POINTER(OrcUplook), f64p, f64p]
         L.orc_uplook_free.argtypes = [C.POINTER(OrcUplook)]
+        L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_get_threads.restype = C.c_int32
         L.orc_gmres.argtypes = [C.c_int32, APPLY_FN, C.c_void_p, APPLY_FN, C.c_void_p, f64p, C.c_double, C.c_int32,
                                 C.c_int32, C.c_void_p, f64p, C.POINTER(GmresInfo)]
         L.orc_csr_matvec.argtypes = [C.c_int32, i32p, i32p, f64p, f64p, f64p]
@@ -165,6 +167,15 @@ class Sym:
                 self._p = None
         except Exception:
             pass
+
+
+def set_threads(k):
+    """Threads of the numeric factorization (the factor is bitwise the same for every count)."""
+    lib().orc_set_threads(int(k))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
 
 
 def chol_factor(sym, values):

@@ -11,29 +11,19 @@ for p in (os.path.join(ROOT, "applied-multiphysics_amd"), os.path.join(ROOT, "or
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
-    # PyTorch-ROCm carries its own HIP runtime: bring its device up before libampsolve initialises /opt/rocm's, so
-    # that tests which hand torch device buffers to the library see the device in both
-    try:
-        import torch
-
-        torch.cuda.is_available()
-    except Exception:
-        pass
 
 
 def gpu_available():
-    try:
-        import torch
+    from ampsolve import device as D
 
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    return D.available()
 
 
 @pytest.fixture(scope="session")
 def device():
-    import torch
+    """Device buffers come from the library's own HIP runtime (ampsolve.device.DeviceArray)."""
+    from ampsolve import device as D
 
-    if not torch.cuda.is_available():
+    if not D.available():
         pytest.fail("GPU test run without a HIP device")
-    return torch.device("cuda:0")
+    return D

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <stdexcept>
 #include <vector>
 
 #include "../../applied-multiphysics_amd/csrc/plan.h"
@@ -191,6 +192,51 @@ void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, 
 
 void* ps_create(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax) {
   return ps_create_params(n, rowptr, colidx, perm, relax, nullptr);
+}
+
+// Assembly map of a synthetic analysis with one tall, wide tiled front: supernode 0 holds columns [0, nc) with rows
+// 0 .. m - 1 (its below rows are the columns of supernode 1, the root, [nc, m)); A holds the diagonal and the entry
+// (m - 1, nc - 1). out: [0] the plan's Lx index of that entry, [1] its position in the reference layout
+// (Lxp[0] + (m - 1) + (nc - 1) m), [2] product tasks of the plan. Returns 0, or 1 when the plan refuses the shape.
+int32_t ps_tall_front_map(int32_t nc, int32_t m, int32_t superPanel, int64_t* out) {
+  Symbolic S;
+  S.n = m;
+  S.nsuper = 2;
+  S.superStart = {0, nc, m};
+  S.Sp = {0, m, m + (m - nc)};
+  S.Si.resize(size_t(m) + (m - nc));
+  for (int32_t i = 0; i < m; i++) S.Si[i] = i;
+  for (int32_t i = nc; i < m; i++) S.Si[size_t(m) + (i - nc)] = i;
+  S.superParent = {1, -1};
+  S.childHead = {-1, 0};
+  S.childNext = {-1, -1};
+  S.colSuper.resize(m);
+  for (int32_t j = 0; j < m; j++) S.colSuper[j] = j < nc ? 0 : 1;
+  S.Lxp = {0, int64_t(m) * nc, int64_t(m) * nc + int64_t(m - nc) * (m - nc)};
+  S.nnzLStored = S.Lxp[2];
+  S.maxFront = m;
+  S.lowerP.assign(size_t(m) + 1, 0);
+  for (int32_t j = 0; j < m; j++) {
+    S.lowerI.push_back(j);
+    if (j == nc - 1) S.lowerI.push_back(m - 1);
+    S.lowerP[j + 1] = int32_t(S.lowerI.size());
+  }
+  S.nnzA = int64_t(S.lowerI.size());
+  S.lowerSrc.resize(S.lowerI.size());
+  for (size_t q = 0; q < S.lowerSrc.size(); q++) S.lowerSrc[q] = int32_t(q);
+  S.nnzPattern = S.nnzA;
+  PlanParams pp;
+  pp.superPanel = superPanel;
+  FactorPlan F;
+  try {
+    F = buildFactorPlan(S, pp);
+  } catch (const std::length_error&) {
+    return 1;
+  }
+  out[0] = F.adst[S.lowerP[nc - 1] + 1];
+  out[1] = S.Lxp[0] + (m - 1) + int64_t(nc - 1) * m;
+  out[2] = int64_t(F.gemm.size());
+  return 0;
 }
 
 void ps_free(void* p) { delete static_cast<Handle*>(p); }

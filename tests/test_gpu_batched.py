@@ -51,16 +51,14 @@ def test_union_fp64_matches_oracle(batch):
 
 
 def test_captured_mixed_sequence(batch, device):
-    import torch
-
     print("batched graph", flush=True)
     pat, v, B, offs, A, sym, Xo = batch
-    dv = torch.from_numpy(v).to(device)
+    dv = device.DeviceArray.from_numpy(v)
     tune = sym.autotune(dv, kind="hip-f32")  # shape thresholds for this batch, timed on the device
     print(f"batched autotune: {tune}", flush=True)
     num = amp.NumericFactor(sym, 0, "hip-f32")
-    dB = torch.from_numpy(B.ravel().copy()).to(device)
-    dX = torch.zeros_like(dB)
+    dB = device.DeviceArray.from_numpy(B.ravel())
+    dX = device.DeviceArray.zeros(dB.shape)
     g = num.graph(dv, dB, dX, NRHS, 3)
     times = []
     for _ in range(3):
@@ -70,14 +68,14 @@ def test_captured_mixed_sequence(batch, device):
         times.append(time.perf_counter() - t0)
         assert st["ok"]
     print(f"batched graph replay: device {st['ms']:.2f} ms, wall {1e3 * min(times):.2f} ms", flush=True)
-    X = dX.cpu().numpy().reshape(NRHS, -1)
+    X = dX.numpy().reshape(NRHS, -1)
     rel = per_system((A @ X.T).T - B, offs) / per_system(B, offs)
     assert rel.max() <= 1e-10, rel.max()
     err = per_system(X - Xo, offs) / per_system(Xo, offs)
     assert err.max() < 1e-8
     # new values in place: 2 A gives x / 2 from the same graph
-    dv.mul_(2.0)
+    dv.upload(2.0 * v)
     g.launch()
     assert g.sync()["ok"]
-    X2 = dX.cpu().numpy().reshape(NRHS, -1)
+    X2 = dX.numpy().reshape(NRHS, -1)
     assert np.abs(2 * X2 - X).max() / np.abs(X).max() < 1e-9

@@ -20,8 +20,6 @@ def relnorm(a, b):
 @pytest.mark.parametrize("dims,B", [((8, 32, 300), 37), ((8, 64, 3000), 64)])
 def test_network_kernels_match_reference(device, dims, B):
     """Forward, loss, gradient (the 3000-output case takes the depth-split product) and one Adam step."""
-    import torch
-
     nIn, H, nOut = dims
     net = SW.WarmStartNet(nIn, H, nOut, 64, seed=5)
     th = net.get_params().astype(np.float64)
@@ -32,11 +30,11 @@ def test_network_kernels_match_reference(device, dims, B):
     th = net.get_params().astype(np.float64)
     X = (2 * W.lcg_uniform(23, B * nIn) - 1).reshape(B, nIn).astype(np.float32)
     T = W.lcg_normal(24, B * nOut).reshape(B, nOut).astype(np.float32)
-    dX, dT = torch.from_numpy(X).to(device), torch.from_numpy(T).to(device)
-    Y = torch.empty((B, nOut), dtype=torch.float32, device=device)
+    dX, dT = device.DeviceArray.from_numpy(X), device.DeviceArray.from_numpy(T)
+    Y = device.DeviceArray((B, nOut), np.float32)
     net.forward(dX, Y)
     _, _, yr = R.forward(th, X, nIn, H, nOut)
-    assert relnorm(Y.cpu().numpy(), yr) < 1e-5
+    assert relnorm(Y.numpy(), yr) < 1e-5
     loss = net.gradient(dX, dT)
     lr_, gr = R.loss_and_grad(th, X, T, nIn, H, nOut)
     assert abs(loss - lr_) / lr_ < 1e-5
@@ -53,25 +51,21 @@ def test_network_kernels_match_reference(device, dims, B):
 
 def test_pcg_with_exact_preconditioner_converges_in_one_step(device):
     """With the FP64 factor of the same matrix as preconditioner CG stops after one iteration from a zero guess."""
-    import torch
-
     prob = SW.ParamHeat3D(8)
     sym = SW.analyze(prob.pat)
     th = 2 * W.lcg_uniform(31, 8) - 1
-    dv = torch.from_numpy(prob.values(th)).to(device)
+    dv = device.DeviceArray.from_numpy(prob.values(th))
     num = SW.NumericFactor(sym, 0, "hip")
     assert num.factor(dv)
-    db = torch.from_numpy(prob.b).to(device)
-    x = torch.zeros_like(db)
+    db = device.DeviceArray.from_numpy(prob.b)
+    x = device.DeviceArray.zeros(db.shape)
     info = SW.pcg(num, dv, db, x, tol=1e-10, maxIter=10)
     assert info["converged"] and info["iters"] == 1 and info["initialResidual"] == 1.0
 
 
 def test_reduced_learned_sweep(device):
-    import torch
-
     print("sweep: reduced config", flush=True)
-    sw = SW.LearnedSweep(ncell=12, hidden=128, device="cuda:0")
+    sw = SW.LearnedSweep(ncell=12, hidden=128)
     train = (2 * W.lcg_uniform(41, 300 * 8) - 1).reshape(300, 8)
     held = (2 * W.lcg_uniform(42, 24 * 8) - 1).reshape(24, 8)
     st = sw.fit(train, epochs=150, batch=32, lr=3e-3)
@@ -83,14 +77,14 @@ def test_reduced_learned_sweep(device):
         x, info = sw.solve(th, X0[i])
         v = sw.prob.values(th)
         A = sp.csr_matrix((v, sw.prob.pat["colidx"], sw.prob.pat["rowptr"]), shape=(sw.prob.pat["n"],) * 2)
-        xh = x.cpu().numpy()
+        xh = x.numpy()
         assert np.linalg.norm(A @ xh - sw.prob.b) / np.linalg.norm(sw.prob.b) <= 1e-10
         xd, _ = sw.solve_direct(th)
-        err.append(relnorm(xh, xd.cpu().numpy()))  # judged against the FP64 direct solve
+        err.append(relnorm(xh, xd.numpy()))  # judged against the FP64 direct solve
         if not info["fallback"]:
             it_net.append(info["iters"])
-            z = torch.zeros_like(X0[i])
-            zi = SW.pcg(sw.cache[info["nearest"]], torch.from_numpy(v).to(device), sw.db, z, maxIter=100)
+            z = device.DeviceArray.zeros(X0[i].shape)
+            zi = SW.pcg(sw.cache[info["nearest"]], device.DeviceArray.from_numpy(v), sw.db, z, maxIter=100)
             it_zero.append(zi["iters"] if zi["converged"] else 100)
     print(f"sweep: PCG iterations with the network's warm start {np.mean(it_net):.2f}, "
           f"from zero {np.mean(it_zero):.2f} over {len(it_net)} points", flush=True)

@@ -45,6 +45,10 @@ def test_js_analysis_and_errors_cpu():
     assert out["errors"][0].startswith("unknown real factor fast")
     assert "values for a pattern with" in out["errors"][1]
     assert out["errors"][2] == "symbolic analysis is for a different sparsity pattern"
+    # wrong element type, short colidx, rowptr shorter than n + 1: refused by the addon before the library reads
+    assert len(out["addon_errors"]) == 3
+    assert "shorter than the pattern" in out["addon_errors"][1]
+    assert "shorter than the pattern" in out["addon_errors"][2]
 
 
 @needs_node
@@ -53,6 +57,7 @@ def test_js_factor_solve_gpu():
     out = run("--gpu")
     assert out["ok"] is True and out["relres"] < 1e-12
     assert out["multi"] < 1e-12
+    assert out["short_b"] == "solve: b / out shorter than n x nrhs"
     assert out["indefinite_ok"] is False
     assert out["solve_error"] == "SparseCholesky: solve without a successful factorization"
     assert out["ldlt"]["ok"] and out["ldlt"]["nNeg"] > 0

@@ -101,3 +101,50 @@ def test_sym_import_roundtrip():
     for f in FIELDS:
         assert np.array_equal(getattr(t, f), getattr(s, f)), f
     assert t.flops == s.flops and t.gemmFlops == s.gemmFlops
+
+
+def _import_desc(s, nnz, overrides):
+    d = _capi.SymDesc()
+    d.n, d.nsuper, d.nnzPattern = s.n, overrides.pop("nsuper", s.nsuper), overrides.pop("nnzPattern", nnz)
+    keep = {}
+    for f in ("perm", "parent", "colCount", "superStart", "Sp", "Si", "lowerP", "lowerI", "lowerSrc"):
+        keep[f] = np.ascontiguousarray(overrides.get(f, getattr(s, f)), dtype=np.int32)
+        setattr(d, f, keep[f].ctypes.data)
+    return d, keep
+
+
+def _corrupt(s, field, fn):
+    a = np.array(getattr(s, field), dtype=np.int32, copy=True)
+    fn(a)
+    return {field: a}
+
+
+CORRUPTIONS = {
+    "perm duplicate": lambda s: _corrupt(s, "perm", lambda a: a.__setitem__(1, a[0])),
+    "parent below child": lambda s: _corrupt(s, "parent", lambda a: a.__setitem__(5, 2)),
+    "parent out of range": lambda s: _corrupt(s, "parent", lambda a: a.__setitem__(0, s.n + 7)),
+    "superStart not increasing": lambda s: _corrupt(s, "superStart", lambda a: a.__setitem__(2, a[1])),
+    "Sp not increasing": lambda s: _corrupt(s, "Sp", lambda a: a.__setitem__(3, a[2] - 1)),
+    "Si row out of range": lambda s: _corrupt(s, "Si", lambda a: a.__setitem__(s.Sp[1] - 1, s.n + 3)),
+    "Si own column": lambda s: _corrupt(s, "Si", lambda a: a.__setitem__(0, 1)),
+    "lowerI out of range": lambda s: _corrupt(s, "lowerI", lambda a: a.__setitem__(len(a) - 1, s.n)),
+    "lowerI off the structure": lambda s: _corrupt(s, "lowerI", lambda a: a.__setitem__(0, s.n - 1)),
+    "lowerSrc out of range": lambda s: _corrupt(s, "lowerSrc", lambda a: a.__setitem__(3, 1 << 30)),
+    "lowerP not increasing": lambda s: _corrupt(s, "lowerP", lambda a: a.__setitem__(4, a[3] - 1)),
+    "colCount zero": lambda s: _corrupt(s, "colCount", lambda a: a.__setitem__(0, 0)),
+    "no supernodes": lambda s: {"nsuper": 0},
+}
+
+
+@pytest.mark.parametrize("what", list(CORRUPTIONS))
+def test_sym_import_rejects_corrupted_descriptors(what):
+    """Every index the plans and the kernels follow is range-checked at the boundary: a corrupted analysis is an
+    argument error, never an out-of-range access."""
+    pat = grid27(6, 5, 4)[0]
+    s = amp.analyze(pat)
+    d, keep = _import_desc(s, pat["nnz"], CORRUPTIONS[what](s))
+    h = C.c_void_p()
+    rc = _capi.lib().amp_sym_import(C.byref(d), C.byref(h))
+    assert rc == _capi.AMP_ERR_ARG, (what, rc)
+    assert not h.value
+    assert _capi.lib().amp_last_error().decode().startswith("amp_sym_import")

@@ -205,3 +205,133 @@ def fnv1a_reference(pat):
 def test_fingerprint_independent():
     for pat in (W.poisson2d(8)[0], grid27(5, 4, 3)[0]):
         assert O.fingerprint(pat) == fnv1a_reference(pat)
+
+
+# ---- symbolic pins: published fill figures and the relaxed-amalgamation thresholds --------------------------------
+
+def tet10_box_pattern(nx, restrict_interior):
+    """Node pattern of the tet10 unit-cube mesh of mesh.ts:123-161 (nx^3 hexes, 6 Kuhn tets each, midside nodes on
+    the (2 nx + 1)^3 fine grid, nid(i, j, k) = (k (NY + 1) + j) (NX + 1) + i): every pair of nodes of a tet is coupled,
+    diagonal included (cellPattern). With restrict_interior the pattern is restricted to the nodes off all six faces,
+    keeping the node order (restrict(), scalar.ts:210-233)."""
+    N = 2 * nx + 1
+    kc = W._kuhn_corners()  # (6, 4, 3) corner offsets, oriented as the mesh orients them
+    hk, hj, hi = np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nx), indexing="ij")
+    base = 2 * np.stack([hi.ravel(), hj.ravel(), hk.ravel()], 1)  # fine-grid origin of each hex
+    edges = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+    pts = np.concatenate([2 * kc] + [kc[:, [a]] + kc[:, [b]] for a, b in edges], axis=1)  # (6, 10, 3)
+    fine = base[:, None, None, :] + pts[None]  # (nh, 6, 10, 3)
+    ids = ((fine[..., 2] * N + fine[..., 1]) * N + fine[..., 0]).reshape(-1, 10)
+    rows = np.repeat(ids, 10, axis=1).ravel()
+    cols = np.tile(ids, (1, 10)).ravel()
+    nn = N ** 3
+    M = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(nn, nn))
+    if restrict_interior:
+        g = np.arange(nn)
+        i, j, k = g % N, (g // N) % N, g // (N * N)
+        inner = (i > 0) & (i < N - 1) & (j > 0) & (j < N - 1) & (k > 0) & (k < N - 1)
+        keep = np.nonzero(inner)[0]
+        M = M[keep][:, keep]
+    return W.csr_dict(M)
+
+
+# DevLog-045-perf-core-1.md:117-128: AMD symbolic flop per DOF of the 3D unit-cube tet10 heat problem at h 1 / 0.7 / 0.5
+# (nx = 8 / 12 / 16): unrestricted node pattern 6.0e4 / 2.7e5 / 7.6e5 per node; restricted to the free DOFs (all six
+# faces Dirichlet: 3,375 / 12,167 / 29,791 DOFs) 1.92e5 at h 0.7 and 7.63e5 at h 0.5.
+PUBLISHED_FLOPS_PER_DOF = [(8, False, "6.0e+04"), (12, False, "2.7e+05"), (16, False, "7.6e+05"),
+                           (12, True, "1.92e+05"), (16, True, "7.63e+05")]
+
+
+@pytest.mark.parametrize("nx,restricted,published", PUBLISHED_FLOPS_PER_DOF)
+def test_amd_fill_matches_published_tet10_figures(nx, restricted, published):
+    """The reference publishes the fill its AMD + analysis produce on this mesh family; a different tie-break, hash,
+    absorption or amalgamation step moves these figures (the natural order, for instance, gives 2.0e5 instead of
+    6.0e4 at nx = 8). Both the oracle and the native symbolic phase must reproduce them to the published digits."""
+    pat = tet10_box_pattern(nx, restricted)
+    if restricted:
+        assert pat["n"] == (2 * nx - 1) ** 3
+    digits = len(published.split("e")[0].split(".")[1])
+    S = O.Sym(pat)
+    assert f"{S.flops / pat['n']:.{digits}e}" == published
+    import ampsolve as amp
+
+    sym = amp.analyze(pat)
+    assert sym.flops == S.flops and np.array_equal(sym.perm, S.perm) and np.array_equal(sym.Si, S.Si)
+
+
+def merge_tree_pattern(ncC, ncP, t, g):
+    """Natural-order pattern with four fundamental supernodes C, P, H, G (columns in that order): C a clique of ncC
+    columns coupled to the first t columns of P; P a clique of ncP columns coupled to all g columns of G; H one column
+    coupled to the first column of G (G's second child, which keeps P and G apart); G a clique (the root)."""
+    n = ncC + ncP + 1 + g
+    C = np.arange(ncC)
+    P = ncC + np.arange(ncP)
+    H = ncC + ncP
+    G = ncC + ncP + 1 + np.arange(g)
+    M = np.zeros((n, n), bool)
+    for blk in (C, P, G):
+        M[np.ix_(blk, blk)] = True
+    M[np.ix_(C, P[:t])] = True
+    M[np.ix_(P, G)] = True
+    M[H, H] = M[H, G[0]] = True
+    M = M | M.T
+    return W.csr_dict(sp.csr_matrix(M.astype(float))), ncC
+
+
+# (ncC, ncP, t, g) -> merged? The merged block's explicit zeros are ncC (ncP + g - t) of
+# nc m - nc (nc - 1) / 2 stored entries (nc = ncC + ncP, m = ncC + ncP + g); zf = zeros / stored, worked by hand:
+MERGE_CASES = [
+    ((3, 1, 1, 200), True),     # nc 4: zf 0.741, merged by nc <= 4
+    ((15, 1, 1, 10), True),     # nc 16: zf 0.507 < 0.8
+    ((16, 1, 1, 10), False),    # nc 17: zf 0.495, above 0.1 and 0.05
+    ((15, 1, 1, 49), True),     # nc 16: zf 735 / 920 = 0.79891 < 0.8
+    ((15, 1, 1, 50), False),    # nc 16: zf 750 / 936 = 0.80128
+    ((47, 1, 1, 2), True),      # nc 48: zf 94 / 1272 = 0.0739 < 0.1
+    ((47, 1, 1, 3), False),     # nc 48: zf 141 / 1320 = 0.1068
+    ((48, 1, 1, 2), False),     # nc 49: zf 96 / 1323 = 0.0726, not < 0.05
+    ((99, 1, 1, 2), True),      # nc 100: zf 198 / 5250 = 0.0377 < 0.05
+    ((99, 1, 1, 3), False),     # nc 100: zf 297 / 5350 = 0.0555
+]
+
+
+@pytest.mark.parametrize("params,merged", MERGE_CASES)
+def test_relaxed_amalgamation_thresholds(params, merged):
+    """sparse.mjs:790-812 (CHOLMOD's defaults): a child supernode joins its parent when nc <= 4, or nc <= 16 and
+    zf < 0.8, or nc <= 48 and zf < 0.1, or zf < 0.05. One case just inside and one just outside each threshold."""
+    pat, ncC = merge_tree_pattern(*params)
+    n = pat["n"]
+    for relax in (True, False):
+        S = O.Sym(pat, perm=np.arange(n), relax=relax)
+        together = S.colSuper[0] == S.colSuper[ncC]
+        assert together == (merged and relax)
+        assert S.colSuper[0] == S.colSuper[ncC - 1]  # C is one fundamental supernode
+        import ampsolve as amp
+
+        sym = amp.analyze(pat, {"perm": np.arange(n, dtype=np.int32), "relax": relax})
+        assert np.array_equal(sym.superStart, S.superStart)
+    # the merged block stores the explicit zeros: nnzLStored grows by exactly that count over the unrelaxed figure
+    ncC_, ncP, t, g = params
+    S1 = O.Sym(pat, perm=np.arange(n), relax=True)
+    S0 = O.Sym(pat, perm=np.arange(n), relax=False)
+    assert S0.nnzL == S1.nnzL
+    if merged:
+        assert S1.nnzLStored - S0.nnzLStored >= ncC_ * (ncP + g - t)
+
+
+def test_threaded_factor_is_bitwise_the_single_thread_one():
+    """The oracle splits front rows and Schur target columns among threads without changing any entry's operation
+    order: factors with 1 and 4 threads are bitwise equal (Cholesky and LDL^T)."""
+    pat, v, _ = W.heat3d(20)
+    S = O.Sym(pat)
+    keep = O.get_threads()
+    try:
+        O.set_threads(1)
+        ok1, L1 = O.chol_factor(S, v)
+        _, Ld1, D1, nn1, _ = O.ldlt_factor(S, v)
+        O.set_threads(4)
+        ok4, L4 = O.chol_factor(S, v)
+        _, Ld4, D4, nn4, _ = O.ldlt_factor(S, v)
+    finally:
+        O.set_threads(keep)
+    assert ok1 and ok4 and S.maxFront > 512
+    assert np.array_equal(L1, L4) and np.array_equal(Ld1, Ld4) and np.array_equal(D1, D4) and nn1 == nn4 == 0

@@ -141,3 +141,19 @@ def test_plan_two_stream_ordering(sim, name, prm):
         assert np.abs(Lg[m] - Lo[m]).max() / np.abs(Lo[m]).max() < 1e-12
     finally:
         sim.ps_free(h)
+
+
+def test_tiled_front_map_past_2_to_31_entries(sim):
+    """A tiled front whose m x nc block passes 2^31 entries (66,000 rows x 33,000 columns) while m x super-panel width
+    stays below it: the assembly map of an entry at column nc - 1, row m - 1 must be its exact position in the
+    reference's block layout (int64), not a wrapped 32-bit offset."""
+    sim.ps_tall_front_map.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
+    sim.ps_tall_front_map.restype = C.c_int32
+    out = (C.c_int64 * 3)()
+    nc, m = 33000, 66000
+    assert sim.ps_tall_front_map(nc, m, 4096, out) == 0
+    assert out[1] == (m - 1) + (nc - 1) * m > 2 ** 31
+    assert out[0] == out[1]
+    assert out[2] > 0
+    # a super-panel wide enough that rows x width reaches 2^31 is still refused
+    assert sim.ps_tall_front_map(nc, m, 32768 + 64, out) == 1
