@@ -89,6 +89,7 @@ SIGNATURES = {
     "amp_factor_device": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.POINTER(FactorInfo)], C.c_int),
     "amp_solve": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "amp_solve_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_solve_device_noalloc": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "amp_refine": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(RefineOpts), C.POINTER(RefineInfo)],
                    C.c_int),
     "amp_refine_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(RefineOpts),

@@ -237,22 +237,23 @@ int ensureVecs(amp_num* num, int32_t nrhs) {
   return AMP_OK;
 }
 
-int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
-  if (!num->ok) return fail(AMP_ERR_STATE, "solve: no successful numeric factorization");
-  int rc = ensureVecs(num, nrhs);
-  if (rc) return rc;
-  amp::ProfileSink* prof = num->profiling ? &num->solveProf : nullptr;
+// The solve with the given scratch (y, fv, pz sized as ensureVecs sizes them). record: keep the per-class times of
+// the handle (host bookkeeping); the allocation-free entry point passes its own scratch and records the total only.
+int runSolveWith(amp_num* num, const double* d_b, double* d_x, int32_t nrhs, void* y, void* fv, void* pz,
+                 bool record) {
+  amp::ProfileSink* prof = record && num->profiling ? &num->solveProf : nullptr;
   if (prof) prof->used = 0;
-  HIP_TRY(hipMemsetAsync(num->fc.err, 0, 4, num->stream));  // the wait-failure flag reports this solve only
+  // the wait-failure flag reports this solve only (cleared by a kernel: no memset node if the caller captures)
+  amp::launchClearFlag(num->fc.err, num->stream);
   HIP_TRY(hipEventRecord(num->ev0, num->stream));
   if (num->prec == AMP_F64)
     HIP_TRY((amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs,
-                                                  num->y.as<double>(), num->fv.as<double>(), num->pz.as<double>(),
-                                                  num->fc, num->stream, prof)));
+                                                  static_cast<double*>(y), static_cast<double*>(fv),
+                                                  static_cast<double*>(pz), num->fc, num->stream, prof)));
   else
     HIP_TRY((amp::launchSolvePlan<float, float>(num->sym->dsp, solveArgs<float>(num), d_b, d_x, nrhs,
-                                                num->y.as<float>(), num->fv.as<float>(), num->pz.as<float>(), num->fc,
-                                                num->stream, prof)));
+                                                static_cast<float*>(y), static_cast<float*>(fv),
+                                                static_cast<float*>(pz), num->fc, num->stream, prof)));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(num->ev1, num->stream));
   int32_t waitErr = 0;
@@ -262,6 +263,17 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
   num->solveMs = ms;
+  return AMP_OK;
+}
+
+int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
+  if (!num->ok) return fail(AMP_ERR_STATE, "solve: no successful numeric factorization");
+  int rc = ensureVecs(num, nrhs);
+  if (rc) return rc;
+  rc = runSolveWith(num, d_b, d_x, nrhs, num->y.p, num->fv.p, num->pz.p, true);
+  if (rc) return rc;
+  const float ms = num->solveMs;
+  amp::ProfileSink* prof = num->profiling ? &num->solveProf : nullptr;
   for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
     it = it->first.rfind("solve_", 0) == 0 ? num->kernelMs.erase(it) : std::next(it);
   auto& total = num->kernelMs["solve_total"];  // recorded with or without profiling
@@ -288,6 +300,22 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
 
 namespace amp {
 int solveOnDevice(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) { return runSolve(num, d_b, d_x, nrhs); }
+
+SolveScratch solveScratchNeed(const amp_num* num, int32_t nrhs) {
+  const amp_sym* h = num->sym;
+  const size_t es = num->prec == AMP_F64 ? 8 : 4;
+  const size_t n = size_t(std::max(h->S.n, 1)) * nrhs;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  SolveScratch s;
+  s.yBytes = up(std::max(n * es, size_t(h->S.n) * 8));
+  s.fvBytes = up(size_t(std::max<int64_t>(h->SP.fvEntries, 1)) * nrhs * es);
+  s.pzBytes = up(size_t(std::max<int64_t>(h->SP.initSlots, 1)) * kTileEdge * std::min(nrhs, kFlowRhs) * es);
+  return s;
+}
+
+int solveWithScratch(amp_num* num, const double* d_b, double* d_x, int32_t nrhs, void* y, void* fv, void* pz) {
+  return runSolveWith(num, d_b, d_x, nrhs, y, fv, pz, false);
+}
 }  // namespace amp
 
 extern "C" {
@@ -738,7 +766,7 @@ int amp_graph_create(amp_num* num, const double* d_values, const double* d_b, do
 int amp_graph_launch(amp_graph* g) {
   if (!g) return fail(AMP_ERR_ARG, "amp_graph_launch: bad arguments");
   HIP_TRY(amp::afterCaller(g->num->stream, g->num->inEv));
-  HIP_TRY(hipMemsetAsync(g->num->fc.err, 0, 4, g->num->stream));  // the wait-failure flag reports this replay only
+  amp::launchClearFlag(g->num->fc.err, g->num->stream);  // the wait-failure flag reports this replay only
   HIP_TRY(hipEventRecord(g->e0, g->num->stream));
   HIP_TRY(hipGraphLaunch(g->exec, g->num->stream));
   HIP_TRY(hipEventRecord(g->e1, g->num->stream));

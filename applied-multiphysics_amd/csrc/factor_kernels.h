@@ -85,6 +85,9 @@ struct FlowCtl {
 };
 constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 values)
 
+// Clear one int32 device flag with a kernel (a kernel node, not a memset node, when the caller captures the stream).
+void launchClearFlag(int32_t* flag, hipStream_t st);
+
 // Solve q systems in place on the device: b (q vectors of n, original order) -> x. y (n q), fv (fvEntries q) and pz
 // (initSlots x 64 x min(q, kFlowRhs)) are scratch of the working precision S; Lx / W / D are the factor's precision
 // T. Raises *fc.epoch once per pass. Enqueues kernels only (no host synchronisation), so it can be captured.

@@ -67,6 +67,15 @@ hipError_t upload(DevBuf& d, const std::vector<T>& v) {
 
 int solveOnDevice(amp_num* num, const double* d_b, double* d_x, int32_t nrhs);
 
+// Bytes of the solve scratch (working-precision y, front vectors fv, backward partial products pz, each rounded up to
+// 256 bytes) that nrhs right-hand sides need, and the solve with scratch supplied by the caller (no allocation).
+struct SolveScratch {
+  size_t yBytes = 0, fvBytes = 0, pzBytes = 0;
+  size_t total() const { return yBytes + fvBytes + pzBytes; }
+};
+SolveScratch solveScratchNeed(const amp_num* num, int32_t nrhs);
+int solveWithScratch(amp_num* num, const double* d_b, double* d_x, int32_t nrhs, void* y, void* fv, void* pz);
+
 }  // namespace amp
 
 struct amp_sym {

@@ -11,6 +11,10 @@
 
 namespace amp {
 
+__global__ void k_clear_flag(int32_t* flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
 // Raise the pass number of the tiled-supernode solve flags (wrapping to 1; flags start at 0) and clear the task
 // counters of the pass's flow launches.
 __global__ void k_epoch_bump(int32_t* epoch, int32_t* ctr, int32_t nctr) {
@@ -891,5 +895,7 @@ template hipError_t launchSolvePlan<float, float>(const DeviceSolvePlan&, const 
                                             int32_t, float*, float*, float*, const FlowCtl&, hipStream_t, ProfileSink*);
 template hipError_t launchSolvePlan<float, double>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
                                              int32_t, double*, double*, double*, const FlowCtl&, hipStream_t, ProfileSink*);
+
+void launchClearFlag(int32_t* flag, hipStream_t st) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, flag); }
 
 }  // namespace amp

@@ -126,6 +126,15 @@ int amp_factor_device(amp_num *num, const double *d_values, int64_t nvalues, dou
 int amp_solve(amp_num *num, const double *b, double *x, int32_t nrhs);
 int amp_solve_device(amp_num *num, const double *d_b, double *d_x, int32_t nrhs);
 
+/* Allocation-free variant of amp_solve_device (no counterpart in the reference): for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). Same kernels,
+ * arguments and results as amp_solve_device; the solve's scratch comes from a fixed array of
+ * AMP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of the handle's growable buffers, and the
+ * call allocates nothing. Requests needing more scratch are refused with AMP_ERR_ARG. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (from several threads or handles) race. */
+#define AMP_NOALLOC_SCRATCH_BYTES (64 << 20)
+int amp_solve_device_noalloc(amp_num *num, const double *d_b, double *d_x, int32_t nrhs);
+
 /* ---- refinement (device, FP64 residual and update) ---------------------------------------------------------- */
 
 #define AMP_REFINE_IR 0    /* solveRefined (ldlt.mjs:253-275): x = solve(b); x += solve(b - A x) */
