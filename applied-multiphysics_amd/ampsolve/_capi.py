@@ -98,6 +98,8 @@ SIGNATURES = {
     "amp_download_factor": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amp_sym_autotune": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneInfo)], C.c_int),
     "amp_sym_set_params": ([C.c_void_p, C.POINTER(TuneInfo)], C.c_int),
+    "amp_shift_invert_apply": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_shift_invert_apply_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "amp_pcg_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts), C.POINTER(PcgInfo)],
                        C.c_int),
     "amp_mlp_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),

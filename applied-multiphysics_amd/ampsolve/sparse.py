@@ -256,6 +256,19 @@ class NumericFactor:
         return x, {"steps": info.steps, "converged": bool(info.converged), "relResidual": info.relResidual,
                    "ms": info.ms}
 
+    def shift_invert(self, M, X, nvec):
+        """Y = A^-1 (M X) for nvec vectors (eigen.ts:156-166): M on the analysed pattern, A the factored matrix.
+        Host arrays (numpy) or device arrays; returns Y of X's kind and layout."""
+        if hasattr(X, "data_ptr") and getattr(X, "is_cuda", False):
+            Y = X.new_empty(X.shape)
+            A.check(A.lib().amp_shift_invert_apply_device(self._h, A.ptr(M), A.ptr(X), A.ptr(Y), int(nvec)))
+            return Y
+        Mv = np.ascontiguousarray(M, dtype=np.float64)
+        Xv = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.empty_like(Xv)
+        A.check(A.lib().amp_shift_invert_apply(self._h, A.ptr(Mv), A.ptr(Xv), A.ptr(Y), int(nvec)))
+        return Y
+
     def download(self):
         """(Lx, D) in the reference's layout (D is None for Cholesky)."""
         Lx = np.zeros(max(self.sym.nnzLStored, 1))

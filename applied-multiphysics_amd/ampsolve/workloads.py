@@ -181,12 +181,10 @@ def heat3d(ncell=60, seed=1, rhs_seed=2):
 _TET_EDGES = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]])
 
 
-def maxwell_edge(ncell=20, sigma=50.0, seed=3, rhs_seed=4, eps_choices=(1.0, 2.1, 11.7)):
-    """Config: shift-inverted 3D Maxwell, lowest-order Nedelec (Whitney) edge elements on the unit cube, ncell^3 hexes
-    x 6 Kuhn tets, PEC walls eliminated, per-element eps_r drawn uniformly from {1.0, 2.1, 11.7}, mu_r = 1; the matrix
-    is K - sigma M (symmetric indefinite). Element matrices: curl w_ij = 2 grad(l_i) x grad(l_j),
-    int w_a . w_b from int l_p l_q = |T| (1 + delta_pq) / 20 (as fem/nedelec1.mjs assembles them). Edges are oriented
-    from the lower to the higher node index. Returns (pat, values, b)."""
+def _maxwell_elements(ncell, seed, eps_choices):
+    """Whitney edge-element matrices on the unit cube (ncell^3 hexes x 6 Kuhn tets): per-tet curl-curl Ke, unit mass
+    Me, eps_r per tet, the signs of the global edge orientation (lower to higher node index) per local pair, the tets'
+    global edge ids, and the mask of edges on the boundary (PEC)."""
     nodes, tets = box_tet_mesh(ncell, ncell, ncell)
     X = nodes[tets]  # (ne, 4, 3)
     J = np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))
@@ -205,7 +203,6 @@ def maxwell_edge(ncell=20, sigma=50.0, seed=3, rhs_seed=4, eps_choices=(1.0, 2.1
     Me = (vol / 20.0)[:, None, None] * (
         d(i_, i_) * GG[:, j_][:, :, j_] - d(i_, j_) * GG[:, j_][:, :, i_]
         - d(j_, i_) * GG[:, i_][:, :, j_] + d(j_, j_) * GG[:, i_][:, :, i_])
-    Ae = Ke - sigma * eps[:, None, None] * Me
     a, b = tets[:, I], tets[:, Jn]  # local edge endpoints (global node ids)
     lo, hi = np.minimum(a, b), np.maximum(a, b)
     sgn = np.where(a < b, 1.0, -1.0)
@@ -213,16 +210,36 @@ def maxwell_edge(ncell=20, sigma=50.0, seed=3, rhs_seed=4, eps_choices=(1.0, 2.1
     keys = lo * nn + hi
     ukeys, eid = np.unique(keys, return_inverse=True)
     eid = eid.reshape(ne, 6)
-    Ae = Ae * sgn[:, :, None] * sgn[:, None, :]
-    A = assemble(len(ukeys), eid, Ae)
+    ss = sgn[:, :, None] * sgn[:, None, :]
     e0, e1 = ukeys // nn, ukeys % nn
     on = np.zeros(len(ukeys), dtype=bool)
     for ax in range(3):
         for v in (0.0, 1.0):
             on |= (np.abs(nodes[e0, ax] - v) < 1e-12) & (np.abs(nodes[e1, ax] - v) < 1e-12)
+    return Ke, Me, eps, ss, eid, len(ukeys), on
+
+
+def maxwell_edge(ncell=20, sigma=50.0, seed=3, rhs_seed=4, eps_choices=(1.0, 2.1, 11.7)):
+    """Config: shift-inverted 3D Maxwell, lowest-order Nedelec (Whitney) edge elements on the unit cube, ncell^3 hexes
+    x 6 Kuhn tets, PEC walls eliminated, per-element eps_r drawn uniformly from {1.0, 2.1, 11.7}, mu_r = 1; the matrix
+    is K - sigma M (symmetric indefinite). Element matrices: curl w_ij = 2 grad(l_i) x grad(l_j),
+    int w_a . w_b from int l_p l_q = |T| (1 + delta_pq) / 20 (as fem/nedelec1.mjs assembles them). Edges are oriented
+    from the lower to the higher node index. Returns (pat, values, b)."""
+    Ke, Me, eps, ss, eid, nedge, on = _maxwell_elements(ncell, seed, eps_choices)
+    A = assemble(nedge, eid, (Ke - sigma * eps[:, None, None] * Me) * ss)
     R, _ = restrict(A, ~on)
     b = lcg_normal(rhs_seed, R.shape[0])
     return csr_dict(R), R.data.astype(np.float64), b
+
+
+def maxwell_km(ncell=8, seed=3, eps_choices=(1.0,)):
+    """The curl-curl stiffness K and the eps_r-weighted mass M of maxwell_edge on one pattern (PEC eliminated), for
+    the eigenproblem K x = k^2 M x. Returns (pat, K values, M values)."""
+    Ke, Me, eps, ss, eid, nedge, on = _maxwell_elements(ncell, seed, eps_choices)
+    K, _ = restrict(assemble(nedge, eid, Ke * ss), ~on)
+    M, _ = restrict(assemble(nedge, eid, eps[:, None, None] * Me * ss), ~on)
+    assert np.array_equal(K.indptr, M.indptr) and np.array_equal(K.indices, M.indices)
+    return csr_dict(K), K.data.astype(np.float64), M.data.astype(np.float64)
 
 
 def heat2d(nx, kel, Dirichlet=True):

@@ -105,6 +105,7 @@ struct amp_num {
   amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv, pz;
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
+  amp::DevBuf opw;  // shift-invert operator: the products M X (n x nvec, FP64)
   // tiled-supernode solve synchronisation: forward and backward block flags, the wait-failure flag
   amp::DevBuf flow, boxes;
   amp::FlowCtl fc{};

@@ -211,6 +211,14 @@ int amp_graph_launch(amp_graph *g);
 int amp_graph_sync(amp_graph *g, amp_factor_info *info);
 void amp_graph_free(amp_graph *g);
 
+/* Shift-invert operator of the eigen driver (shiftInvertEigen's `op`, eigen.ts:156-166, applied by blockArnoldi,
+ * arnoldi.mjs:39-157): Y = A^-1 (M X) for nvec vectors stored one after another, with M given by its values on the
+ * analysed CSR pattern and A = K - sigma M factored in num (Cholesky or LDL^T, FP64 or FP32). The products with M are
+ * FP64; the nvec solves are one multi-right-hand-side pass. The pattern must be known (amp_sym_set_pattern for an
+ * imported analysis). */
+int amp_shift_invert_apply(amp_num *num, const double *M, const double *X, double *Y, int32_t nvec);
+int amp_shift_invert_apply_device(amp_num *num, const double *d_M, const double *d_X, double *d_Y, int32_t nvec);
+
 /* Preconditioned conjugate gradients on the device (no counterpart in the reference; the parametric sweep's solver):
  * A x = b with A given by d_values on the analysed CSR pattern, the preconditioner being the factor held by `num`
  * (typically of a nearby matrix), starting from the guess in d_x; stops when |b - A x| / |b| <= tol or after maxIter
