@@ -17,6 +17,7 @@ AMP_CHOLESKY = 0
 AMP_LDLT = 1
 AMP_F64 = 0
 AMP_F32 = 1
+AMP_F32_BF16X3 = 2
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -8,8 +8,9 @@ jwt625/applied-multiphysics (declarations in sparse.d.mts:28-57 and ldlt.d.mts:4
   SparseCholesky(pat, perm, sym=..., factor=...)        sparse.mjs:1312-1372
   SupernodalLDLT(sym, factor=...)                       ldlt.mjs:417-645
   setDefaultRealFactor(kind)                            front-real.mjs:196-200
-The factor kinds are 'hip' (FP64 on the matrix cores) and 'hip-f32' (FP32 factor; use with refinement). There is no
-host fallback: without libampsolve.so or a GPU every numeric call raises.
+The factor kinds are 'hip' (FP64 on the matrix cores), 'hip-f32' (FP32 factor; use with refinement) and 'hip-bf16x3'
+(FP32 factor whose bulk trailing updates run as three bf16 products; use with refinement). There is no host fallback:
+without libampsolve.so or a GPU every numeric call raises.
 A CSR pattern is a dict {'n', 'rowptr', 'colidx', 'nnz'} as the reference's CsrPattern (types.d.mts:5-10).
 """
 import ctypes as C
@@ -19,7 +20,8 @@ import numpy as np
 
 from . import _capi as A
 
-REAL_FACTOR_KINDS = ("hip", "hip-f32")
+REAL_FACTOR_KINDS = ("hip", "hip-f32", "hip-bf16x3")
+PRECISION_OF_KIND = {"hip": A.AMP_F64, "hip-f32": A.AMP_F32, "hip-bf16x3": A.AMP_F32_BF16X3}
 DEFAULT_REAL_FACTOR = "hip"
 
 
@@ -95,7 +97,7 @@ class Symbolic:
         """Time the factor and one solve of `values` (a device tensor on the pattern) on the device for candidate
         shape thresholds and keep the fastest plan (amp_sym_autotune). Call before creating numeric factors."""
         t = A.TuneInfo()
-        prec = A.AMP_F32 if kind == "hip-f32" else A.AMP_F64
+        prec = PRECISION_OF_KIND[kind]
         A.check(A.lib().amp_sym_autotune(self._h, A.ptr(values), int(mode), prec, C.byref(t)))
         self._refresh()
         return {f: getattr(t, f) for f, _ in A.TuneInfo._fields_}
@@ -195,7 +197,7 @@ class NumericFactor:
         self.sym = sym
         self.mode = mode
         self.kernel = kind
-        self.precision = A.AMP_F32 if kind == "hip-f32" else A.AMP_F64
+        self.precision = PRECISION_OF_KIND[kind]
         h = C.c_void_p()
         A.check(A.lib().amp_num_create(sym._h, mode, self.precision, C.byref(h)))
         self._h = h

@@ -121,6 +121,7 @@ amp::FactorArgs<T> factorArgs(amp_num* num) {
   a.amap = h->amap.as<int32_t>();
   a.ldlt = num->mode == AMP_LDLT;
   a.tau = 0;
+  a.bulkBf16 = num->prec == AMP_F32_BF16X3;
   return a;
 }
 
@@ -513,7 +514,8 @@ int amp_sym_get_arrays(const amp_sym* h, amp_sym_arrays* a) {
 void amp_sym_free(amp_sym* h) { delete h; }
 
 int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
-  if (!h || !out || (mode != AMP_CHOLESKY && mode != AMP_LDLT) || (precision != AMP_F64 && precision != AMP_F32))
+  if (!h || !out || (mode != AMP_CHOLESKY && mode != AMP_LDLT) ||
+      (precision != AMP_F64 && precision != AMP_F32 && precision != AMP_F32_BF16X3))
     return fail(AMP_ERR_ARG, "amp_num_create: bad arguments");
   *out = nullptr;
   int ndev = 0;
@@ -849,7 +851,8 @@ int amp_sym_set_params(amp_sym* h, const amp_tune_info* t) {
 }
 
 int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t precision, amp_tune_info* info) {
-  if (!h || !d_values || (mode != AMP_CHOLESKY && mode != AMP_LDLT) || (precision != AMP_F64 && precision != AMP_F32))
+  if (!h || !d_values || (mode != AMP_CHOLESKY && mode != AMP_LDLT) ||
+      (precision != AMP_F64 && precision != AMP_F32 && precision != AMP_F32_BF16X3))
     return fail(AMP_ERR_ARG, "amp_sym_autotune: bad arguments");
   if (h->liveNums) return fail(AMP_ERR_STATE, "amp_sym_autotune: numeric handles of this analysis exist");
   if (h->S.n == 0) return fail(AMP_ERR_ARG, "amp_sym_autotune: empty system");

@@ -85,6 +85,11 @@ struct FlowCtl {
 };
 constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 values)
 
+// Bulk trailing-update launch of an FP32 factor on the bf16 matrix cores (three bf16 products per FP32 product; modes 0
+// and 1 of GemmTask, tiles of kGemmEdge).
+void launchGemmBf16x3(const GemmTask* tasks, unsigned grid, float* Lx, float* U, float* W, const float* D, bool ldlt,
+                      hipStream_t st);
+
 // Clear one int32 device flag with a kernel (a kernel node, not a memset node, when the caller captures the stream).
 void launchClearFlag(int32_t* flag, hipStream_t st);
 

@@ -540,6 +540,13 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         break;
       case kGemm: {
         const GemmTask* gt = reinterpret_cast<const GemmTask*>(dp.gemm) + L.first;
+        if constexpr (std::is_same<T, float>::value) {
+          // bulk launches (side stream: modes 0 / 1 only) of a factor with bf16 bulk updates
+          if (a.bulkBf16 && L.edge == kGemmEdge && (L.sync & kSyncSide)) {
+            launchGemmBf16x3(gt, grid, a.Lx, a.U, a.W, a.D, a.ldlt != 0, st);
+            break;
+          }
+        }
         if (L.edge == kGemmEdge) {
           if (a.ldlt)
             hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge, true>), dim3(grid), dim3(512), 0, st, gt, a.Lx, a.U, a.W, a.D);

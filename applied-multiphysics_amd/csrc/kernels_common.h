@@ -74,6 +74,7 @@ struct FactorArgs {
   const int32_t* amap;
   int32_t ldlt;
   double tau;
+  int32_t bulkBf16;  // FP32 factor: bulk trailing updates as three bf16 products (k_gemm_bf16x3)
 };
 
 }  // namespace amp

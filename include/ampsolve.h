@@ -31,6 +31,8 @@ extern "C" {
 
 #define AMP_F64 0 /* factor, update matrices and solves in FP64 (MFMA f64) */
 #define AMP_F32 1 /* factor and update matrices in FP32 (MFMA f32); solves in FP32 */
+#define AMP_F32_BF16X3 2 /* as AMP_F32, with the bulk trailing updates of large fronts as three bf16 products per FP32
+                          * product (bf16 matrix cores, FP32 accumulation; relative error about 2^-16 per product) */
 
 typedef struct amp_sym amp_sym; /* symbolic analysis + execution plan (host), device copy of the plan */
 typedef struct amp_num amp_num; /* numeric factor resident in HBM */

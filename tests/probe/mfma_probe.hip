@@ -17,6 +17,34 @@ __global__ void k_probe(const double* A, const double* B, double* raw64, float* 
   }
 }
 
+// bf16 16x16x32: lane l supplies A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8 (A 16 x 32, B 32 x 16,
+// small integers, exact in bf16); raw[4 l + r] as it comes out
+typedef __bf16 bf16x8 __attribute__((__vector_size__(16)));
+__global__ void k_probe_bf16(const float* A, const float* B, float* raw) {
+  const int l = threadIdx.x;
+  bf16x8 a, b;
+  for (int j = 0; j < 8; j++) {
+    a[j] = (__bf16)A[(l % 16) * 32 + 8 * (l / 16) + j];
+    b[j] = (__bf16)B[(8 * (l / 16) + j) * 16 + l % 16];
+  }
+  v4f d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, v4f{0, 0, 0, 0}, 0, 0, 0);
+  for (int r = 0; r < 4; r++) raw[4 * l + r] = d[r];
+}
+
+extern "C" int probe_mfma_bf16(const float* A, const float* B, float* raw) {
+  float *dA, *dB, *dr;
+  if (hipMalloc(&dA, 512 * 4) || hipMalloc(&dB, 512 * 4) || hipMalloc(&dr, 256 * 4)) return -1;
+  (void)hipMemcpy(dA, A, 512 * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dB, B, 512 * 4, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_probe_bf16, dim3(1), dim3(64), 0, 0, dA, dB, dr);
+  hipError_t e = hipDeviceSynchronize();
+  (void)hipMemcpy(raw, dr, 256 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dr);
+  return e == hipSuccess ? 0 : -2;
+}
+
 extern "C" int probe_mfma(const double* A, const double* B, double* raw64, float* raw32) {
   double *dA, *dB, *d64;
   float* d32;

@@ -1,5 +1,6 @@
 """The matrix-core lane layouts the tile kernels rely on (kernels_common.h: mfmaRow): FP64 component r of lane l is
-row 4 r + l / 16, FP32 component r is row 4 (l / 16) + r; column l % 16 for both."""
+row 4 r + l / 16, FP32 component r is row 4 (l / 16) + r; column l % 16 for both. The bf16 16x16x32 product (lane l
+supplies A[l % 16][8 (l / 16) + j], B[8 (l / 16) + j][l % 16]) leaves row 4 (l / 16) + r, column l % 16."""
 import ctypes as C
 import os
 
@@ -26,3 +27,18 @@ def test_mfma_16x16x4_layouts():
             col = l % 16
             assert raw64[4 * l + r] == D[4 * r + l // 16, col], ("f64", l, r)
             assert raw32[4 * l + r] == D[4 * (l // 16) + r, col], ("f32", l, r)
+
+
+def test_mfma_16x16x32_bf16_layout():
+    lib = C.CDLL(LIB)
+    rng = np.random.default_rng(1)
+    A = rng.integers(-4, 5, size=(16, 32)).astype(np.float32)
+    B = rng.integers(-4, 5, size=(32, 16)).astype(np.float32)
+    raw = np.zeros(256, dtype=np.float32)
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C")
+    lib.probe_mfma_bf16.argtypes = [f32p, f32p, f32p]
+    assert lib.probe_mfma_bf16(np.ascontiguousarray(A), np.ascontiguousarray(B), raw) == 0
+    D = A.astype(np.float64) @ B.astype(np.float64)
+    for l in range(64):
+        for r in range(4):
+            assert raw[4 * l + r] == D[4 * (l // 16) + r, l % 16], ("bf16", l, r)
