@@ -103,6 +103,8 @@ SIGNATURES = {
     "amp_shift_invert_apply_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "amp_pcg_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgOpts), C.POINTER(PcgInfo)],
                        C.c_int),
+    "amp_pcg_batch_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                              C.POINTER(PcgOpts), C.POINTER(PcgInfo)], C.c_int),
     "amp_mlp_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
     "amp_mlp_param_count": ([C.c_void_p], C.c_int64),
     "amp_mlp_set_params": ([C.c_void_p, f32p, C.c_int64], C.c_int),
@@ -121,6 +123,14 @@ SIGNATURES = {
     "amp_num_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "amp_num_get_kernel_ms": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
     "amp_num_free": ([C.c_void_p], None),
+    "amp_asm_create": ([C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
+                       C.c_int),
+    "amp_asm_get_pattern": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_int32)),
+                            C.POINTER(C.POINTER(C.c_int32))], C.c_int),
+    "amp_assemble_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_asm_set_modes": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "amp_param_coefficients_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "amp_asm_free": ([C.c_void_p], None),
     "amp_last_error": ([], C.c_char_p),
 }
 

@@ -2,15 +2,22 @@
 
 The parametric system is 3D P1 heat on the unit cube (ncell^3 hexes x 6 Kuhn tets, Dirichlet z = 0) with conductivity
 k(c) = exp(sum_i theta_i phi_i(c)) at the tet centroid c, phi_i separable cosine modes. The sweep:
-  1. solves training points theta with the direct FP64 factor (the data);
-  2. trains the warm-start network theta -> solution on all nodes (WarmStartNet: hand-written device kernels,
+  1. assembles the values of every parameter point on the device (amp_param_coefficients_device, then
+     amp_assemble_device: assembleCells / elementSlots / restrict of the P1 operator);
+  2. solves the training points with the direct solver's mixed path (FP32 factor + FP64 iterative refinement to 1e-12)
+     and keeps every training factor resident in HBM (workspace released, solve storage only);
+  3. trains the warm-start network theta -> solution on all nodes (WarmStartNet: hand-written device kernels,
      amp_mlp_*);
-  3. keeps the FP32 factor of every solved training point resident in HBM (workspace released, solve storage only);
-  4. for a new theta: x0 = the network's prediction, then preconditioned CG (amp_pcg_device) with the cached factor
-     of the nearest solved point to |b - A x| / |b| <= tol within a fixed iteration budget, else a fresh factor (the
-     fallback).
-The training solves are the direct solver's mixed path (FP32 factor + FP64 iterative refinement to 1e-12), whose
-factors are the ones cached.
+  4. solves new points one by one (group = 1) or in groups sharing a factor: the network's predictions are the
+     initial guesses, a cached factor preconditions CG (amp_pcg_batch_device: one multi-right-hand-side solve per
+     iteration for all the group's systems) to |b - A x| / |b| <= tol within a fixed iteration budget, and a point
+     that misses it gets a fresh factor (the fallback).
+The cached factor of a point (or group) is the one whose conductivity field is closest in the sense that bounds the
+preconditioned condition number: for a factor of A(theta_c) preconditioning A(theta), the eigenvalues lie in
+[exp(min_e d_e), exp(max_e d_e)] with d_e = Phi_e (theta - theta_c), so the factor minimising the spread
+max_e d_e - min_e d_e (the largest over a group's points) is taken, among the Euclidean-nearest candidates of each
+point, on a sample of the modes' values. In eight parameters the training points are far apart, so a factor shared by
+a group preconditions each member much worse than its own best one: one point per solve is the default.
 """
 import ctypes as C
 import time
@@ -24,6 +31,11 @@ from .device import DeviceArray, synchronize
 from .sparse import NumericFactor, analyze
 
 MODES = np.array([(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1), (2, 0, 0)])
+
+
+def mode_values(points):
+    """phi_i at the given points (..., 3) -> (..., 8)."""
+    return np.prod(np.cos(np.pi * MODES * points[..., None, :]), axis=-1)
 
 
 class ParamHeat3D:
@@ -56,6 +68,48 @@ class ParamHeat3D:
         self.pat = {"n": n, "rowptr": rowptr.astype(np.int32), "colidx": cols.astype(np.int32), "nnz": len(ukeys)}
         self.Cmat = sp.csr_matrix((coef, (slot, el)), shape=(len(ukeys), ne))
         self.b = 2.0 * W.lcg_uniform(rhs_seed, n) - 1.0
+        self.nodes, self.tets = nodes, tets
+        self._asm = None
+
+    def device_assembly(self):
+        """The device assembly of this system (amp_asm, created on first use): restricted pattern, modes set."""
+        if self._asm is None:
+            h = C.c_void_p()
+            nodes = np.ascontiguousarray(self.nodes, np.float64)
+            tets = np.ascontiguousarray(self.tets, np.int32)
+            free = np.ascontiguousarray(self.free, np.uint8)
+            A.check(A.lib().amp_asm_create(3, len(nodes), nodes.ctypes.data, len(tets), tets.ctypes.data,
+                                           free.ctypes.data, C.byref(h)))
+            self._asm = h
+            phi = np.ascontiguousarray(self.Phi, np.float64)
+            A.check(A.lib().amp_asm_set_modes(h, phi.shape[1], phi.ctypes.data))
+            n, nnz = C.c_int32(), C.c_int64()
+            rp, ci = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+            A.check(A.lib().amp_asm_get_pattern(h, C.byref(n), C.byref(nnz), C.byref(rp), C.byref(ci)))
+            self.asm_rowptr = np.ctypeslib.as_array(rp, shape=(n.value + 1,)).copy()
+            self.asm_colidx = np.ctypeslib.as_array(ci, shape=(nnz.value,)).copy()
+        return self._asm
+
+    def device_values(self, thetas):
+        """Values of the points' matrices on the device, (len(thetas), nnz): k_e = exp(Phi_e theta) and the assembly,
+        both on the device."""
+        h = self.device_assembly()
+        th = np.ascontiguousarray(np.atleast_2d(thetas), np.float64)
+        nb = th.shape[0]
+        dth = DeviceArray.from_numpy(th)
+        dk = DeviceArray((nb, len(self.tets)))
+        A.check(A.lib().amp_param_coefficients_device(h, A.ptr(dth), A.ptr(dk), nb))
+        dv = DeviceArray((nb, self.pat["nnz"]))
+        A.check(A.lib().amp_assemble_device(h, A.ptr(dk), A.ptr(dv), nb))
+        return dv
+
+    def __del__(self):
+        h = getattr(self, "_asm", None)
+        if h is not None and h.value:
+            try:
+                A.lib().amp_asm_free(h)
+            except Exception:
+                pass
 
     def values(self, theta):
         return self.Cmat @ np.exp(self.Phi @ np.asarray(theta, np.float64))
@@ -147,41 +201,59 @@ def pcg(num, values, b, x, tol=1e-10, maxIter=50):
             "initialResidual": info.initialResidual, "ms": info.ms}
 
 
+def pcg_batch(num, values, b, x, nb, tol=1e-10, maxIter=50, shared_b=True):
+    """amp_pcg_batch_device: nb systems (values nb x nnz, x nb x n, device) preconditioned together by `num`."""
+    o = A.PcgOpts()
+    o.maxIter, o.tol = int(maxIter), float(tol)
+    infos = (A.PcgInfo * nb)()
+    stride = 0 if shared_b else num.sym.n
+    A.check(A.lib().amp_pcg_batch_device(num._h, A.ptr(values), A.ptr(b), stride, A.ptr(x), int(nb), C.byref(o),
+                                         infos))
+    return [{"iters": i.iters, "converged": bool(i.converged), "relResidual": i.relResidual,
+             "initialResidual": i.initialResidual, "ms": i.ms} for i in infos]
+
+
 class LearnedSweep:
     """The sweep of the module docstring on one device."""
 
-    def __init__(self, ncell=30, hidden=256, maxBatch=512):
+    def __init__(self, ncell=30, hidden=256, maxBatch=512, group=1, candidates=16):
         self.prob = ParamHeat3D(ncell)
         self.sym = analyze(self.prob.pat)
         self.db = DeviceArray.from_numpy(self.prob.b)
         self.net = WarmStartNet(MODES.shape[0], hidden, self.prob.nnodes, maxBatch)
         self.direct = NumericFactor(self.sym, 0, "hip")
+        self.group, self.candidates = group, candidates
+        g = np.linspace(0.0, 1.0, 9)  # sample of the modes' values for the spread of d_e (corners and edges included)
+        self.phi_sample = mode_values(np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3))
 
     def device_values(self, theta):
-        return DeviceArray.from_numpy(self.prob.values(theta))
+        return self.prob.device_values(theta)[0]
 
-    def solve_direct(self, theta):
-        """Fresh FP64 factor + solve (the refactoring method): (x, device ms of factor + solve)."""
-        dv = self.device_values(theta)
+    def solve_direct(self, theta, dv=None):
+        """Fresh FP64 factor + solve (judging the learned solutions): (x, device ms of factor + solve)."""
+        dv = self.device_values(theta) if dv is None else dv
         assert self.direct.factor(dv)
         fms = self.direct.info["ms"]
         x = self.direct.solve(self.db)
         return x, fms + self.direct.kernel_ms("solve_total")[0]
 
-    def fit(self, thetas, epochs=150, batch=64, lr=3e-3, seed=5):
+    def fit(self, thetas, epochs=150, batch=64, lr=3e-3, seed=5, chunk=64):
         """Training solves (FP32 factor + FP64 refinement, factors cached), network training. Returns statistics."""
         t0 = time.perf_counter()
-        self.cache_theta = np.asarray(thetas, np.float64)
+        thetas = np.asarray(thetas, np.float64)
+        self.cache_theta = thetas
         self.cache, xs = [], []
-        for th in thetas:
-            dv = self.device_values(th)
-            num = NumericFactor(self.sym, 0, "hip-f32")
-            assert num.factor(dv)
-            x, info = num.refine(dv, self.db, "ir", maxSteps=20, tol=1e-12)
-            assert info["converged"], info
-            num.release_workspace()
-            self.cache.append(num)
-            xs.append(x.numpy())
+        for c0 in range(0, len(thetas), chunk):
+            dvals = self.prob.device_values(thetas[c0:c0 + chunk])
+            for i in range(dvals.shape[0]):
+                dv = dvals[i]
+                num = NumericFactor(self.sym, 0, "hip-f32")
+                assert num.factor(dv)
+                x, info = num.refine(dv, self.db, "ir", maxSteps=20, tol=1e-12)
+                assert info["converged"], info
+                num.release_workspace()
+                self.cache.append(num)
+                xs.append(x.numpy())
         synchronize()
         t_data = time.perf_counter() - t0
         full = self.prob.to_nodes(np.stack(xs))  # (N, nnodes) FP64
@@ -206,20 +278,69 @@ class LearnedSweep:
         full = Y.numpy().astype(np.float64) * self.sigma + self.mu
         return DeviceArray.from_numpy(full[:, self.prob.free])
 
+    def spread(self, thetas, idx):
+        """Largest spread max_e d_e - min_e d_e over the points `thetas` (g x 8) for each cached factor in idx."""
+        d = (thetas[:, None, :] - self.cache_theta[idx][None, :, :]) @ self.phi_sample.T  # (g, c, samples)
+        return (d.max(axis=2) - d.min(axis=2)).max(axis=0)
+
     def nearest(self, theta):
         return int(np.argmin(np.linalg.norm(self.cache_theta - np.asarray(theta), axis=1)))
 
-    def solve(self, theta, x0, tol=1e-10, maxIter=100):
-        """PCG from x0 preconditioned by the cached factor of the nearest solved point; a fresh FP64 factor when it
-        misses tol within maxIter. Returns (x, info)."""
-        dv = self.device_values(theta)
-        a = self.nearest(theta)
-        x = x0.clone()
-        info = pcg(self.cache[a], dv, self.db, x, tol, maxIter)
-        info["nearest"] = a
-        info["fallback"] = False
-        if not info["converged"]:
-            x, ms = self.solve_direct(theta)
-            info["fallback"] = True
-            info["ms"] += ms
-        return x, info
+    def choose_factor(self, thetas):
+        """The cached factor for a group of points: the smallest largest spread among each point's Euclidean-nearest
+        candidates."""
+        cand = set()
+        for th in thetas:
+            cand.update(np.argsort(np.linalg.norm(self.cache_theta - th, axis=1))[:self.candidates].tolist())
+        cand = np.array(sorted(cand))
+        return int(cand[np.argmin(self.spread(np.atleast_2d(thetas), cand))])
+
+    def order_points(self, thetas):
+        """A sweep order in which consecutive points are close (greedy nearest neighbour from the first point), so
+        that the groups of consecutive points share a good factor."""
+        thetas = np.asarray(thetas)
+        left = list(range(1, len(thetas)))
+        order = [0]
+        while left:
+            last = thetas[order[-1]]
+            j = int(np.argmin(np.linalg.norm(thetas[left] - last, axis=1)))
+            order.append(left.pop(j))
+        return np.array(order)
+
+    def solve_group(self, thetas, x0=None, tol=1e-10, maxIter=100, factor=None):
+        """Batched PCG for a group of points (their values assembled on the device, one cached factor), from the
+        network's warm starts (or from x0: a (g, n) device array, zeros for a zero guess); a point that misses tol
+        within maxIter gets a fresh factor. Returns (x (g, n) device array, infos, chosen factor)."""
+        thetas = np.atleast_2d(np.asarray(thetas, np.float64))
+        g = len(thetas)
+        a = self.choose_factor(thetas) if factor is None else factor
+        dv = self.prob.device_values(thetas)
+        X = self.predict(thetas) if x0 is None else x0.clone()
+        infos = pcg_batch(self.cache[a], dv, self.db, X, g, tol, maxIter)
+        for i, info in enumerate(infos):
+            info["factor"] = a
+            info["fallback"] = False
+            if not info["converged"]:
+                x, ms = self.solve_direct(thetas[i], dv[i])
+                X[i].copy_(x)
+                info["fallback"] = True
+                info["ms"] += ms
+        return X, infos, a
+
+    def refactor_graph(self, ir_steps=2):
+        """The fastest refactoring path for comparison: a captured FP32 factor + solve + FP64 refinement sequence on one
+        values buffer (amp_graph); returns (run(dv) -> x device array, the handles to keep)."""
+        num = NumericFactor(self.sym, 0, "hip-f32")
+        buf = DeviceArray((self.prob.pat["nnz"],))
+        x = DeviceArray.zeros((self.prob.pat["n"],))
+        buf.copy_(self.device_values(np.zeros(MODES.shape[0])))
+        g = num.graph(buf, self.db, x, 1, ir_steps)
+
+        def run(dv):
+            buf.copy_(dv)
+            g.launch()
+            st = g.sync()
+            assert st["ok"]
+            return x
+
+        return run, (num, g, buf, x)

@@ -278,6 +278,135 @@ int runPcg(amp_num* num, const double* d_values, const double* d_b, double* d_x,
   return AMP_OK;
 }
 
+// Preconditioned CG on nb independent systems A_j x_j = b_j in lockstep, every A_j on the analysed pattern (values of
+// system j at j nnz), one shared preconditioner (the factor held by num): each iteration's preconditioner step is one
+// multi-right-hand-side solve of all systems. A system stops updating (zero step) once it meets tol; the loop ends
+// when all have, or after maxIter iterations. The dot products of all systems are read back together (three per
+// iteration); the true residual of every system is recomputed at exit.
+int runPcgBatch(amp_num* num, const double* d_values, const double* d_b, int64_t bStride, double* d_x, int32_t nb,
+                const amp_pcg_opts* o, amp_pcg_info* info) {
+  amp_sym* h = num->sym;
+  const int32_t n = h->S.n;
+  const int64_t nnz = h->S.nnzPattern;
+  if (!num->ok) return fail(AMP_ERR_STATE, "pcg: no successful numeric factorization");
+  if (o->maxIter < 0) return fail(AMP_ERR_ARG, "pcg: negative iteration count");
+  int rc = patternToDevice(h);
+  if (rc) return rc;
+  const size_t vb = size_t(std::max(n, 1)) * nb * 8;
+  // r, z, q, p (nb vectors each), per-system coefficients (2 nb), reduction scratch
+  if ((rc = ensure(num->r, vb)) || (rc = ensure(num->dvec, vb)) || (rc = ensure(num->w, vb)) ||
+      (rc = ensure(num->V, vb)))
+    return rc;
+  if ((rc = ensure(num->partial, size_t(amp::partialEntries(std::max(nb, amp::kMaxRestart + 2))) * 8))) return rc;
+  if ((rc = ensure(num->small, size_t(std::max(4 * nb, amp::kMaxRestart + 2)) * 8))) return rc;
+  if (!num->evR0) {
+    HIP_TRY(hipEventCreate(&num->evR0));
+    HIP_TRY(hipEventCreate(&num->evR1));
+  }
+  hipStream_t st = num->stream;
+  HIP_TRY(hipEventRecord(num->evR0, st));
+  const int32_t* rp = h->dRowptr.as<int32_t>();
+  const int32_t* ci = h->dColidx.as<int32_t>();
+  double* r = num->r.as<double>();
+  double* z = num->dvec.as<double>();
+  double* q = num->w.as<double>();
+  double* p = num->V.as<double>();
+  double* part = num->partial.as<double>();
+  double* dd = num->small.as<double>();  // [0, nb): reduction results; [nb, 2 nb): a_j; [2 nb, 3 nb): c_j
+  std::vector<double> hv(nb), bn(nb), rn(nb), rz(nb), coef(2 * size_t(nb));
+  std::vector<char> done(nb, 0);
+  auto dots = [&](const double* u, const double* w, std::vector<double>& out) -> int {
+    amp::launchBatchDot(u, w, n, nb, part, dd, st);
+    HIP_TRY(hipMemcpyAsync(out.data(), dd, size_t(nb) * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMP_OK;
+  };
+  auto setCoef = [&](const std::vector<double>& a, const std::vector<double>* c) -> int {
+    for (int32_t j = 0; j < nb; j++) {
+      coef[j] = a[j];
+      coef[nb + j] = c ? (*c)[j] : 1.0;
+    }
+    HIP_TRY(hipMemcpyAsync(dd + nb, coef.data(), 2 * size_t(nb) * 8, hipMemcpyHostToDevice, st));
+    return AMP_OK;
+  };
+  // |b_j|: a shared right-hand side is read through the residual of x = 0 (r = b)
+  amp::launchBatchResidual(rp, ci, d_values, nnz, d_x, d_b, bStride, r, n, nb, st);
+  {
+    // b_j . b_j from r_j + A_j x_j would round differently; copy b_j into q and take its norm
+    if (bStride == 0)
+      for (int32_t j = 0; j < nb; j++)
+        HIP_TRY(hipMemcpyAsync(q + int64_t(j) * n, d_b, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
+    else
+      for (int32_t j = 0; j < nb; j++)
+        HIP_TRY(hipMemcpyAsync(q + int64_t(j) * n, d_b + int64_t(j) * bStride, size_t(n) * 8,
+                               hipMemcpyDeviceToDevice, st));
+    if ((rc = dots(q, q, bn))) return rc;
+  }
+  if ((rc = dots(r, r, rn))) return rc;
+  for (int32_t j = 0; j < nb; j++) {
+    bn[j] = std::sqrt(bn[j]);
+    if (!(bn[j] != 0) || std::isnan(bn[j])) bn[j] = 1;
+    rn[j] = std::sqrt(rn[j]);
+    info[j] = amp_pcg_info{};
+    info[j].initialResidual = rn[j] / bn[j];
+    done[j] = !(rn[j] / bn[j] > o->tol) || o->maxIter == 0;
+  }
+  int32_t it = 0;
+  auto allDone = [&] {
+    for (char d : done)
+      if (!d) return false;
+    return true;
+  };
+  if (!allDone()) {
+    if ((rc = amp::solveOnDevice(num, r, z, nb))) return rc;
+    HIP_TRY(hipMemcpyAsync(p, z, vb, hipMemcpyDeviceToDevice, st));
+    if ((rc = dots(r, z, rz))) return rc;
+    std::vector<double> alpha(nb), negAlpha(nb), beta(nb), pq(nb), rzNew(nb);
+    for (it = 1; it <= o->maxIter && !allDone(); it++) {
+      amp::launchBatchResidual(rp, ci, d_values, nnz, p, nullptr, 0, q, n, nb, st);  // q_j = A_j p_j
+      if ((rc = dots(p, q, pq))) return rc;
+      for (int32_t j = 0; j < nb; j++) {
+        if (!done[j] && !(pq[j] > 0)) done[j] = 1;  // not positive on p_j: stop that system with its iterate
+        alpha[j] = done[j] ? 0.0 : rz[j] / pq[j];
+        negAlpha[j] = -alpha[j];
+        if (!done[j]) info[j].iters = it;
+      }
+      if ((rc = setCoef(alpha, nullptr))) return rc;
+      amp::launchBatchAxpby(dd + nb, p, dd + 2 * nb, d_x, n, nb, st);  // x_j += a_j p_j
+      if ((rc = setCoef(negAlpha, nullptr))) return rc;
+      amp::launchBatchAxpby(dd + nb, q, dd + 2 * nb, r, n, nb, st);  // r_j -= a_j q_j
+      if ((rc = dots(r, r, rn))) return rc;
+      for (int32_t j = 0; j < nb; j++)
+        if (!done[j] && !(std::sqrt(rn[j]) / bn[j] > o->tol)) done[j] = 1;
+      if (allDone()) break;
+      if ((rc = amp::solveOnDevice(num, r, z, nb))) return rc;
+      if ((rc = dots(r, z, rzNew))) return rc;
+      std::vector<double> one(nb, 1.0);
+      for (int32_t j = 0; j < nb; j++) {
+        beta[j] = done[j] ? 1.0 : rzNew[j] / rz[j];
+        if (!done[j]) rz[j] = rzNew[j];
+      }
+      // p_j = z_j + beta_j p_j for the running systems (a converged one keeps its p: it takes zero steps)
+      for (int32_t j = 0; j < nb; j++) one[j] = done[j] ? 0.0 : 1.0;
+      if ((rc = setCoef(one, &beta))) return rc;
+      amp::launchBatchAxpby(dd + nb, z, dd + 2 * nb, p, n, nb, st);
+    }
+  }
+  // true residuals at exit
+  amp::launchBatchResidual(rp, ci, d_values, nnz, d_x, d_b, bStride, r, n, nb, st);
+  if ((rc = dots(r, r, rn))) return rc;
+  HIP_TRY(hipEventRecord(num->evR1, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, num->evR0, num->evR1));
+  for (int32_t j = 0; j < nb; j++) {
+    info[j].relResidual = std::sqrt(rn[j]) / bn[j];
+    info[j].converged = info[j].relResidual <= o->tol;
+    info[j].ms = ms;
+  }
+  return AMP_OK;
+}
+
 // Y = A^-1 (M X) for nvec vectors (eigen.ts:156-166, `op`): FP64 products with M on the analysed CSR pattern, then
 // one multi-right-hand-side solve with the factor of A = K - sigma M held by num.
 int runShiftInvert(amp_num* num, const double* d_M, const double* d_X, double* d_Y, int32_t nvec) {
@@ -298,6 +427,19 @@ int runShiftInvert(amp_num* num, const double* d_M, const double* d_X, double* d
 }  // namespace
 
 extern "C" {
+
+int amp_pcg_batch_device(amp_num* num, const double* d_values, const double* d_b, int64_t bStride, double* d_x,
+                         int32_t nb, const amp_pcg_opts* o, amp_pcg_info* info) {
+  if (!num || !o || !info || nb < 1 || nb > 64 || bStride < 0 || (num->sym->S.n > 0 && (!d_values || !d_b || !d_x)))
+    return fail(AMP_ERR_ARG, "amp_pcg_batch_device: bad arguments");
+  if (num->sym->S.n == 0) {
+    for (int32_t j = 0; j < nb; j++) info[j] = amp_pcg_info{0, 1, 0, 0, 0};
+    return AMP_OK;
+  }
+  if (bStride != 0 && bStride < num->sym->S.n) return fail(AMP_ERR_ARG, "amp_pcg_batch_device: bStride < n");
+  HIP_TRY(amp::afterCaller(num->stream, num->inEv));
+  return runPcgBatch(num, d_values, d_b, bStride, d_x, nb, o, info);
+}
 
 int amp_shift_invert_apply_device(amp_num* num, const double* d_M, const double* d_X, double* d_Y, int32_t nvec) {
   if (!num || nvec < 0 || (nvec > 0 && num->sym->S.n > 0 && (!d_M || !d_X || !d_Y)))

@@ -91,6 +91,48 @@ __global__ void k_combine(const double* __restrict__ Z, int64_t ldz, int32_t nz,
   }
 }
 
+// ---- batches of independent systems (one right-hand side each, vectors of system j at j n) --------------------------
+
+// r_j = b_j - A_j x_j (b == nullptr: r_j = A_j x_j); values of system j at j nnz, b_j at j bStride
+__global__ __launch_bounds__(256) void k_batch_residual(const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colidx,
+                                                        const double* __restrict__ val, int64_t nnz,
+                                                        const double* __restrict__ x, const double* __restrict__ b,
+                                                        int64_t bStride, double* __restrict__ r, int32_t n, int32_t nb) {
+  const int32_t j = blockIdx.y;
+  const double* vj = val + int64_t(j) * nnz;
+  const double* xj = x + int64_t(j) * n;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = b ? b[int64_t(j) * bStride + i] : 0.0;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; p++) s -= vj[p] * xj[colidx[p]];
+    r[int64_t(j) * n + i] = b ? s : -s;
+  }
+}
+
+// partial[j gridDim.x + block] = block's share of u_j . w_j
+__global__ __launch_bounds__(kRedThreads) void k_batch_dot(const double* __restrict__ u, const double* __restrict__ w,
+                                                           int32_t n, double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads / 64];
+  const int32_t j = blockIdx.y;
+  const double* uj = u + int64_t(j) * n;
+  const double* wj = w + int64_t(j) * n;
+  double s = 0;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += uj[i] * wj[i];
+  s = blockSum(s, sh);
+  if (threadIdx.x == 0) partial[int64_t(j) * gridDim.x + blockIdx.x] = s;
+}
+
+// y_j = a_j x_j + c_j y_j with per-system coefficients on the device; c_j == 0 does not read y_j
+__global__ void k_batch_axpby(const double* __restrict__ a, const double* __restrict__ x,
+                              const double* __restrict__ c, double* __restrict__ y, int32_t n) {
+  const int32_t j = blockIdx.y;
+  const double aj = a[j], cj = c ? c[j] : 1.0;
+  const double* xj = x + int64_t(j) * n;
+  double* yj = y + int64_t(j) * n;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    yj[i] = cj == 0.0 ? aj * xj[i] : aj * xj[i] + cj * yj[i];
+}
+
 static unsigned gridFor(int32_t n) { return unsigned(std::min<int64_t>((int64_t(n) + 255) / 256, 4096)); }
 
 void launchResidual(const int32_t* rowptr, const int32_t* colidx, const double* val, const double* x, const double* b,
@@ -116,6 +158,25 @@ void launchAxpby(double a, const double* x, double c, double* y, int32_t n, hipS
 void launchCombine(const double* Z, int64_t ldz, int32_t nz, const double* coef, double* x, int32_t n,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_combine, dim3(gridFor(n)), dim3(256), 0, st, Z, ldz, nz, coef, x, n);
+}
+
+void launchBatchResidual(const int32_t* rowptr, const int32_t* colidx, const double* val, int64_t nnz, const double* x,
+                         const double* b, int64_t bStride, double* r, int32_t n, int32_t nb, hipStream_t st) {
+  const unsigned g = unsigned(std::min<int64_t>((int64_t(n) + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_batch_residual, dim3(g, nb), dim3(256), 0, st, rowptr, colidx, val, nnz, x, b, bStride, r, n, nb);
+}
+
+void launchBatchDot(const double* u, const double* w, int32_t n, int32_t nb, double* partial, double* out,
+                    hipStream_t st) {
+  const unsigned g = unsigned(std::min<int64_t>((int64_t(n) + kRedThreads - 1) / kRedThreads, kRedBlocks));
+  hipLaunchKernelGGL(k_batch_dot, dim3(g, nb), dim3(kRedThreads), 0, st, u, w, n, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, st, partial, int32_t(g), nb, out);
+}
+
+void launchBatchAxpby(const double* a, const double* x, const double* c, double* y, int32_t n, int32_t nb,
+                      hipStream_t st) {
+  const unsigned g = unsigned(std::min<int64_t>((int64_t(n) + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_batch_axpby, dim3(g, nb), dim3(256), 0, st, a, x, c, y, n);
 }
 
 int64_t partialEntries(int32_t maxVectors) { return int64_t(kRedBlocks) * maxVectors; }

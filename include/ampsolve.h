@@ -236,6 +236,13 @@ typedef struct {
 int amp_pcg_device(amp_num *num, const double *d_values, const double *d_b, double *d_x, const amp_pcg_opts *opts,
                    amp_pcg_info *info);
 
+/* The same for nb <= 64 independent systems in lockstep, sharing the preconditioner: system j has the matrix values
+ * d_values + j nnzPattern, the right-hand side d_b + j bStride (bStride 0: one shared right-hand side) and its guess and
+ * solution at d_x + j n. Each iteration applies the preconditioner to all systems in one multi-right-hand-side solve;
+ * a system stops updating once it meets tol. info[j] reports system j (ms: the whole call). */
+int amp_pcg_batch_device(amp_num *num, const double *d_values, const double *d_b, int64_t bStride, double *d_x,
+                         int32_t nb, const amp_pcg_opts *opts, amp_pcg_info *info);
+
 /* Warm-start network (no counterpart in the reference): an FP32 multilayer perceptron nIn -> nHidden -> nHidden -> nOut
  * (tanh hidden layers, linear output) with its training on the device. Parameters (flat, amp_mlp_param_count):
  * W1 (nHidden x nIn, row-major), b1, W2 (nHidden x nHidden), b2, W3 (nOut x nHidden), b3. Activations are batch-major
@@ -251,6 +258,30 @@ int amp_mlp_forward(amp_mlp *mlp, const float *d_x, int32_t batch, float *d_y);
 int amp_mlp_gradient(amp_mlp *mlp, const float *d_x, const float *d_t, int32_t batch, double *loss);
 int amp_mlp_adam(amp_mlp *mlp, float lr, float beta1, float beta2, float eps);
 void amp_mlp_free(amp_mlp *mlp);
+
+/* ---- device assembly of P1 diffusion matrices (FEM assembly feeding the factor) --------------------------------- */
+
+typedef struct amp_asm amp_asm;
+
+/* assembleCells (packages/physics/src/common/scalar.ts:96-144) for P1 simplices (dim 2: tri3, dim 3: tet4) with a
+ * per-element coefficient, set up once on the host: the element pattern (buildElementPattern, sparse.mjs:86-110), the
+ * element slots (elementSlots, sparse.mjs:113-125), the restriction to the free nodes (restrict, scalar.ts:210-233;
+ * freeMask NULL keeps every node) and the geometric element matrices |T| grad N_a . grad N_b. nodes: nn x dim,
+ * conn: ne x (dim + 1). */
+int amp_asm_create(int32_t dim, int32_t nn, const double *nodes, int32_t ne, const int32_t *conn,
+                   const uint8_t *freeMask, amp_asm **out);
+/* The restricted pattern (valid while the handle lives): n free DOFs, full symmetric CSR with sorted rows. */
+int amp_asm_get_pattern(const amp_asm *a, int32_t *n, int64_t *nnz, const int32_t **rowptr, const int32_t **colidx);
+/* Values of nb matrices on the restricted pattern (d_values: nb x nnz, device) from nb coefficient sets (d_k: nb x ne,
+ * device): every entry sums k_e Kgeo_e over its element contributions in element order, as the reference's scatter
+ * adds them. */
+int amp_assemble_device(amp_asm *a, const double *d_k, double *d_values, int32_t nb);
+/* Parametric coefficients of the learned sweep: k_e = exp(sum_m theta_m Phi_em) for nb parameter vectors (d_theta:
+ * nb x nm, device; d_k: nb x ne, device), with the mode values Phi (ne x nm, host) set once. No counterpart in the
+ * reference (its sweeps assemble on the host per point). */
+int amp_asm_set_modes(amp_asm *a, int32_t nm, const double *Phi);
+int amp_param_coefficients_device(amp_asm *a, const double *d_theta, double *d_k, int32_t nb);
+void amp_asm_free(amp_asm *a);
 
 /* Message of the last failure on the calling thread ("" when none). */
 const char *amp_last_error(void);

@@ -106,7 +106,15 @@ void orc_gmres(int32_t n, orc_apply_fn matvec, void *mctx, orc_apply_fn precond,
 void orc_csr_matvec(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *val, const double *x,
                     double *y);
 
+/* P1 diffusion assembly (buildElementPattern sparse.mjs:86-110, elementSlots :113-125, assembleCells scalar.ts:96-144,
+ * restrict scalar.ts:210-233): nodes nn x dim, conn ne x (dim + 1), kel per element, freeMask NULL = all free. Writes the
+ * restricted pattern and values (rowptr: free + 1, colidx / values: capacity ne (dim + 1)^2 + nn) and returns nnz. */
+int64_t orc_assemble_p1(int32_t dim, int32_t nn, const double *nodes, int32_t ne, const int32_t *conn,
+                        const double *kel, const uint8_t *freeMask, int32_t *rowptr, int32_t *colidx,
+                        double *values);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

@@ -90,6 +90,8 @@ def lib():
         L.orc_uplook_factor.restype = C.c_int
         L.orc_uplook_solve.argtypes = [C.POINTER(OrcUplook), f64p, f64p]
         L.orc_uplook_free.argtypes = [C.POINTER(OrcUplook)]
+        L.orc_assemble_p1.argtypes = [C.c_int32, C.c_int32, f64p, C.c_int32, i32p, f64p, C.c_void_p, i32p, i32p, f64p]
+        L.orc_assemble_p1.restype = C.c_int64
         L.orc_set_threads.argtypes = [C.c_int32]
         L.orc_get_threads.restype = C.c_int32
         L.orc_gmres.argtypes = [C.c_int32, APPLY_FN, C.c_void_p, APPLY_FN, C.c_void_p, f64p, C.c_double, C.c_int32,
@@ -167,6 +169,24 @@ class Sym:
                 self._p = None
         except Exception:
             pass
+
+
+def assemble_p1(nodes, conn, kel, free=None):
+    """P1 diffusion assembly restricted to the free nodes (assembleCells + elementSlots + restrict): returns the CSR
+    pattern dict and the values."""
+    nodes = np.ascontiguousarray(nodes, np.float64)
+    conn = np.ascontiguousarray(conn, np.int32)
+    nn, dim = nodes.shape
+    ne, k = conn.shape
+    kel = np.ascontiguousarray(kel, np.float64)
+    fm = None if free is None else np.ascontiguousarray(free, np.uint8)
+    nf = nn if fm is None else int(fm.sum())
+    cap = ne * k * k + nn
+    rp = np.zeros(nf + 1, np.int32)
+    ci = np.zeros(cap, np.int32)
+    v = np.zeros(cap)
+    nnz = lib().orc_assemble_p1(dim, nn, nodes, ne, conn, kel, None if fm is None else fm.ctypes.data, rp, ci, v)
+    return {"n": nf, "rowptr": rp, "colidx": ci[:nnz].copy(), "nnz": int(nnz)}, v[:nnz].copy()
 
 
 def set_threads(k):

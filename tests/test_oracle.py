@@ -335,3 +335,58 @@ def test_threaded_factor_is_bitwise_the_single_thread_one():
         O.set_threads(keep)
     assert ok1 and ok4 and S.maxFront > 512
     assert np.array_equal(L1, L4) and np.array_equal(Ld1, Ld4) and np.array_equal(D1, D4) and nn1 == nn4 == 0
+
+
+# ---- P1 assembly (assembleCells / elementSlots / restrict restated in oracle/assembly.c) -----------------------------
+
+def test_assembly_single_reference_tet_closed_form():
+    """The unit tet's P1 stiffness is k |T| G G^T with |T| = 1/6, G rows (-1,-1,-1), e1, e2, e3."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    pat, v = O.assemble_p1(nodes, np.array([[0, 1, 2, 3]]), np.array([2.5]))
+    K = sp.csr_matrix((v, pat["colidx"], pat["rowptr"]), shape=(4, 4)).toarray()
+    ref = 2.5 / 6 * np.array([[3, -1, -1, -1], [-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]], float)
+    assert np.abs(K - ref).max() < 1e-15
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_assembly_row_sums_patch_and_energy(dim):
+    """Unrestricted operator: zero row sums (constants), K u = 0 at interior nodes for a linear u with uniform k
+    (patch test), and u^T K u = sum_e k_e |T_e| |grad u|^2 for a linear u with random k."""
+    if dim == 3:
+        nodes, conn = W.box_tet_mesh(6, 5, 4)
+        g = np.array([0.3, -1.2, 0.7])
+    else:
+        nodes, conn = W.rect_tri_mesh(7, 6)
+        g = np.array([0.3, -1.2])
+    kel = 10.0 ** (2.0 * W.lcg_uniform(5, len(conn)))
+    pat, v = O.assemble_p1(nodes, conn, kel)
+    n = pat["n"]
+    K = sp.csr_matrix((v, pat["colidx"], pat["rowptr"]), shape=(n, n))
+    assert np.abs(K @ np.ones(n)).max() < 1e-13 * np.abs(v).max()
+    u = nodes @ g + 2.0
+    X = nodes[conn]
+    J = np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))
+    vol = np.abs(np.linalg.det(J)) / (6.0 if dim == 3 else 2.0)
+    assert abs(u @ (K @ u) / (g @ g * np.sum(kel * vol)) - 1) < 1e-13
+    patU, vU = O.assemble_p1(nodes, conn, np.ones(len(conn)))
+    KU = sp.csr_matrix((vU, patU["colidx"], patU["rowptr"]), shape=(n, n))
+    lo, hi = nodes.min(0), nodes.max(0)
+    interior = np.all((nodes > lo + 1e-12) & (nodes < hi - 1e-12), axis=1)
+    assert np.abs((KU @ u)[interior]).max() < 1e-12
+
+
+def test_assembly_restriction_matches_the_configs():
+    """The restricted pattern and values equal the workload generators' (heat3d, poisson2d), which build the
+    reference's element pattern with numpy."""
+    nodes, tets = W.box_tet_mesh(8, 8, 8)
+    kel = 10.0 ** (2.0 * W.lcg_uniform(1, len(tets)))
+    pat, v = O.assemble_p1(nodes, tets, kel, nodes[:, 2] > 1e-12)
+    pw, vw, _ = W.heat3d(8)
+    assert np.array_equal(pat["rowptr"], pw["rowptr"]) and np.array_equal(pat["colidx"], pw["colidx"])
+    assert np.abs(v - vw).max() <= 4e-15 * np.abs(vw).max()  # sums in another order: a few ulps of the largest
+    nodes, tris = W.rect_tri_mesh(16, 16)
+    x, y = nodes[:, 0], nodes[:, 1]
+    free = (x > 1e-12) & (x < 1 - 1e-12) & (y > 1e-12) & (y < 1 - 1e-12)
+    pat, v = O.assemble_p1(nodes, tris, np.ones(len(tris)), free)
+    pw, vw, _ = W.poisson2d(16)
+    assert np.array_equal(pat["colidx"], pw["colidx"]) and np.abs(v - vw).max() <= 4e-15 * np.abs(vw).max()
