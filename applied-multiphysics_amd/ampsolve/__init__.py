@@ -14,3 +14,4 @@ from .sparse import (  # noqa: F401
     solveNumeric,
     solveNumericMulti,
 )
+from .complex import ComplexSupernodalLDLT  # noqa: F401,E402

@@ -150,6 +150,7 @@ def analyze(pat, opts=None):
                                 1 if opts.get("relax", True) else 0, C.byref(h)))
     sym = Symbolic(h.value)
     sym.timing = {"symbolic_ms": 1e3 * (time.perf_counter() - t0)}
+    sym.pattern = {"n": n, "rowptr": rp.copy(), "colidx": ci.copy(), "nnz": int(rp[n]) if n else 0}
     return sym
 
 

@@ -113,6 +113,17 @@ int64_t orc_assemble_p1(int32_t dim, int32_t nn, const double *nodes, int32_t ne
                         const double *kel, const uint8_t *freeMask, int32_t *rowptr, int32_t *colidx,
                         double *values);
 
+/* ComplexSupernodalLDLT (ldlt.mjs:648-866): factor (Lr, Li: nnzLStored; Dr, Di: n; returns 1 ok, 0 on a non-finite
+ * pivot), solve (:780-829) and complex iterative refinement (:838-862, returns the steps taken). */
+int orc_cldlt_factor(const orc_sym *s, const double *re, const double *im, double pivotTol, double *Lr, double *Li,
+                     double *Dr, double *Di, int32_t *nPert);
+void orc_cldlt_solve(const orc_sym *s, const double *Lr, const double *Li, const double *Dr, const double *Di,
+                     const double *br, const double *bi, double *xr, double *xi);
+int32_t orc_cldlt_solve_refined(const orc_sym *s, const double *Lr, const double *Li, const double *Dr,
+                                const double *Di, const int32_t *rowptr, const int32_t *colidx, const double *re,
+                                const double *im, const double *br, const double *bi, int32_t maxSteps, double tol,
+                                double *xr, double *xi, double *rel);
+
 #ifdef __cplusplus
 }
 #endif

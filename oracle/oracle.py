@@ -92,6 +92,13 @@ def lib():
         L.orc_uplook_free.argtypes = [C.POINTER(OrcUplook)]
         L.orc_assemble_p1.argtypes = [C.c_int32, C.c_int32, f64p, C.c_int32, i32p, f64p, C.c_void_p, i32p, i32p, f64p]
         L.orc_assemble_p1.restype = C.c_int64
+        L.orc_cldlt_factor.argtypes = [C.POINTER(OrcSym), f64p, f64p, C.c_double, f64p, f64p, f64p, f64p,
+                                       C.POINTER(C.c_int32)]
+        L.orc_cldlt_factor.restype = C.c_int
+        L.orc_cldlt_solve.argtypes = [C.POINTER(OrcSym), f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_cldlt_solve_refined.argtypes = [C.POINTER(OrcSym), f64p, f64p, f64p, f64p, i32p, i32p, f64p, f64p, f64p,
+                                              f64p, C.c_int32, C.c_double, f64p, f64p, C.POINTER(C.c_double)]
+        L.orc_cldlt_solve_refined.restype = C.c_int32
         L.orc_set_threads.argtypes = [C.c_int32]
         L.orc_get_threads.restype = C.c_int32
         L.orc_gmres.argtypes = [C.c_int32, APPLY_FN, C.c_void_p, APPLY_FN, C.c_void_p, f64p, C.c_double, C.c_int32,
@@ -187,6 +194,38 @@ def assemble_p1(nodes, conn, kel, free=None):
     v = np.zeros(cap)
     nnz = lib().orc_assemble_p1(dim, nn, nodes, ne, conn, kel, None if fm is None else fm.ctypes.data, rp, ci, v)
     return {"n": nf, "rowptr": rp, "colidx": ci[:nnz].copy(), "nnz": int(nnz)}, v[:nnz].copy()
+
+
+def cldlt_factor(sym, re, im, pivot_tol=1e-13):
+    """ComplexSupernodalLDLT.factor: (ok, Lr, Li, Dr, Di, nPerturbed)."""
+    L = lib()
+    nl = max(sym.nnzLStored, 1)
+    Lr, Li = np.zeros(nl), np.zeros(nl)
+    Dr, Di = np.zeros(max(sym.n, 1)), np.zeros(max(sym.n, 1))
+    npert = C.c_int32()
+    ok = L.orc_cldlt_factor(sym._p, np.ascontiguousarray(re, np.float64), np.ascontiguousarray(im, np.float64),
+                            float(pivot_tol), Lr, Li, Dr, Di, C.byref(npert))
+    return bool(ok), Lr, Li, Dr, Di, npert.value
+
+
+def cldlt_solve(sym, Lr, Li, Dr, Di, br, bi):
+    n = sym.n
+    xr, xi = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    lib().orc_cldlt_solve(sym._p, Lr, Li, Dr, Di, np.ascontiguousarray(br, np.float64),
+                          np.ascontiguousarray(bi, np.float64), xr, xi)
+    return xr[:n], xi[:n]
+
+
+def cldlt_solve_refined(sym, Lr, Li, Dr, Di, pat, re, im, br, bi, max_steps=3, tol=1e-14):
+    n = sym.n
+    xr, xi = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    rel = C.c_double()
+    rp, ci = _pat(pat)
+    steps = lib().orc_cldlt_solve_refined(sym._p, Lr, Li, Dr, Di, rp, ci, np.ascontiguousarray(re, np.float64),
+                                          np.ascontiguousarray(im, np.float64), np.ascontiguousarray(br, np.float64),
+                                          np.ascontiguousarray(bi, np.float64), int(max_steps), float(tol), xr, xi,
+                                          C.byref(rel))
+    return xr[:n], xi[:n], rel.value, steps
 
 
 def set_threads(k):

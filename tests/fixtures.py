@@ -123,3 +123,68 @@ def hex_matrix(nx=6, ny=5, nz=4):
             free[N(0, j, k)] = False
     R, _ = W.restrict(A, free)
     return W.csr_dict(R), R.data.astype(np.float64)
+
+
+def complex_grid27(N, zeroEvery=0, shift=0.0):
+    """grid() of la-complex-solve.test.mjs:16-36: 27-point pattern on an N^3 grid (hex-element pattern), complex-
+    symmetric values; `shift` makes the real part indefinite. Returns (pat, re, im)."""
+    n = N * N * N
+    idf = lambda i, j, k: i + N * (j + N * k)  # noqa: E731
+    el = []
+    for k in range(N - 1):
+        for j in range(N - 1):
+            for i in range(N - 1):
+                for a, b, c in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1)):
+                    el.append(idf(i + a, j + b, k + c))
+    pat = element_pattern(n, np.array(el, dtype=np.int64).reshape(-1, 8))
+    rows = np.repeat(np.arange(n), np.diff(pat["rowptr"]))
+    cols = pat["colidx"].astype(np.int64)
+    lo, hi = np.minimum(rows, cols), np.maximum(rows, cols)
+    diag = rows == cols
+    re = np.where(diag, 26.5 + 0.01 * (rows % 7) - shift, -1 + 0.001 * ((lo * 31 + hi * 17) % 11))
+    if zeroEvery:
+        re = np.where(~diag & ((rows + cols) % zeroEvery == 0), 0.0, re)
+    im = np.where(diag, 0.2 + 0.001 * (rows % 5), np.where((lo + hi) % 3 == 0, 0.0, 0.01 * ((lo * 7 + hi) % 5)))
+    return pat, re.astype(np.float64), im.astype(np.float64)
+
+
+def complex_ldlt_system(nb, seed=4242):
+    """system(nb, true) of la-ldlt.test.mjs:12-45: 2x2 diagonal blocks [[0, a], [a, 0]] with complex a chained by
+    complex couplings; the exact solution x = xr + i xi and b = A x. Returns (pat, re, im, xr, xi, br, bi)."""
+    n = 2 * nb
+    rnd = lcg_rnd(seed)
+    ent = {}
+
+    def add(i, j, vr, vi):
+        for a, b in ([(i, j)] if i == j else [(i, j), (j, i)]):
+            v = ent.get((a, b), (0.0, 0.0))
+            ent[(a, b)] = (v[0] + vr, v[1] + vi)
+
+    for b in range(nb):
+        vr = 2 + rnd()
+        vi = 0.3 * rnd()
+        add(2 * b, 2 * b + 1, vr, vi)
+        add(2 * b, 2 * b, 0.0, 0.0)
+        add(2 * b + 1, 2 * b + 1, 0.0, 0.0)
+        if b + 1 < nb:
+            vr = 0.3 * rnd()
+            vi = 0.1 * rnd()
+            add(2 * b + 1, 2 * b + 2, vr, vi)
+    keys = sorted(ent.keys())
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    for (i, _j) in keys:
+        rowptr[i + 1] += 1
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    colidx = np.array([j for (_i, j) in keys], dtype=np.int32)
+    re = np.array([ent[k][0] for k in keys])
+    im = np.array([ent[k][1] for k in keys])
+    pat = {"n": n, "rowptr": rowptr, "colidx": colidx, "nnz": len(keys)}
+    xr = np.array([rnd() for _ in range(n)])
+    xi = np.array([rnd() for _ in range(n)])
+    br, bi = np.zeros(n), np.zeros(n)
+    for i in range(n):
+        for p in range(rowptr[i], rowptr[i + 1]):
+            j = colidx[p]
+            br[i] += re[p] * xr[j] - im[p] * xi[j]
+            bi[i] += re[p] * xi[j] + im[p] * xr[j]
+    return pat, re, im, xr, xi, br, bi

@@ -390,3 +390,60 @@ def test_assembly_restriction_matches_the_configs():
     pat, v = O.assemble_p1(nodes, tris, np.ones(len(tris)), free)
     pw, vw, _ = W.poisson2d(16)
     assert np.array_equal(pat["colidx"], pw["colidx"]) and np.abs(v - vw).max() <= 4e-15 * np.abs(vw).max()
+
+
+# ---- complex-symmetric LDL^T (ComplexSupernodalLDLT restated in oracle/complex.c) -----------------------------------
+
+def _cdense(pat, re, im):
+    A = sp.csr_matrix((re + 1j * im, pat["colidx"], pat["rowptr"]), shape=(pat["n"], pat["n"]))
+    return A.toarray()
+
+
+@pytest.mark.parametrize("N,shift", [(6, 0.0), (6, 13.0), (5, 26.0)])
+def test_complex_ldlt_matches_dense_solve(N, shift):
+    """The complex factor and solve against LAPACK's complex solve of the dense matrix (the la-complex-solve grids)."""
+    from fixtures import complex_grid27
+
+    pat, re, im = complex_grid27(N, shift=shift)
+    n = pat["n"]
+    S = O.Sym(pat)
+    ok, Lr, Li, Dr, Di, npert = O.cldlt_factor(S, re, im)
+    assert ok and npert == 0
+    br, bi = np.sin(0.3 * np.arange(n)), 0.5 * np.cos(0.7 * np.arange(n))
+    xr, xi = O.cldlt_solve(S, Lr, Li, Dr, Di, br, bi)
+    x = np.linalg.solve(_cdense(pat, re, im), br + 1j * bi)
+    assert np.abs(xr + 1j * xi - x).max() / np.abs(x).max() < 1e-11
+
+
+def test_complex_ldlt_zero_imaginary_part_is_the_real_factor():
+    """With im = 0 the complex factor carries the real LDL^T's values (another summation order: to round-off) and
+    zero imaginary parts exactly."""
+    from fixtures import complex_grid27
+
+    pat, re, _ = complex_grid27(6, shift=13.0)
+    im = np.zeros_like(re)
+    S = O.Sym(pat)
+    ok, Lr, Li, Dr, Di, npert = O.cldlt_factor(S, re, im)
+    okr, Lx, D, nneg, nper = O.ldlt_factor(S, re)
+    assert ok and okr and not np.any(Li) and not np.any(Di)
+    assert np.abs(Dr - D).max() / np.abs(D).max() < 1e-13
+    m = np.zeros(S.nnzLStored, bool)
+    for s in range(S.nsuper):
+        nc, mm = S.superStart[s + 1] - S.superStart[s], S.Sp[s + 1] - S.Sp[s]
+        m[S.Lxp[s]:S.Lxp[s + 1]] = np.tril(np.ones((mm, nc), bool)).T.ravel()
+    assert np.abs(Lr[m] - Lx[m]).max() / np.abs(Lx[m]).max() < 1e-12
+
+
+def test_complex_ldlt_perturbed_pivots_and_refinement():
+    """la-ldlt.test.mjs:70-82: exact zero pivots in the identity order are perturbed; the plain solve is visibly wrong
+    and 30 complex refinement steps recover the exact solution to 1e-10."""
+    from fixtures import complex_ldlt_system
+
+    pat, re, im, xr, xi, br, bi = complex_ldlt_system(40)
+    S = O.Sym(pat, perm=np.arange(pat["n"]))
+    ok, Lr, Li, Dr, Di, npert = O.cldlt_factor(S, re, im)
+    assert ok and npert > 0
+    pr, pi = O.cldlt_solve(S, Lr, Li, Dr, Di, br, bi)
+    assert np.abs(pr - xr).max() / np.abs(xr).max() > 1e-6
+    rr, ri, rel, steps = O.cldlt_solve_refined(S, Lr, Li, Dr, Di, pat, re, im, br, bi, max_steps=30)
+    assert np.abs(rr - xr).max() / np.abs(xr).max() < 1e-10 and np.abs(ri - xi).max() / np.abs(xi).max() < 1e-10
