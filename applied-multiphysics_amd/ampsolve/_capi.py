@@ -122,6 +122,7 @@ SIGNATURES = {
     "amp_graph_free": ([C.c_void_p], None),
     "amp_num_set_profiling": ([C.c_void_p, C.c_int32], C.c_int),
     "amp_num_get_kernel_ms": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
+    "amp_num_set_growth_limit": ([C.c_void_p, C.c_double], C.c_int),
     "amp_num_free": ([C.c_void_p], None),
     "amp_asm_create": ([C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
                        C.c_int),

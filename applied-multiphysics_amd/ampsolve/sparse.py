@@ -282,6 +282,11 @@ class NumericFactor:
     def set_profiling(self, enable=True):
         A.check(A.lib().amp_num_set_profiling(self._h, 1 if enable else 0))
 
+    def set_growth_limit(self, limit):
+        """FP32 factors: diagonal blocks whose largest elimination multiplier exceeds `limit` are redone in FP64
+        (0: never). The count of the last factor is kernel_ms('diag_fp64_redo')[1]."""
+        A.check(A.lib().amp_num_set_growth_limit(self._h, float(limit)))
+
     def release_workspace(self):
         """Free the update-matrix workspace (solves keep working; the next factor allocates it again)."""
         A.check(A.lib().amp_num_release_workspace(self._h))

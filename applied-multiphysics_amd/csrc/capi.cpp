@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "handles.h"
@@ -29,6 +30,9 @@ int fail(int code, const std::string& msg) {
 }  // namespace amp
 
 namespace {
+
+// Largest elimination multiplier of a diagonal block an FP32 factor keeps in FP32 arithmetic
+constexpr double kDefaultGrowthLimit = 1e4;
 
 // Factor and solve plans of the analysis; AMP_ERR_ARG when the system exceeds the plans' index ranges (the handle's
 // plans are then unchanged).
@@ -122,6 +126,8 @@ amp::FactorArgs<T> factorArgs(amp_num* num) {
   a.ldlt = num->mode == AMP_LDLT;
   a.tau = 0;
   a.bulkBf16 = num->prec == AMP_F32_BF16X3;
+  a.growthLimit = std::is_same<T, float>::value ? num->growthLimit : 0.0;
+  a.fallbacks = num->counts.as<int32_t>() + 2;
   return a;
 }
 
@@ -176,11 +182,11 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
                              num->profiling ? &num->prof : nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(num->ev1, num->stream));
-  int32_t counts[2] = {0, 0}, failCol = INT32_MAX;
+  int32_t counts[3] = {0, 0, 0}, failCol = INT32_MAX;
   if (n > 0) {
     if (num->mode == AMP_LDLT) amp::launchLdltCounts<T>(a, n, num->counts.as<int32_t>(), num->stream);
     HIP_TRY(hipMemcpyAsync(&failCol, num->failCol.p, 4, hipMemcpyDeviceToHost, num->stream));
-    HIP_TRY(hipMemcpyAsync(counts, num->counts.p, 8, hipMemcpyDeviceToHost, num->stream));
+    HIP_TRY(hipMemcpyAsync(counts, num->counts.p, 12, hipMemcpyDeviceToHost, num->stream));
   }
   HIP_TRY(hipStreamSynchronize(num->stream));
   float ms = 0;
@@ -200,6 +206,9 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
       lv.second += 1;
     }
   }
+  auto& fb = num->kernelMs["diag_fp64_redo"];  // FP32 factors: diagonal blocks redone in FP64 (count)
+  fb.first = 0;
+  fb.second = counts[2];
   num->ok = failCol == INT32_MAX;
   if (info) {
     info->ok = num->ok ? 1 : 0;
@@ -542,7 +551,9 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   HIP_TRY(num->D.alloc(size_t(std::max(h->S.n, 1)) * es));
   HIP_TRY(num->pert.alloc(size_t(std::max(h->S.n, 1))));
   HIP_TRY(num->failCol.alloc(4));
-  HIP_TRY(num->counts.alloc(8));
+  HIP_TRY(num->counts.alloc(16));  // nNeg, nPerturbed, FP64 diagonal-block redos
+  // static pivoting (LDL^T) can grow the multipliers; a Cholesky factor of an SPD matrix cannot (|L|^2 <= |A|)
+  num->growthLimit = precision == AMP_F64 || mode == AMP_CHOLESKY ? 0.0 : kDefaultGrowthLimit;
   HIP_TRY(num->values.alloc(size_t(std::max<int64_t>(h->S.nnzPattern, 1)) * 8));
   {
     const amp::SolvePlan& SP = h->SP;
@@ -656,6 +667,12 @@ int amp_num_get_kernel_ms(amp_num* num, const char* kernel, double* total_ms, in
   auto it = num->kernelMs.find(kernel);
   *total_ms = it == num->kernelMs.end() ? 0.0 : it->second.first;
   *launches = it == num->kernelMs.end() ? 0 : it->second.second;
+  return AMP_OK;
+}
+
+int amp_num_set_growth_limit(amp_num* num, double limit) {
+  if (!num || !(limit >= 0)) return fail(AMP_ERR_ARG, "amp_num_set_growth_limit: bad arguments");
+  num->growthLimit = num->prec == AMP_F64 ? 0.0 : limit;
   return AMP_OK;
 }
 

@@ -487,6 +487,7 @@ template <typename T>
 void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t main,
                       hipStream_t side, const std::vector<hipEvent_t>& sync, ProfileSink* prof) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, main, a.failCol, INT32_MAX);
+  if (a.fallbacks) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, main, a.fallbacks, 0);
   if (prof) prof->used = 0;
   hipEvent_t last[2] = {nullptr, nullptr};  // most recent signal of each stream
   size_t ev = 0;

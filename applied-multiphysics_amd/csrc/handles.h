@@ -112,6 +112,7 @@ struct amp_num {
   bool ok = false;
   bool profiling = false;
   int32_t liveGraphs = 0;  // captured graphs holding this handle's buffer addresses (amp_graph_create / _free)
+  double growthLimit = 0;  // FP32 factors: FP64 redo of diagonal blocks with a larger elimination multiplier
   amp::ProfileSink prof, solveProf;
   float solveMs = 0;
   std::map<std::string, std::pair<double, int32_t>> kernelMs;

@@ -75,6 +75,9 @@ struct FactorArgs {
   int32_t ldlt;
   double tau;
   int32_t bulkBf16;  // FP32 factor: bulk trailing updates as three bf16 products (k_gemm_bf16x3)
+  double growthLimit;  // FP32 factor: diagonal blocks whose largest elimination multiplier exceeds it are redone in
+                       // FP64 arithmetic (0: never)
+  int32_t* fallbacks;  // count of such blocks in the current factorization
 };
 
 }  // namespace amp

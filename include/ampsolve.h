@@ -175,6 +175,13 @@ int amp_download_factor(amp_num *num, double *Lx, double *D);
 int amp_num_set_profiling(amp_num *num, int32_t enable);
 int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, int32_t *launches);
 
+/* FP32 factors (no counterpart in the reference): a 64 x 64 diagonal block of a tiled front whose largest elimination
+ * multiplier (|l_ik / l_kk| for Cholesky, |l_ik| for LDL^T) exceeds `limit` is factored again in FP64 arithmetic, with
+ * FP64 pivots, and stored rounded to FP32 (0: never; default 1e4 for LDL^T, whose static pivots can make the
+ * multipliers grow, and 0 for Cholesky, whose factor of an SPD matrix cannot; ignored by FP64 factors). The number of blocks redone
+ * in the last factor is reported as the launch count of the kernel class "diag_fp64_redo" (amp_num_get_kernel_ms). */
+int amp_num_set_growth_limit(amp_num *num, double limit);
+
 void amp_num_free(amp_num *num);
 /* Free the factor's update-matrix workspace (needed only while factoring; the next factor allocates it again), so
  * that many factors can stay resident for solves only, as the parametric sweep's factor cache keeps them.
