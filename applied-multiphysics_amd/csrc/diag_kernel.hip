@@ -149,6 +149,7 @@ __global__ __launch_bounds__(64 * kDiagWaves) void k_diag_block(const DiagTask* 
   constexpr int J = kDiagSlots;
   __shared__ double ldsRaw[3 * 2 * kDiagStep * kTileEdge];
   __shared__ double gred[kDiagWaves];
+  chainPriority();
   const DiagTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

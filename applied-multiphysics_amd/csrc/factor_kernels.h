@@ -90,6 +90,11 @@ constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 va
 void launchGemmBf16x3(const GemmTask* tasks, unsigned grid, float* Lx, float* U, float* W, const float* D, bool ldlt,
                       hipStream_t st);
 
+// Dynamic LDS for a side-stream bulk launch of `kernel` (`threads` per workgroup, `staticLds` bytes of static LDS):
+// padding that lets one workgroup fewer fit on a CU than without it (0 when the kernel fits at most two), so that the
+// panel chain's kernels find room beside the bulk updates on every CU. Cached per kernel and device.
+size_t bulkLds(const void* kernel, int threads, size_t staticLds);
+
 // Clear one int32 device flag with a kernel (a kernel node, not a memset node, when the caller captures the stream).
 void launchClearFlag(int32_t* flag, hipStream_t st);
 

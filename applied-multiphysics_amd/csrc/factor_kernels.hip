@@ -4,7 +4,10 @@
 // The arithmetic per front is the reference's dense partial factorization (packages/core/src/la/sparse.mjs:992-1098,
 // ldlt.mjs:492-535, pivots as wasm/front-real.mjs:367-389), reorganised as a blocked right-looking factorization
 // whose trailing products run on the matrix cores.
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "factor_kernels.h"
 #include "pivot.h"
@@ -35,6 +38,7 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_small_front(const int32_t* __restrict__ fronts, FactorArgs<T> a,
                                                     const double* __restrict__ values, int32_t maxM) {
   constexpr int NW = NT / 64;
+  chainPriority();
   extern __shared__ unsigned char smem[];
   T* F = reinterpret_cast<T*>(smem);
   T* wcol = F + maxM * maxM;
@@ -98,6 +102,7 @@ template <typename T, int MB>
 __global__ __launch_bounds__(256) void k_small_front_wave(const int32_t* __restrict__ fronts, int32_t count,
                                                           FactorArgs<T> a, const double* __restrict__ values) {
   __shared__ T Fs[4][MB * MB];
+  chainPriority();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t idx = int64_t(blockIdx.x) * 4 + w;
   if (idx >= count) return;  // whole waves only: no workgroup barrier below
@@ -159,6 +164,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
                                                     const EaRange* __restrict__ ranges, FactorArgs<T> a) {
   constexpr int R = 8, CH = 64 * R, NWV = 4;  // rows in flight per lane, rows per chunk, waves
+  chainPriority();
   const EaTask t = tasks[blockIdx.x];
   const int32_t wave = int32_t(threadIdx.x >> 6);
   const bool factorCols = t.j0 < a.superStart[t.s + 1] - a.superStart[t.s];
@@ -222,14 +228,22 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
 // double-buffer the chunks in LDS (one barrier per chunk).
 // Workgroups are dispatched round-robin over the eight XCDs; each XCD is given a contiguous run of the task list,
 // whose neighbouring tiles share operand panels in that XCD's L2.
+// chain != 0 (launches of the main stream): the waves raise their issue priority over the side stream's bulk updates,
+// whose launches (chain == 0) take padding LDS so that each CU keeps room for the chain's workgroups (bulkLds).
 // Loads are issued on every path (addresses clamped into the operands, chunks past the end re-read its last depth),
 // and the entries outside an operand are replaced by zeros when a chunk is written to LDS: the wait before a chunk is
 // written then counts only the loads issued before it, so the chunk loaded two steps ahead stays in flight.
 // LDLT: the kernel built for LDL^T products scales the B operand by the pivots of its depths (B D) when a chunk is
 // written to LDS (update tasks; the triangular-solve tasks are not scaled).
+// static LDS of a k_gemm_tile instance (its two operand buffers)
+template <typename T, int G>
+constexpr size_t gemmTileLds() {
+  return size_t(2) * (G == 128 ? 2 : 1) * ((G == 128 || sizeof(T) == 8) ? 16 : 32) * (G + 4) * sizeof(T);
+}
+
 template <typename T, int G, bool LDLT>
 __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) void k_gemm_tile(const GemmTask* __restrict__ tasks, T* Lx, T* U,
-                                                                   T* W, const T* __restrict__ D) {
+                                                                   T* W, const T* __restrict__ D, int32_t chain) {
   using V4 = typename Vec4<T>::type;
   constexpr int WRW = G == 128 ? 4 : 2, WCW = 2, NT = 64 * WRW * WCW;  // wave grid, threads
   constexpr int WTR = G / WRW, WTC = G / WCW, NBI = WTR / 16, NBJ = WTC / 16;
@@ -240,6 +254,8 @@ __global__ __launch_bounds__(G == 128 ? 512 : 256, G == 128 && !LDLT ? 4 : 1) vo
   constexpr int NBUF = G == 128 ? 2 : 1;
   __shared__ T As[NBUF][KC * PA];
   __shared__ T Bs[NBUF][KC * PA];
+  static_assert(sizeof(As) + sizeof(Bs) == gemmTileLds<T, G>(), "LDS of the operand buffers");
+  if (chain) chainPriority();
   const int nblk = gridDim.x, q8 = nblk / 8, r8 = nblk % 8, x8 = blockIdx.x % 8;
   const GemmTask t = tasks[x8 * q8 + min(x8, r8) + blockIdx.x / 8];
   // buffers chosen by selects of the kernel arguments (not an indexed array), so that the compiler keeps them in the
@@ -483,6 +499,34 @@ __global__ __launch_bounds__(256) void k_ldlt_counts(const T* __restrict__ D, co
 __global__ void k_set_int(int32_t* p, int32_t v) { *p = v; }
 
 // ---- launchers ------------------------------------------------------------------------------------------------
+size_t bulkLds(const void* kernel, int threads, size_t staticLds) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> pads;  // (kernel, device) -> dynamic LDS bytes
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = pads.find({kernel, dev});
+  if (it != pads.end()) return it->second;
+  size_t pad = 0;
+  int perCu = 0, ldsCu = 0, ldsBlock = 0;
+  // kernels that fit two workgroups or fewer per CU are left alone: halving their residency cost more bulk throughput
+  // than the chain gained (measured on the 128-tile FP64 and bf16 bulk kernels)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, threads, 0) == hipSuccess && perCu >= 3 &&
+      hipDeviceGetAttribute(&ldsCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+      hipDeviceGetAttribute(&ldsBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
+    // smallest workgroup footprint (static + padding, in 1 KiB steps) at which only perCu - 1 workgroups fit
+    const size_t want = size_t(ldsCu) / size_t(perCu) + 1;
+    size_t p = want > staticLds ? (want - staticLds + 1023) / 1024 * 1024 : 0;
+    if (p > 0 && staticLds + p <= size_t(ldsBlock)) {
+      int fit = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, kernel, threads, p) == hipSuccess && fit == perCu - 1)
+        pad = p;
+    }
+  }
+  pads.emplace(std::make_pair(kernel, dev), pad);
+  return pad;
+}
+
 template <typename T>
 void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t main,
                       hipStream_t side, const std::vector<hipEvent_t>& sync, ProfileSink* prof) {
@@ -548,15 +592,20 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
             break;
           }
         }
+        // main-stream launches: chain priority; side stream: padding LDS that leaves room for the chain
+        auto go = [&](auto kernel, int threads, size_t staticLds) {
+          const size_t pad = si ? bulkLds(reinterpret_cast<const void*>(kernel), threads, staticLds) : 0;
+          hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), pad, st, gt, a.Lx, a.U, a.W, a.D, 1 - si);
+        };
         if (L.edge == kGemmEdge) {
           if (a.ldlt)
-            hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge, true>), dim3(grid), dim3(512), 0, st, gt, a.Lx, a.U, a.W, a.D);
+            go(k_gemm_tile<T, kGemmEdge, true>, 512, gemmTileLds<T, kGemmEdge>());
           else
-            hipLaunchKernelGGL((k_gemm_tile<T, kGemmEdge, false>), dim3(grid), dim3(512), 0, st, gt, a.Lx, a.U, a.W, a.D);
+            go(k_gemm_tile<T, kGemmEdge, false>, 512, gemmTileLds<T, kGemmEdge>());
         } else if (a.ldlt) {
-          hipLaunchKernelGGL((k_gemm_tile<T, kTileEdge, true>), dim3(grid), dim3(256), 0, st, gt, a.Lx, a.U, a.W, a.D);
+          go(k_gemm_tile<T, kTileEdge, true>, 256, gemmTileLds<T, kTileEdge>());
         } else {
-          hipLaunchKernelGGL((k_gemm_tile<T, kTileEdge, false>), dim3(grid), dim3(256), 0, st, gt, a.Lx, a.U, a.W, a.D);
+          go(k_gemm_tile<T, kTileEdge, false>, 256, gemmTileLds<T, kTileEdge>());
         }
         break;
       }

@@ -7,6 +7,7 @@
 // super-panel) stays FP32; only the launches of the side stream (every later column and the Schur complement from a
 // factored super-panel) take this kernel, and FP64 iterative refinement restores the solution's accuracy.
 //
+// Launched with padding LDS (bulkLds) so that each CU keeps room for the panel chain's workgroups.
 // Tiles of 128 x 128 (modes 0: C -= A B^T and 1: C = -A B^T of plan.h, LDL^T: B scaled by the pivots of its depths
 // before the split); eight waves in 4 x 2, each 32 rows x 64 columns of 16 x 16 blocks. Depth chunks of 32: a thread
 // loads 8 consecutive depths of one row of A and of B (FP32, coalesced along rows), splits them and writes the heads
@@ -178,10 +179,14 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
 
 void launchGemmBf16x3(const GemmTask* tasks, unsigned grid, float* Lx, float* U, float* W, const float* D, bool ldlt,
                       hipStream_t st) {
+  constexpr size_t lds = sizeof(unsigned char[2][4][kG * kRowBytes]);
   if (ldlt)
-    hipLaunchKernelGGL(k_gemm_bf16x3<true>, dim3(grid), dim3(512), 0, st, tasks, Lx, U, W, D);
+    hipLaunchKernelGGL(k_gemm_bf16x3<true>, dim3(grid), dim3(512),
+                       bulkLds(reinterpret_cast<const void*>(&k_gemm_bf16x3<true>), 512, lds), st, tasks, Lx, U, W, D);
   else
-    hipLaunchKernelGGL(k_gemm_bf16x3<false>, dim3(grid), dim3(512), 0, st, tasks, Lx, U, W, D);
+    hipLaunchKernelGGL(k_gemm_bf16x3<false>, dim3(grid), dim3(512),
+                       bulkLds(reinterpret_cast<const void*>(&k_gemm_bf16x3<false>), 512, lds), st, tasks, Lx, U, W,
+                       D);
 }
 
 }  // namespace amp

@@ -8,6 +8,11 @@
 
 namespace amp {
 
+// Kernels of the factor's panel chain (main stream) raise their waves' issue priority: the side stream's bulk updates
+// keep every CU occupied with long-running workgroups, and a chain wave placed beside them would otherwise take a
+// round-robin share of its SIMD's issue slots while every later super-panel waits on it.
+__device__ inline void chainPriority() { __builtin_amdgcn_s_setprio(3); }
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
