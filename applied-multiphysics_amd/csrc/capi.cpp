@@ -39,7 +39,7 @@ constexpr double kDefaultGrowthLimit = 1e4;
 int buildPlans(amp_sym* h, const amp::PlanParams& p) {
   try {
     amp::FactorPlan F = amp::buildFactorPlan(h->S, p);
-    amp::SolvePlan SP = amp::buildSolvePlan(h->S, F);
+    amp::SolvePlan SP = amp::buildSolvePlan(h->S, F, p);
     h->F = std::move(F);
     h->SP = std::move(SP);
   } catch (const std::length_error& e) {
@@ -828,7 +828,8 @@ int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
 
 bool validParams(const amp::PlanParams& p) {
   return p.smallMax >= 1 && p.smallMax <= amp::kSmallMax && p.smallNarrow >= 1 && p.smallNarrow <= p.smallMax &&
-         p.wideLaunch >= 1 && p.superPanel >= amp::kTileEdge && p.superPanel % amp::kTileEdge == 0;
+         p.wideLaunch >= 1 && p.superPanel >= amp::kTileEdge && p.superPanel % amp::kTileEdge == 0 &&
+         p.flowWideMin >= 0;
 }
 
 // factor + one solve (zero right-hand side), device milliseconds of the second of two runs
@@ -863,6 +864,7 @@ int amp_sym_set_params(amp_sym* h, const amp_tune_info* t) {
   p.smallNarrow = t->smallNarrow;
   p.wideLaunch = t->wideLaunch;
   p.superPanel = t->superPanel;
+  p.flowWideMin = t->flowWideMin;
   if (!validParams(p)) return fail(AMP_ERR_ARG, "amp_sym_set_params: thresholds out of range");
   return rebuildPlans(h, p);
 }
@@ -884,6 +886,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
   int32_t tried = 1;
   // one threshold at a time, from the defaults
   const int32_t superPanels[] = {128, 512}, wides[] = {256, 1024, 1 << 30}, smalls[] = {64}, narrows[] = {16, 64};
+  const int32_t flowWides[] = {64, 1 << 30};  // forward solve launches: wide tasks from 64 tasks on, or never
   auto attempt = [&](amp::PlanParams p) -> int {
     if (!validParams(p)) return AMP_OK;
     int r = rebuildPlans(h, p);
@@ -919,6 +922,11 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     p.smallNarrow = std::min(v, p.smallMax);
     if ((rc = attempt(p))) return rc;
   }
+  for (int32_t v : flowWides) {
+    amp::PlanParams p = best;
+    p.flowWideMin = v;
+    if ((rc = attempt(p))) return rc;
+  }
   rc = rebuildPlans(h, best);
   if (rc) return rc;
   if (info) {
@@ -926,6 +934,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     info->smallNarrow = best.smallNarrow;
     info->wideLaunch = best.wideLaunch;
     info->superPanel = best.superPanel;
+    info->flowWideMin = best.flowWideMin;
     info->candidates = tried;
     info->defaultMs = defaultMs;
     info->bestMs = bestMs;

@@ -343,7 +343,7 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
   return P;
 }
 
-SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
+SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F, const PlanParams& prm) {
   SolvePlan P;
   const int32_t ns = S.nsuper;
   P.fvEntries = ns ? S.Sp[ns] : 0;
@@ -375,6 +375,29 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
       push(kind, L, first, int64_t(P.fronts.size()) - first, maxM);
     }
   };
+  // a flow launch with its task lists for FP32 and FP64 factors: `emit(nq)` appends the tasks of nq blocks to P.blocks;
+  // a forward launch takes wide tasks when it has at least prm.flowWideMin of them
+  auto pushFlow = [&](SolveKind kind, int32_t L, auto emit) {
+    auto list = [&](int32_t wide, int64_t& first, int64_t& count, int32_t& nq) {
+      first = int64_t(P.blocks.size());
+      nq = 1;
+      if (kind == kSolveFwdFlow) {
+        emit(wide);
+        if (int64_t(P.blocks.size()) - first >= prm.flowWideMin) {
+          nq = wide;
+          count = int64_t(P.blocks.size()) - first;
+          return;
+        }
+        P.blocks.resize(size_t(first));
+      }
+      emit(1);
+      count = int64_t(P.blocks.size()) - first;
+    };
+    SolveLaunch sl{kind, L, 0, 0, 0};
+    list(kFlowQuadsF32, sl.first, sl.count, sl.nq);
+    list(kFlowQuadsF64, sl.first2, sl.count2, sl.nq2);
+    if (sl.count > 0) P.launches.push_back(sl);
+  };
   // forward, leaves first
   for (int32_t L = 0; L < F.nlevels; L++) {
     std::vector<int32_t> big;
@@ -387,23 +410,24 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     int32_t maxM = 0;
     for (int32_t s : big) maxM = std::max(maxM, frontRows(S, s));
     push(kSolveGather, L, first, int64_t(big.size()), maxM);
-    // every task depends only on tasks listed before it: own tiles by block index across the level's supernodes,
-    // then the below tiles (which consume every block of their supernode)
-    const int64_t bf = int64_t(P.blocks.size());
+    // every task depends only on tasks listed before it: the chain tasks by block index across the level's
+    // supernodes, then the below tasks (which consume every block of their supernode)
     int32_t maxBlk = 0;
     for (int32_t s : big) maxBlk = std::max(maxBlk, nblocks(s));
-    for (int32_t t = 0; t < maxBlk; t++)
+    pushFlow(kSolveFwdFlow, L, [&](int32_t nq) {
+      for (int32_t t = 0; t < maxBlk; t += nq)
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), nb = nblocks(s);
+          if (t < nb)
+            P.blocks.push_back(BlockTask{s, t, t * kTileEdge, std::min(nc, (t + nq) * kTileEdge),
+                                         std::min(nq, nb - t), flag0[s], 0, 0});
+        }
       for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s);
-        if (t * kTileEdge < nc)
-          P.blocks.push_back(BlockTask{s, t, t * kTileEdge, std::min(nc, (t + 1) * kTileEdge), 1, flag0[s], 0, 0});
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+        for (int32_t r = nc; r < m; r += nq * kTileEdge)
+          P.blocks.push_back(BlockTask{s, nblocks(s), r, std::min(m, r + nq * kTileEdge), 0, flag0[s], 0, 0});
       }
-    for (int32_t s : big) {
-      const int32_t nc = frontCols(S, s), m = frontRows(S, s);
-      for (int32_t r = nc; r < m; r += kTileEdge)
-        P.blocks.push_back(BlockTask{s, nblocks(s), r, std::min(m, r + kTileEdge), 0, flag0[s], 0, 0});
-    }
-    push(kSolveFwdFlow, L, bf, int64_t(P.blocks.size()) - bf);
+    });
   }
   // backward, root first; blocks of each supernode from the last one down
   for (int32_t L = F.nlevels - 1; L >= 0; L--) {
@@ -415,7 +439,7 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
     // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk); y[below] is read through
     // the row structure, so no gather launch precedes them
     std::vector<int32_t> p0(ns, 0), np(ns, 0);
-    int64_t bf = int64_t(P.blocks.size());
+    const int64_t bf = int64_t(P.blocks.size());
     for (int32_t s : big) {
       const int32_t nc = frontCols(S, s), mu = frontRows(S, s) - nc;
       np[s] = (mu + kInitRows - 1) / kInitRows;
@@ -426,18 +450,18 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F) {
                                        int32_t(P.initSlots++), 1});
     }
     push(kSolveBwdInit, L, bf, int64_t(P.blocks.size()) - bf);
-    bf = int64_t(P.blocks.size());
     int32_t maxBlk = 0;
     for (int32_t s : big) maxBlk = std::max(maxBlk, nblocks(s));
-    for (int32_t i = 0; i < maxBlk; i++)
-      for (int32_t s : big) {
-        const int32_t nc = frontCols(S, s), nb = nblocks(s);
-        if (i >= nb) continue;
-        const int32_t b = nb - 1 - i;
-        P.blocks.push_back(BlockTask{s, b, b * kTileEdge, std::min(nc, (b + 1) * kTileEdge), 1, flag0[s],
-                                     p0[s] + b * np[s], np[s]});
-      }
-    push(kSolveBwdFlow, L, bf, int64_t(P.blocks.size()) - bf);
+    pushFlow(kSolveBwdFlow, L, [&](int32_t nq) {
+      for (int32_t i = 0; i < maxBlk; i += nq)
+        for (int32_t s : big) {
+          const int32_t nc = frontCols(S, s), nb = nblocks(s);
+          if (i >= nb) continue;
+          const int32_t hi = nb - i, lo = std::max(0, hi - nq);  // blocks [lo, hi)
+          P.blocks.push_back(BlockTask{s, lo, lo * kTileEdge, std::min(nc, hi * kTileEdge), hi - lo, flag0[s],
+                                       p0[s] + lo * np[s], np[s]});
+        }
+    });
   }
   return P;
 }

@@ -106,6 +106,8 @@ struct FactorPlan {
   int32_t signals = 0;  // launches with kSyncSignal
 };
 
+constexpr int kFlowWideMinTasks = 160;  // default of PlanParams::flowWideMin (see the solve plan below)
+
 // Thresholds that pick kernels by supernode shape; the defaults are the constants above, and a device autotuner may
 // choose others (each at most its constant, which sizes the kernels' static storage).
 struct PlanParams {
@@ -114,6 +116,8 @@ struct PlanParams {
                                       // fronts of at most kWaveFrontMax rows take one wave each regardless
   int32_t wideLaunch = kWideLaunch;   // tiles a product launch needs to use kGemmEdge tiles
   int32_t superPanel = kSuperPanel;   // columns of a super-panel (a multiple of kTileEdge)
+  int32_t flowWideMin = kFlowWideMinTasks;  // wide tasks a forward solve launch needs to use them (else one block per
+                                            // task)
 };
 
 FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams());
@@ -130,11 +134,23 @@ constexpr int kFlowRhs = 8;       // right-hand sides per pass of the tiled-supe
 constexpr int kSolveWaveMax = 64;  // whole-front supernodes with m <= this are solved by one wave each
 constexpr int kInitRows = 256;    // below rows per task of the backward z = -L21^T y[below] partial products
 
-// One task of a tiled-supernode solve launch. Forward: rows [r0, r1) of supernode s after consuming the solutions of
-// blocks 0 .. blk - 1; an own-column tile (writer = 1) then solves block blk itself and raises its flag. Backward:
-// column block blk (columns [r0, r1)), consuming blocks blk + 1 .. nb - 1 and the np partial products at slots
-// p0 .. p0 + np - 1. Backward initialisation: partial product of column block blk over below rows [r0, r1) into slot
-// p0. `flag0` is the supernode's first flag.
+// Blocks per task of the tiled-supernode solve launches: a task has one 256-thread quarter per 64-row (forward) or
+// 64-column (backward) block. Wide tasks pass their blocks from one quarter to the next without a round trip through
+// device memory, but stream a wider panel per workgroup: they are used for forward launches with many tasks (levels of
+// many small fronts), where the round trips dominate; launches with few tasks (the large fronts near the root) keep one
+// block per task, where the streaming per workgroup does. A wide task has four quarters in the kernels of FP32 factors
+// and two in those of FP64 factors (twice the registers per thread); the plan holds a task list for each precision.
+constexpr int kFlowQuadsF32 = 4, kFlowQuadsF64 = 2;
+
+// One task of a tiled-supernode solve launch; `flag0` is the supernode's first flag.
+// Forward: rows [r0, r1) of supernode s, 64 rows per quarter. A chain task (writer = w >= 1) owns the blocks
+// blk .. blk + w - 1 (r0 = 64 blk, r1 = min(nc, 64 (blk + w))): it consumes the solutions of blocks 0 .. blk - 1, then
+// solves its blocks in order, raising each one's flag. A below task (writer = 0, blk = nb) takes below rows and
+// consumes every block.
+// Backward: a chain task owns the column blocks blk .. blk + w - 1 (columns [r0, r1)): it consumes blocks
+// blk + w .. nb - 1 and, for its block blk + i, the np partial products at slots p0 + i np .. p0 + i np + np - 1,
+// then solves its blocks from the last one down. Backward initialisation: partial product of column block blk over
+// below rows [r0, r1) into slot p0.
 struct BlockTask {
   int32_t s, blk;
   int32_t r0, r1;
@@ -156,9 +172,12 @@ enum SolveKind : int32_t {
 struct SolveLaunch {
   SolveKind kind;
   int32_t level;
-  int64_t first, count;  // small / gather: range in `fronts`; flow kinds: range in `blocks`
+  int64_t first, count;  // small / gather: range in `fronts`; flow kinds: range in `blocks` for FP32 factors; init:
+                         // range in `blocks` of the initialisation tasks
   int32_t maxM;          // small: largest front height; gather: largest m, which sets the number of row slices per
                          // front
+  int64_t first2 = 0, count2 = 0;  // flow kinds: range in `blocks` of the same launch for FP64 factors
+  int32_t nq = 1, nq2 = 1;          // flow kinds: blocks per task of the two lists (1, or kFlowQuadsF32 / kFlowQuadsF64)
 };
 
 struct SolvePlan {
@@ -170,6 +189,6 @@ struct SolvePlan {
   std::vector<SolveLaunch> launches;
 };
 
-SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F);
+SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F, const PlanParams& prm = PlanParams());
 
 }  // namespace amp

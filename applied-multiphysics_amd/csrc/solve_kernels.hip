@@ -274,12 +274,6 @@ __device__ inline int32_t flowRun(const int32_t* flags, int32_t k, int32_t lim, 
   return s_r;
 }
 
-// Raise a block's flag after the workgroup's mailbox stores (relaxed: consumers validate the words themselves).
-__device__ inline void flowPublish(int32_t* flag, int32_t val) {
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Mailbox word index of value r (0..63) of right-hand side jj of the block with flag `flag` (see FlowCtl).
 __device__ inline int64_t boxSlot(int32_t flag, int32_t jj, int32_t r) {
   return (int64_t(flag) * kFlowRhs + jj) * 64 + r;
@@ -327,76 +321,93 @@ __device__ inline double boxLoad<double>(const uint64_t* box, int64_t slot, int3
 
 }  // namespace
 
-// Forward solve of the tiled supernodes of one level. A task owns up to 64 rows r of supernode s; thread (row, g)
-// accumulates sum_k L[r, block k] x_k over the 16 columns 16 g .. 16 g + 15 of each consumed block. L values are
-// loaded one block ahead; the consumed x blocks are staged in LDS a run of complete blocks at a time. Below tiles then
-// store v_below - sum into the front vector; own-column tiles form v_blk - sum, apply the block's inverted diagonal
-// block (x = W v) and publish x into y. Q is the number of right-hand sides the kernel is built for (q <= Q).
-template <typename T, typename S, int Q>
+// Forward solve of the tiled supernodes of one level. A task has NQ quarters of 256 threads, each owning 64 rows
+// (r0 + 64 qd ..); thread (wave w, lane) of a quarter holds row 16 w + (lane & 15) and the columns (lane >> 4) + 4 i of
+// every consumed 64-column block, so a row sum is completed by two lane shuffles. The x blocks are staged in LDS once
+// for all quarters, a run of complete blocks at a time, and a thread's L values are loaded one block ahead. Below
+// tasks (writer == 0) store v_below - sum into the front vector. A chain task owns the `writer` blocks blk, blk + 1,
+// ...: after the blocks before blk, quarter i forms v = v_own - sum for block blk + i, applies the block's inverted
+// diagonal block (x = W v), publishes x (y, mailbox, flag) and hands it to the later quarters through LDS, which apply
+// it to their rows before the next quarter's turn: inside a task, a block passes to the next without a round trip
+// through device memory. Q is the number of right-hand sides the kernel is built for (q <= Q).
+template <typename T, typename S, int Q, int NQ>
 __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, S* fv, int32_t q,
-                                        int64_t fvN, const FlowCtl& c, int32_t want, S (&part)[4][Q][kTileEdge],
-                                        S (&xs)[Q][kFlowRun * kTileEdge]) {
+                                        int64_t fvN, const FlowCtl& c, int32_t want, S (&xs)[Q][kFlowRun * kTileEdge],
+                                        S (&vh)[Q][kTileEdge]) {
   constexpr int E = kTileEdge;
-  const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
+  const int tid = threadIdx.x, qd = tid >> 8, lane = tid & 63;
+  const int rq = 16 * ((tid >> 6) & 3) + (lane & 15), cp = lane >> 4;  // row in the quarter, column phase
   const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const T* L = a.Lx + a.Lxp[s];
-  const int32_t r = t.r0 + rr;
+  const int32_t r = t.r0 + E * qd + rq;
   const bool live = r < t.r1;
-  const T* Lr = L + (live ? r : t.r0) + int64_t(16 * g) * m;
+  const T* Lr = a.Lx + a.Lxp[s] + (live ? r : t.r0) + int64_t(cp) * m;
   const int32_t nb = (nc + E - 1) / E;
-  // a below tile of a single-block supernode forms x_0 = W_0 v_own itself instead of waiting for it
-  const bool local0 = !t.writer && nb == 1;
-  T wv[16];
-  if (t.writer || local0) {
-    const T* Wt = a.W + a.Woff[s] + int64_t(t.writer ? t.blk : 0) * E * E;
+  const bool own = qd < t.writer;            // the quarter solves block blk + qd
+  const bool local0 = !t.writer && nb == 1;  // a below task of a single-block supernode forms x_0 = W_0 v_own itself
+  T wv[16];  // row rq of the quarter's inverted diagonal block (quarter 0's W_0 for local0), columns cp + 4 i
+  if (own || (local0 && qd == 0)) {
+    const T* Wt = a.W + a.Woff[s] + int64_t(own ? t.blk + qd : 0) * E * E;
 #pragma unroll
-    for (int j = 0; j < 16; j++) wv[j] = Wt[rr + (16 * g + j) * E];
+    for (int i = 0; i < 16; i++) wv[i] = Wt[rq + (cp + 4 * i) * E];
   }
-  S acc[Q], vown[Q];  // vown: this row's front-vector value, read before any wait (the gather wrote it earlier)
+  S acc[Q], vown[Q];  // acc: the thread's column phase of the row sums; vown: the row's front-vector value (read
+                      // before any wait: the gather wrote it earlier)
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) {
     acc[jj] = S(0);
-    vown[jj] = (tid < E && live && jj < q) ? fv[jj * fvN + a.Sp[s] + r] : S(0);
+    vown[jj] = (cp == 0 && live && jj < q) ? fv[jj * fvN + a.Sp[s] + r] : S(0);
   }
-  const int32_t steps = t.blk;
+  // blocks the row consumes: those before blk (all of them for a below task), then in a chain task the task's own
+  // blocks before the quarter's
+  const int32_t steps = t.writer ? t.blk : nb;
+  const int32_t total = steps + (own ? qd : 0);
   T cur[16], nxt[16];
   auto loadL = [&](T* dst, int32_t k) {
-    const int32_t cb = k * E + 16 * g;
+    const int32_t cb = k * E + cp;
 #pragma unroll
-    for (int j = 0; j < 16; j++) dst[j] = (live && cb + j < nc) ? Lr[int64_t(k * E + j) * m] : T(0);
+    for (int i = 0; i < 16; i++) dst[i] = (live && cb + 4 * i < nc) ? Lr[int64_t(k * E + 4 * i) * m] : T(0);
   };
-  if (steps > 0) loadL(cur, 0);
-  if (local0) {
-    if (tid < E)
-      for (int jj = 0; jj < q; jj++) xs[jj][E + rr] = rr < nc ? fv[jj * fvN + a.Sp[s] + rr] : S(0);
-    __syncthreads();
-#pragma unroll
-    for (int jj = 0; jj < Q; jj++) {
-      S sum = S(0);
-      if (jj < q) {
-#pragma unroll
-        for (int j = 0; j < 16; j++) sum += S(wv[j]) * xs[jj][E + 16 * g + j];
-      }
-      part[g][jj][rr] = sum;
-    }
-    __syncthreads();
-    if (tid < E)
-      for (int jj = 0; jj < q; jj++) xs[jj][rr] = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
-    __syncthreads();
+  // acc += L[r, block] x_block for x in LDS at xb (column cp + 4 i of the block at xb[cp + 4 i])
+  auto apply = [&](const T* lv, int xo) {
 #pragma unroll
     for (int jj = 0; jj < Q; jj++) {
       if (jj < q) {
         S sum = S(0);
 #pragma unroll
-        for (int j = 0; j < 16; j++) sum += S(cur[j]) * xs[jj][16 * g + j];
+        for (int i = 0; i < 16; i++) sum += S(lv[i]) * xs[jj][xo + 4 * i];
         acc[jj] += sum;
       }
     }
+  };
+  // sum over the four column phases of a row (lanes 16 apart): every lane gets the row's total
+  auto rowTotal = [](S v) {
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+  };
+  if (total > 0) loadL(cur, 0);
+  if (local0) {
+    if (tid < E)
+      for (int jj = 0; jj < q; jj++) vh[jj][tid] = tid < nc ? fv[jj * fvN + a.Sp[s] + tid] : S(0);
     __syncthreads();
+    if (qd == 0) {
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        S sum = S(0);
+        if (jj < q) {
+#pragma unroll
+          for (int i = 0; i < 16; i++) sum += S(wv[i]) * vh[jj][cp + 4 * i];
+        }
+        sum = rowTotal(sum);
+        if (cp == 0) xs[jj][rq] = (jj < q && rq < nc) ? sum : S(0);
+      }
+    }
+    __syncthreads();
+    apply(cur, cp);
   }
   for (int32_t k0 = 0; k0 < (local0 ? 0 : steps);) {
     const int32_t k1 = flowRun(c.fwdFlag + t.flag0, k0, min(steps, k0 + kFlowRun), want);
-    for (int idx = tid; idx < (k1 - k0) * E; idx += 256) {
+    for (int idx = tid; idx < (k1 - k0) * E; idx += 256 * NQ) {
       const int32_t col = k0 * E + idx;
 #pragma unroll
       for (int jj = 0; jj < Q; jj++)
@@ -406,70 +417,75 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
     }
     __syncthreads();
     for (int32_t k = k0; k < k1; k++) {
-      if (k + 1 < steps) loadL(nxt, k + 1);
-      const int xo = (k - k0) * E + 16 * g;
+      if (k + 1 < total) loadL(nxt, k + 1);
+      apply(cur, (k - k0) * E + cp);
 #pragma unroll
-      for (int jj = 0; jj < Q; jj++) {
-        if (jj < q) {
-          S sum = S(0);
-#pragma unroll
-          for (int j = 0; j < 16; j++) sum += S(cur[j]) * xs[jj][xo + j];
-          acc[jj] += sum;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 16; j++) cur[j] = nxt[j];
+      for (int i = 0; i < 16; i++) cur[i] = nxt[i];
     }
     k0 = k1;
-    if (k0 < steps) __syncthreads();  // the next run is staged into xs (after the last run, part is written first)
+    if (k0 < steps) __syncthreads();  // the next run is staged into xs
   }
-#pragma unroll
-  for (int jj = 0; jj < Q; jj++) part[g][jj][rr] = acc[jj];
-  __syncthreads();
-  S* vs = &xs[0][0];  // reused: v of the own block, Q x 64
-  if (tid < E) {
+  if (!t.writer) {
 #pragma unroll
     for (int jj = 0; jj < Q; jj++) {
-      if (jj >= q) continue;
-      S v = S(0);
-      if (live) {
-        v = vown[jj] - ((part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]));
-        if (!t.writer) fv[jj * fvN + a.Sp[s] + r] = v;
+      const S v = rowTotal(acc[jj]);
+      if (jj < q && cp == 0 && live) fv[jj * fvN + a.Sp[s] + r] = vown[jj] - v;
+    }
+    return;
+  }
+  // the task's blocks in order: quarter i solves block blk + i, the later quarters apply it
+  for (int32_t i = 0; i < t.writer; i++) {
+    const int32_t kb = t.blk + i;
+    if (qd == i) {
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        const S v = rowTotal(acc[jj]);
+        if (cp == 0) vh[jj][rq] = (jj < q && live) ? vown[jj] - v : S(0);
       }
-      vs[jj * E + rr] = v;
+    }
+    __syncthreads();  // v of block kb in vh (the previous hand-over in xs has been applied)
+    if (qd == i) {
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        S sum = S(0);
+        if (jj < q) {
+#pragma unroll
+          for (int i2 = 0; i2 < 16; i2++) sum += S(wv[i2]) * vh[jj][cp + 4 * i2];
+        }
+        sum = rowTotal(sum);
+        if (cp == 0) {
+          xs[jj][rq] = (jj < q && live) ? sum : S(0);  // hand-over to the later quarters
+          if (jj < q && live) {
+            y[int64_t(jj) * a.n + f + r] = sum;
+            if (nb > 1) boxStore(c.fwdBox, boxSlot(t.flag0 + kb, jj, rq), want, sum);
+          }
+        }
+      }
+    }
+    __syncthreads();  // x of block kb in xs[.][0, 64); its mailbox words are stored
+    // raise the block's flag (relaxed: consumers validate the mailbox words themselves); single blocks have no waiters
+    if (nb > 1 && tid == 256 * i)
+      __hip_atomic_store(c.fwdFlag + t.flag0 + kb, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (qd > i && qd < t.writer) {
+      if (steps + i + 1 < total) loadL(nxt, steps + i + 1);
+      apply(cur, cp);
+#pragma unroll
+      for (int i2 = 0; i2 < 16; i2++) cur[i2] = nxt[i2];
     }
   }
-  if (!t.writer) return;
-  __syncthreads();
-#pragma unroll
-  for (int jj = 0; jj < Q; jj++) {
-    S sum = S(0);
-    if (jj < q) {
-#pragma unroll
-      for (int j = 0; j < 16; j++) sum += S(wv[j]) * vs[jj * E + 16 * g + j];
-    }
-    part[g][jj][rr] = sum;
-  }
-  __syncthreads();
-  if (tid < E && live)
-    for (int jj = 0; jj < q; jj++) {
-      const S xv = (part[0][jj][rr] + part[1][jj][rr]) + (part[2][jj][rr] + part[3][jj][rr]);
-      y[int64_t(jj) * a.n + f + r] = xv;
-      if (nb > 1) boxStore(c.fwdBox, boxSlot(t.flag0 + t.blk, jj, rr), want, xv);
-    }
-  if (nb > 1) flowPublish(c.fwdFlag + t.flag0 + t.blk, want);  // single blocks have no waiters
 }
 
 // Persistent form: workgroups take tasks in list order (nextTask). Every task waits only on tasks listed before it, so
 // the lowest unfinished task never waits on an unfinished one and is being run: the launch always drains.
-template <typename T, typename S, int Q>
-__global__ __launch_bounds__(256) void k_solve_fwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
-                                                        int32_t* ctr, SolveArgs<T> a, S* y, S* fv, int32_t q,
-                                                        int64_t fvN, FlowCtl c) {
-  __shared__ S part[4][Q][kTileEdge];
+template <typename T, typename S, int Q, int NQ>
+__global__ __launch_bounds__(256 * NQ) void k_solve_fwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
+                                                             int32_t* ctr, SolveArgs<T> a, S* y, S* fv, int32_t q,
+                                                             int64_t fvN, FlowCtl c) {
   __shared__ S xs[Q][kFlowRun * kTileEdge];
+  __shared__ S vh[Q][kTileEdge];
   const int32_t want = *c.epoch;  // this pass's number (raised by the kernel that starts the pass)
-  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr)) fwdTask<T, S, Q>(tasks[i], a, y, fv, q, fvN, c, want, part, xs);
+  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr))
+    fwdTask<T, S, Q, NQ>(tasks[i], a, y, fv, q, fvN, c, want, xs, vh);
 }
 
 template <typename T, typename S, int Q>
@@ -641,25 +657,29 @@ __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restr
   }
 }
 
-// Backward solve of the tiled supernodes of one level. A task owns column block blk of supernode s; wave w handles
-// its columns 16 w .. 16 w + 15 (lane & 15) and lane >> 4 selects every fourth row, so each load instruction reads
-// four consecutive rows of 16 columns. z starts from y_own minus the block's partial products of L21^T y[below]
-// (summed in slot order before any wait); then z -= L[block k, blk]^T x_k for the blocks k > blk, a run of complete blocks at a time
-// (x staged in LDS, L loaded one block ahead), and x = W^T z is published.
-template <typename T, typename S, int Q>
+// Backward solve of the tiled supernodes of one level. A task has NQ quarters of 256 threads, quarter qd owning column
+// block blk + qd; wave w of a quarter handles the columns 16 w .. 16 w + 15 (lane & 15) and lane >> 4 selects every
+// fourth row, so each load instruction reads four consecutive rows of 16 columns. z starts from y_own minus the
+// block's partial products of L21^T y[below] (summed in slot order before any wait); then z -= L[block k, column
+// block]^T x_k for the blocks k after the task's (x staged in LDS once for all quarters, a run of complete blocks at a
+// time, L loaded one block ahead). The task's blocks are then solved from the last one down: quarter i forms z,
+// applies its block's transposed inverted diagonal block (x = W^T z), publishes x and hands it to the earlier quarters
+// through LDS, which apply it to their columns before the next quarter's turn.
+template <typename T, typename S, int Q, int NQ>
 __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, const S* pz, int32_t q,
                                         const FlowCtl& c, int32_t want, S (&xs)[Q][kFlowRun * kTileEdge],
                                         S (&zs)[Q][kTileEdge]) {
   constexpr int E = kTileEdge;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int cj = 16 * w + (lane & 15), rs = lane >> 4;
+  const int tid = threadIdx.x, qd = tid >> 8, lane = tid & 63;
+  const int cj = 16 * ((tid >> 6) & 3) + (lane & 15), rs = lane >> 4;
   const int32_t s = t.s, f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t nb = (nc + E - 1) / E, c0 = t.blk * E, col = c0 + cj;
-  const bool live = col < nc;
-  const T* Lc = a.Lx + a.Lxp[s] + int64_t(live ? col : c0) * m;
+  const int32_t nb = (nc + E - 1) / E, cb = t.blk + qd, col = cb * E + cj;
+  const bool own = qd < t.writer;
+  const bool live = own && col < nc;
+  const T* Lc = a.Lx + a.Lxp[s] + int64_t(live ? col : t.r0) * m;
   T wv[16];
-  {
-    const T* Wt = a.W + a.Woff[s] + int64_t(t.blk) * E * E;
+  if (own) {
+    const T* Wt = a.W + a.Woff[s] + int64_t(cb) * E * E;
 #pragma unroll
     for (int i = 0; i < 16; i++) wv[i] = Wt[(rs + 4 * i) + cj * E];
   }
@@ -668,8 +688,8 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) {
     S below = S(0);
-    if (jj < q && rs == 0)
-      for (int32_t p = 0; p < t.np; p++) below += pz[(int64_t(t.p0 + p) * q + jj) * E + cj];
+    if (own && jj < q && rs == 0)
+      for (int32_t p = 0; p < t.np; p++) below += pz[(int64_t(t.p0 + qd * t.np + p) * q + jj) * E + cj];
     acc[jj] = below;
     yown[jj] = (jj < q && rs == 0 && live) ? y[int64_t(jj) * a.n + f + col] : S(0);
   }
@@ -679,12 +699,31 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
     for (int i = 0; i < 16; i++) dst[i] = (live && rb + 4 * i < nc) ? Lc[rb + 4 * i] : T(0);
   };
-  const int32_t kEnd = t.blk;  // blocks nb - 1 down to blk + 1
+  // acc += L[block, col]^T x_block for x in LDS at xs[.][xo + 4 i] (row rs + 4 i of the block)
+  auto apply = [&](const T* lv, int xo) {
+#pragma unroll
+    for (int jj = 0; jj < Q; jj++) {
+      if (jj < q) {
+        S sum = S(0);
+#pragma unroll
+        for (int i = 0; i < 16; i++) sum += S(lv[i]) * xs[jj][xo + 4 * i];
+        acc[jj] += sum;
+      }
+    }
+  };
+  // sum over the four row phases of a column (lanes 16 apart): every lane gets the column's total
+  auto colTotal = [](S v) {
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+  };
+  const int32_t kExt = t.blk + t.writer - 1;  // blocks after kExt are solved by earlier tasks
+  const int32_t kEnd = own ? cb : nb - 1;     // the quarter's column block consumes the blocks after kEnd
   if (nb - 1 > kEnd) loadL(cur, nb - 1);
-  for (int32_t k0 = nb - 1; k0 > kEnd;) {
-    const int32_t k1 = flowRun(c.bwdFlag + t.flag0, k0, max(kEnd, k0 - kFlowRun), want);
+  for (int32_t k0 = nb - 1; k0 > kExt;) {
+    const int32_t k1 = flowRun(c.bwdFlag + t.flag0, k0, max(kExt, k0 - kFlowRun), want);
     // blocks k0 down to k1 + 1 are complete; stage their x (block k at slot k0 - k)
-    for (int idx = tid; idx < (k0 - k1) * E; idx += 256) {
+    for (int idx = tid; idx < (k0 - k1) * E; idx += 256 * NQ) {
       const int32_t row = (k0 - idx / E) * E + (idx % E);
 #pragma unroll
       for (int jj = 0; jj < Q; jj++)
@@ -695,65 +734,71 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
     __syncthreads();
     for (int32_t k = k0; k > k1; k--) {
       if (k - 1 > kEnd) loadL(nxt, k - 1);
-      const int xo = (k0 - k) * E + rs;
-#pragma unroll
-      for (int jj = 0; jj < Q; jj++) {
-        if (jj < q) {
-          S sum = S(0);
-#pragma unroll
-          for (int i = 0; i < 16; i++) sum += S(cur[i]) * xs[jj][xo + 4 * i];
-          acc[jj] += sum;
-        }
-      }
+      apply(cur, (k0 - k) * E + rs);
 #pragma unroll
       for (int i = 0; i < 16; i++) cur[i] = nxt[i];
     }
     k0 = k1;
-    if (k0 > kEnd) __syncthreads();  // the next run is staged into xs (the tail uses zs only)
+    if (k0 > kExt) __syncthreads();  // the next run is staged into xs
   }
+  // the task's blocks from the last one down: quarter i solves block blk + i, the earlier quarters apply it
+  for (int32_t i = t.writer - 1; i >= 0; i--) {
+    const int32_t kb = t.blk + i;
+    if (qd == i) {
 #pragma unroll
-  for (int jj = 0; jj < Q; jj++) {
-    acc[jj] += __shfl_xor(acc[jj], 16, 64);
-    acc[jj] += __shfl_xor(acc[jj], 32, 64);
-  }
-  if (rs == 0) {
-#pragma unroll
-    for (int jj = 0; jj < Q; jj++)
-      if (jj < q) zs[jj][cj] = live ? yown[jj] - acc[jj] : S(0);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int jj = 0; jj < Q; jj++) {
-    S sum = S(0);
-    if (jj < q) {
-#pragma unroll
-      for (int i = 0; i < 16; i++) sum += S(wv[i]) * zs[jj][rs + 4 * i];
+      for (int jj = 0; jj < Q; jj++) {
+        const S v = colTotal(acc[jj]);
+        if (rs == 0) zs[jj][cj] = (jj < q && live) ? yown[jj] - v : S(0);
+      }
     }
-    sum += __shfl_xor(sum, 16, 64);
-    sum += __shfl_xor(sum, 32, 64);
-    if (jj < q && rs == 0 && live) {
-      y[int64_t(jj) * a.n + f + col] = sum;
-      if (t.blk > 0) boxStore(c.bwdBox, boxSlot(t.flag0 + t.blk, jj, cj), want, sum);
+    __syncthreads();  // z of block kb in zs (the previous hand-over in xs has been applied)
+    if (qd == i) {
+#pragma unroll
+      for (int jj = 0; jj < Q; jj++) {
+        S sum = S(0);
+        if (jj < q) {
+#pragma unroll
+          for (int i2 = 0; i2 < 16; i2++) sum += S(wv[i2]) * zs[jj][rs + 4 * i2];
+        }
+        sum = colTotal(sum);
+        if (rs == 0) {
+          xs[jj][cj] = (jj < q && live) ? sum : S(0);  // hand-over to the earlier quarters
+          if (jj < q && live) {
+            y[int64_t(jj) * a.n + f + col] = sum;
+            if (kb > 0) boxStore(c.bwdBox, boxSlot(t.flag0 + kb, jj, cj), want, sum);
+          }
+        }
+      }
+    }
+    __syncthreads();  // x of block kb in xs[.][0, 64); its mailbox words are stored
+    // raise the block's flag (relaxed, as forward); block 0 has no waiters
+    if (kb > 0 && tid == 256 * i)
+      __hip_atomic_store(c.bwdFlag + t.flag0 + kb, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (qd < i) {
+      if (kb - 1 > kEnd) loadL(nxt, kb - 1);
+      apply(cur, rs);
+#pragma unroll
+      for (int i2 = 0; i2 < 16; i2++) cur[i2] = nxt[i2];
     }
   }
-  if (t.blk > 0) flowPublish(c.bwdFlag + t.flag0 + t.blk, want);  // block 0 has no waiters
 }
 
 // Persistent form, as k_solve_fwd_flow.
-template <typename T, typename S, int Q>
-__global__ __launch_bounds__(256) void k_solve_bwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
-                                                        int32_t* ctr, SolveArgs<T> a, S* y, const S* pz, int32_t q,
-                                                        FlowCtl c) {
+template <typename T, typename S, int Q, int NQ>
+__global__ __launch_bounds__(256 * NQ) void k_solve_bwd_flow(const BlockTask* __restrict__ tasks, int32_t ntasks,
+                                                             int32_t* ctr, SolveArgs<T> a, S* y, const S* pz,
+                                                             int32_t q, FlowCtl c) {
   __shared__ S xs[Q][kFlowRun * kTileEdge];
   __shared__ S zs[Q][kTileEdge];
   const int32_t want = *c.epoch;
-  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr)) bwdTask<T, S, Q>(tasks[i], a, y, pz, q, c, want, xs, zs);
+  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr))
+    bwdTask<T, S, Q, NQ>(tasks[i], a, y, pz, q, c, want, xs, zs);
 }
 
 namespace {
 
-// Co-resident workgroups of 256 threads for `kernel` on device `dev` (cached per kernel and device).
-int residentGroups(const void* kernel, int dev) {
+// Co-resident workgroups of `threads` threads for `kernel` on device `dev` (cached per kernel and device).
+int residentGroups(const void* kernel, int threads, int dev) {
   static std::mutex mu;
   static std::map<std::pair<const void*, int>, int> cache;
   std::lock_guard<std::mutex> lock(mu);
@@ -761,23 +806,23 @@ int residentGroups(const void* kernel, int dev) {
   if (it != cache.end()) return it->second;
   int cus = 0, perCu = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, threads, 0) != hipSuccess)
     return 0;
   return cache[{kernel, dev}] = cus * perCu;
 }
 
-// Launch of a persistent flow kernel over `ntasks` tasks with at most the co-resident number of workgroups (more
-// would only wait for a slot); an ordinary launch on the pass's stream, so it shares the hardware queue of the pass's
-// other kernels.
-template <typename K, typename... Args>
+// Launch of a persistent flow kernel (NQ quarters of 256 threads per workgroup) over `ntasks` tasks with at most the
+// co-resident number of workgroups (more would only wait for a slot); an ordinary launch on the pass's stream, so it
+// shares the hardware queue of the pass's other kernels.
+template <int NQ, typename K, typename... Args>
 hipError_t launchFlow(K kernel, int64_t ntasks, hipStream_t st, const BlockTask* tasks, int32_t* ctr, Args... args) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  const int cap = residentGroups((const void*)kernel, dev);
+  const int cap = residentGroups((const void*)kernel, 256 * NQ, dev);
   if (cap <= 0) return hipErrorInvalidConfiguration;
   const unsigned grid = unsigned(std::min<int64_t>(ntasks, cap));
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, st, tasks, int32_t(ntasks), ctr, args...);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(256 * NQ), 0, st, tasks, int32_t(ntasks), ctr, args...);
   return hipGetLastError();
 }
 
@@ -823,12 +868,23 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
         hipLaunchKernelGGL((k_solve_gather<T, S>), dim3(grid, unsigned((L.maxM + kGatherRows - 1) / kGatherRows)),
                            dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
-      case kSolveFwdFlow:
-        err = q == 1 ? launchFlow(k_solve_fwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++, a,
-                                  y, fv, q, dp.fvEntries, fc)
-                     : launchFlow(k_solve_fwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++,
-                                  a, y, fv, q, dp.fvEntries, fc);
+      case kSolveFwdFlow: {
+        // the launch's task list for this factor precision, with its blocks per task
+        constexpr int NQ = sizeof(T) == 4 ? kFlowQuadsF32 : kFlowQuadsF64;
+        const BlockTask* tk = dp.blocks + (NQ == kFlowQuadsF32 ? L.first : L.first2);
+        const int64_t nt = NQ == kFlowQuadsF32 ? L.count : L.count2;
+        const bool wide = (NQ == kFlowQuadsF32 ? L.nq : L.nq2) == NQ;
+        int32_t* ctr = fc.ctr + nflow++;
+        if (wide)
+          err = q == 1 ? launchFlow<NQ>(k_solve_fwd_flow<T, S, 1, NQ>, nt, st, tk, ctr, a, y, fv, q, dp.fvEntries, fc)
+                       : launchFlow<NQ>(k_solve_fwd_flow<T, S, kFlowRhs, NQ>, nt, st, tk, ctr, a, y, fv, q,
+                                        dp.fvEntries, fc);
+        else
+          err = q == 1 ? launchFlow<1>(k_solve_fwd_flow<T, S, 1, 1>, nt, st, tk, ctr, a, y, fv, q, dp.fvEntries, fc)
+                       : launchFlow<1>(k_solve_fwd_flow<T, S, kFlowRhs, 1>, nt, st, tk, ctr, a, y, fv, q, dp.fvEntries,
+                                       fc);
         break;
+      }
       case kSolveSmallBwd:
         if (L.maxM <= kSolveWaveMax) {
           const unsigned wg = unsigned((L.count + 3) / 4);
@@ -855,12 +911,15 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
           hipLaunchKernelGGL((k_solve_bwd_init<T, S, kFlowRhs>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a,
                              y, pz, q);
         break;
-      case kSolveBwdFlow:
-        err = q == 1 ? launchFlow(k_solve_bwd_flow<T, S, 1>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++, a,
-                                  y, pz, q, fc)
-                     : launchFlow(k_solve_bwd_flow<T, S, kFlowRhs>, L.count, st, dp.blocks + L.first, fc.ctr + nflow++,
-                                  a, y, pz, q, fc);
+      case kSolveBwdFlow: {
+        // backward launches keep one block per task (their column loads make a wide task's panel slow to stream)
+        const BlockTask* tk = dp.blocks + (sizeof(T) == 4 ? L.first : L.first2);
+        const int64_t nt = sizeof(T) == 4 ? L.count : L.count2;
+        int32_t* ctr = fc.ctr + nflow++;
+        err = q == 1 ? launchFlow<1>(k_solve_bwd_flow<T, S, 1, 1>, nt, st, tk, ctr, a, y, pz, q, fc)
+                     : launchFlow<1>(k_solve_bwd_flow<T, S, kFlowRhs, 1>, nt, st, tk, ctr, a, y, pz, q, fc);
         break;
+      }
     }
     if (rec) {
       (void)hipEventRecord(prof->events[2 * prof->used + 1], st);
@@ -893,8 +952,6 @@ template hipError_t launchSolvePlan<double, double>(const DeviceSolvePlan&, cons
                                               int32_t, double*, double*, double*, const FlowCtl&, hipStream_t, ProfileSink*);
 template hipError_t launchSolvePlan<float, float>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
                                             int32_t, float*, float*, float*, const FlowCtl&, hipStream_t, ProfileSink*);
-template hipError_t launchSolvePlan<float, double>(const DeviceSolvePlan&, const SolveArgs<float>&, const double*, double*,
-                                             int32_t, double*, double*, double*, const FlowCtl&, hipStream_t, ProfileSink*);
 
 void launchClearFlag(int32_t* flag, hipStream_t st) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, flag); }
 

@@ -164,13 +164,14 @@ struct Handle {
   FactorPlan F;
   SolvePlan SP;
   Ctx c;
+  int32_t flowWidth = kFlowQuadsF32;  // task width of the flow launches ps_solve simulates
 };
 
 }  // namespace
 
 extern "C" {
 
-// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel
+// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin
 void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax,
                        const int32_t* prm) {
   Handle* h = new Handle();
@@ -184,9 +185,10 @@ void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, 
     pp.smallNarrow = prm[1];
     pp.wideLaunch = prm[2];
     pp.superPanel = prm[3];
+    pp.flowWideMin = prm[4];
   }
   h->F = buildFactorPlan(h->S, pp);
-  h->SP = buildSolvePlan(h->S, h->F);
+  h->SP = buildSolvePlan(h->S, h->F, pp);
   return h;
 }
 
@@ -467,62 +469,98 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
   return bad;
 }
 
-// Ordering contract of the tiled-supernode solve launches (their workgroups wait on flags raised by earlier tasks of
-// the same launch, so an order violation would be a deadlock on the device): returns the number of violations.
+// Task list of a flow launch for the kernels of FP32 factors (list == kFlowQuadsF32) or FP64 factors, and its blocks
+// per task.
+static int32_t flowRange(const SolveLaunch& L, int32_t list, int64_t& first, int64_t& count) {
+  first = list == kFlowQuadsF32 ? L.first : L.first2;
+  count = list == kFlowQuadsF32 ? L.count : L.count2;
+  return list == kFlowQuadsF32 ? L.nq : L.nq2;
+}
+
+// Ordering contract of the tiled-supernode solve launches, for the task lists of both precisions (their workgroups wait on flags
+// raised by earlier tasks of the same launch, so an order violation would be a deadlock on the device): returns the
+// number of violations.
 int64_t ps_check_flow(void* p) {
   Handle* h = static_cast<Handle*>(p);
   const Symbolic& S = h->S;
   const SolvePlan& P = h->SP;
   int64_t bad = 0;
-  std::vector<int32_t> flag0(S.nsuper, -1);
-  std::vector<int8_t> seenFwd(size_t(P.flowFlags), 0), seenBwd(size_t(P.flowFlags), 0);
-  for (const SolveLaunch& L : P.launches) {
-    if (L.kind != kSolveFwdFlow && L.kind != kSolveBwdFlow) continue;
-    const bool fwd = L.kind == kSolveFwdFlow;
-    std::vector<int8_t>& seen = fwd ? seenFwd : seenBwd;
-    std::map<int32_t, std::vector<int8_t>> done;  // per supernode: blocks solved so far in this launch
-    for (int64_t t = L.first; t < L.first + L.count; t++) {
-      const BlockTask& b = P.blocks[t];
-      const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], nb = (nc + kTileEdge - 1) / kTileEdge;
-      if (flag0[b.s] == -1) flag0[b.s] = b.flag0;
-      if (flag0[b.s] != b.flag0 || b.flag0 < 0 || b.flag0 + nb > P.flowFlags) bad++;
-      auto& d = done[b.s];
-      d.resize(nb, 0);
-      if (fwd) {
-        for (int32_t k = 0; k < std::min(b.blk, nb); k++) bad += d[k] == 0;
-      } else {
-        for (int32_t k = b.blk + 1; k < nb; k++) bad += d[k] == 0;
-      }
-      if (b.writer) {
-        if (b.blk < 0 || b.blk >= nb || d[b.blk]) {
+  for (const int32_t nq : {kFlowQuadsF32, kFlowQuadsF64}) {
+    std::vector<int32_t> flag0(S.nsuper, -1);
+    std::vector<int8_t> seenFwd(size_t(P.flowFlags), 0), seenBwd(size_t(P.flowFlags), 0);
+    for (const SolveLaunch& L : P.launches) {
+      if (L.kind != kSolveFwdFlow && L.kind != kSolveBwdFlow) continue;
+      const bool fwd = L.kind == kSolveFwdFlow;
+      std::vector<int8_t>& seen = fwd ? seenFwd : seenBwd;
+      std::map<int32_t, std::vector<int8_t>> done;  // per supernode: blocks solved so far in this launch
+      int64_t first, count;
+      const int32_t width = flowRange(L, nq, first, count);
+      bad += width != 1 && (width != nq || !fwd);
+      for (int64_t t = first; t < first + count; t++) {
+        const BlockTask& b = P.blocks[t];
+        const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], nb = (nc + kTileEdge - 1) / kTileEdge;
+        const int32_t m = S.Sp[b.s + 1] - S.Sp[b.s];
+        if (flag0[b.s] == -1) flag0[b.s] = b.flag0;
+        if (flag0[b.s] != b.flag0 || b.flag0 < 0 || b.flag0 + nb > P.flowFlags) bad++;
+        auto& d = done[b.s];
+        d.resize(nb, 0);
+        if (b.writer < 0 || b.writer > width) {
           bad++;
           continue;
         }
-        d[b.blk] = 1;
-        seen[b.flag0 + b.blk]++;
+        if (!b.writer) {  // forward below task: every block, at most nq x 64 below rows
+          bad += !fwd || b.blk != nb || b.r0 < nc || b.r1 > m || b.r1 <= b.r0 || b.r1 - b.r0 > width * kTileEdge;
+          for (int32_t k = 0; k < nb; k++) bad += d[k] == 0;
+          continue;
+        }
+        const int32_t lo = b.blk, hi = b.blk + b.writer;  // the task's blocks [lo, hi)
+        if (lo < 0 || hi > nb || b.r0 != lo * kTileEdge || b.r1 != std::min(nc, hi * kTileEdge)) {
+          bad++;
+          continue;
+        }
+        if (fwd)
+          for (int32_t k = 0; k < lo; k++) bad += d[k] == 0;
+        else
+          for (int32_t k = hi; k < nb; k++) bad += d[k] == 0;
+        for (int32_t k = lo; k < hi; k++) {
+          bad += d[k] != 0;
+          d[k] = 1;
+          seen[b.flag0 + k]++;
+        }
       }
     }
-  }
-  for (int64_t i = 0; i < P.flowFlags; i++) bad += (seenFwd[i] != 1) + (seenBwd[i] != 1);
-  // every initialisation slot is written by exactly one task of an earlier launch than the flow task that reads it
-  std::vector<int32_t> written(size_t(P.initSlots), -1);
-  for (size_t li = 0; li < P.launches.size(); li++) {
-    const SolveLaunch& L = P.launches[li];
-    for (int64_t t = L.first; t < L.first + L.count; t++) {
-      const BlockTask& b = P.blocks[t];
+    for (int64_t i = 0; i < P.flowFlags; i++) bad += (seenFwd[i] != 1) + (seenBwd[i] != 1);
+    // every initialisation slot is written by exactly one task of an earlier launch than the flow task that reads it
+    std::vector<int32_t> written(size_t(P.initSlots), -1);
+    for (size_t li = 0; li < P.launches.size(); li++) {
+      const SolveLaunch& L = P.launches[li];
       if (L.kind == kSolveBwdInit) {
-        if (b.p0 < 0 || b.p0 >= P.initSlots || written[b.p0] != -1) bad++;
-        else written[b.p0] = int32_t(li);
+        for (int64_t t = L.first; t < L.first + L.count; t++) {
+          const BlockTask& b = P.blocks[t];
+          if (b.p0 < 0 || b.p0 >= P.initSlots || written[b.p0] != -1)
+            bad++;
+          else
+            written[b.p0] = int32_t(li);
+        }
       } else if (L.kind == kSolveBwdFlow) {
-        const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], mu = S.Sp[b.s + 1] - S.Sp[b.s] - nc;
-        if (b.np != (mu + kInitRows - 1) / kInitRows) bad++;
-        for (int32_t p = b.p0; p < b.p0 + b.np; p++) bad += p < 0 || p >= P.initSlots || written[p] < 0 || written[p] >= int32_t(li);
+        int64_t first, count;
+        flowRange(L, nq, first, count);
+        for (int64_t t = first; t < first + count; t++) {
+          const BlockTask& b = P.blocks[t];
+          const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], mu = S.Sp[b.s + 1] - S.Sp[b.s] - nc;
+          if (b.np != (mu + kInitRows - 1) / kInitRows) bad++;
+          for (int32_t p = b.p0; p < b.p0 + b.writer * b.np; p++)
+            bad += p < 0 || p >= P.initSlots || written[p] < 0 || written[p] >= int32_t(li);
+        }
       }
     }
+    for (int32_t w : written) bad += w < 0;
   }
-  for (int32_t w : written) bad += w < 0;
   return bad;
 }
+
+// Task lists of the flow launches that ps_solve simulates: those of FP32 factors (kFlowQuadsF32) or FP64 factors.
+void ps_set_flow_width(void* p, int32_t nq) { static_cast<Handle*>(p)->flowWidth = nq; }
 
 // solve q right-hand sides (one after another, original ordering) with the last factor
 void ps_solve(void* p, const double* b, double* x, int32_t q) {
@@ -545,7 +583,9 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
         for (int32_t j = 0; j < n; j++) y[size_t(k) * n + j] /= c.D[j];
       scaled = true;
     }
-    for (int64_t t = L.first; t < L.first + L.count; t++) {
+    int64_t first = L.first, count = L.count;
+    if (L.kind == kSolveFwdFlow || L.kind == kSolveBwdFlow) flowRange(L, h->flowWidth, first, count);
+    for (int64_t t = first; t < first + count; t++) {
       for (int32_t k = 0; k < q; k++) {
         double* yk = y.data() + size_t(k) * n;
         double* fk = fv.data() + size_t(k) * P.fvEntries;
@@ -584,23 +624,32 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
           }
           for (int32_t i = 0; i < m; i++) fk[S.Sp[s] + i] = v[i];
         } else if (L.kind == kSolveFwdFlow) {
-          // tasks run in list order: every block a task consumes was solved by an earlier task
+          // tasks run in list order: every block a task consumes was solved by an earlier task or, in a chain task,
+          // by the task itself before
           const BlockTask& bt = P.blocks[t];
-          const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
+          const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f;
+          const int32_t m = S.Sp[s + 1] - S.Sp[s];
           const double* Ls = Lx + S.Lxp[s];
-          std::vector<double> v(kTileEdge, 0.0);
-          for (int32_t r = bt.r0; r < bt.r1; r++) {
-            double sum = 0;
-            for (int32_t col = 0; col < std::min(nc, bt.blk * kTileEdge); col++) sum += Ls[r + int64_t(col) * m] * yk[f + col];
-            v[r - bt.r0] = fk[S.Sp[s] + r] - sum;
-            if (!bt.writer) fk[S.Sp[s] + r] = v[r - bt.r0];
-          }
-          if (bt.writer) {
-            const double* W = c.W.data() + F.Woff[s] + int64_t(bt.blk) * kTileEdge * kTileEdge;
-            for (int32_t i = 0; i < bt.r1 - bt.r0; i++) {
+          if (!bt.writer) {
+            for (int32_t r = bt.r0; r < bt.r1; r++) {
               double sum = 0;
-              for (int32_t j = 0; j < kTileEdge; j++) sum += W[i + int64_t(j) * kTileEdge] * v[j];
-              yk[f + bt.r0 + i] = sum;
+              for (int32_t col = 0; col < nc; col++) sum += Ls[r + int64_t(col) * m] * yk[f + col];
+              fk[S.Sp[s] + r] -= sum;
+            }
+          }
+          for (int32_t i = 0; i < bt.writer; i++) {
+            const int32_t kb = bt.blk + i, r0 = kb * kTileEdge, r1 = std::min(nc, r0 + kTileEdge);
+            std::vector<double> v(kTileEdge, 0.0);
+            for (int32_t r = r0; r < r1; r++) {
+              double sum = 0;
+              for (int32_t col = 0; col < r0; col++) sum += Ls[r + int64_t(col) * m] * yk[f + col];
+              v[r - r0] = fk[S.Sp[s] + r] - sum;
+            }
+            const double* W = c.W.data() + F.Woff[s] + int64_t(kb) * kTileEdge * kTileEdge;
+            for (int32_t r = r0; r < r1; r++) {
+              double sum = 0;
+              for (int32_t j = 0; j < kTileEdge; j++) sum += W[(r - r0) + int64_t(j) * kTileEdge] * v[j];
+              yk[f + r] = sum;
             }
           }
         } else if (L.kind == kSolveSmallBwd) {
@@ -623,19 +672,23 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
           const BlockTask& bt = P.blocks[t];
           const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
           const double* Ls = Lx + S.Lxp[s];
-          const double* W = c.W.data() + F.Woff[s] + int64_t(bt.blk) * kTileEdge * kTileEdge;
-          std::vector<double> z(kTileEdge, 0.0);
-          for (int32_t i = bt.r0; i < bt.r1; i++) {
-            double below = 0;  // the partial products of the initialisation tasks, in slot order
-            for (int32_t p = 0; p < bt.np; p++) below += pz[(size_t(bt.p0 + p) * q + k) * kTileEdge + (i - bt.r0)];
-            double sum = yk[f + i] - below;
-            for (int32_t r = bt.r1; r < nc; r++) sum -= Ls[r + int64_t(i) * m] * yk[f + r];
-            z[i - bt.r0] = sum;
-          }
-          for (int32_t j = 0; j < bt.r1 - bt.r0; j++) {
-            double sum = 0;
-            for (int32_t i = 0; i < kTileEdge; i++) sum += W[i + int64_t(j) * kTileEdge] * z[i];
-            yk[f + bt.r0 + j] = sum;
+          for (int32_t i = bt.writer - 1; i >= 0; i--) {
+            const int32_t kb = bt.blk + i, c0 = kb * kTileEdge, c1 = std::min(nc, c0 + kTileEdge);
+            const double* W = c.W.data() + F.Woff[s] + int64_t(kb) * kTileEdge * kTileEdge;
+            std::vector<double> z(kTileEdge, 0.0);
+            for (int32_t col = c0; col < c1; col++) {
+              double below = 0;  // the partial products of the initialisation tasks, in slot order
+              for (int32_t pp = 0; pp < bt.np; pp++)
+                below += pz[(size_t(bt.p0 + i * bt.np + pp) * q + k) * kTileEdge + (col - c0)];
+              double sum = yk[f + col] - below;
+              for (int32_t r = c1; r < nc; r++) sum -= Ls[r + int64_t(col) * m] * yk[f + r];
+              z[col - c0] = sum;
+            }
+            for (int32_t j = 0; j < c1 - c0; j++) {
+              double sum = 0;
+              for (int32_t i2 = 0; i2 < kTileEdge; i2++) sum += W[i2 + int64_t(j) * kTileEdge] * z[i2];
+              yk[f + c0 + j] = sum;
+            }
           }
         }
       }

@@ -177,6 +177,46 @@ def test_ldlt_well_conditioned_indefinite_matches_oracle(ncell):
         assert relmax(Xg[k], Xo[k]) < 1e-12
 
 
+@pytest.mark.parametrize("flow_wide", [0, 2**30])
+@pytest.mark.parametrize("kind", ["hip", "hip-f32"])
+def test_solve_task_widths_match_oracle(kind, flow_wide):
+    """Forward solve launches with multi-block tasks everywhere (flowWideMin = 0: a workgroup solves several 64-row
+    blocks and hands each to the next inside LDS) and with one block per task: Cholesky and LDL^T (quasi-definite)
+    solves of one and three right-hand sides. FP64 factors: within 1e-12 of the oracle's solutions. FP32 factors:
+    within 1e-5 of the oracle's FP64 solves with the same (downloaded) FP32 factor, and refined to 1e-12 of the
+    oracle's solution."""
+    prm = {"smallMax": 96, "smallNarrow": 32, "wideLaunch": 512, "superPanel": 256, "flowWideMin": flow_wide}
+    for mode, (pat, v) in ((0, W.heat3d(20)[:2]), (1, quasi_definite(14)[:2])):
+        n = pat["n"]
+        print(f"kind {kind} flowWideMin {flow_wide} mode {mode} n {n}", flush=True)
+        sym = amp.analyze(pat)
+        sym.set_params(prm)
+        So = O.Sym(pat)
+        if mode == 0:
+            _, Lo = O.chol_factor(So, v)
+            ref_solve = lambda L, D, b: O.chol_solve(So, L, b)  # noqa: E731
+            Do = None
+        else:
+            _, Lo, Do, _, _ = O.ldlt_factor(So, v)
+            ref_solve = lambda L, D, b: O.ldlt_solve(So, L, D, b)  # noqa: E731
+        num = amp.NumericFactor(sym, mode, kind)
+        assert num.factor(v)
+        B = np.stack([2 * W.lcg_uniform(31 + k, n) - 1 for k in range(3)])
+        exact = [ref_solve(Lo, Do, B[k]) for k in range(3)]
+        if kind == "hip":
+            ref, tol = exact, 1e-12
+        else:
+            Lg, Dg = num.download()
+            ref, tol = [ref_solve(Lg, Dg if mode else None, B[k]) for k in range(3)], 1e-5
+        assert relmax(num.solve(B[0]), ref[0]) < tol
+        X = num.solve(B.ravel(), nrhs=3).reshape(3, n)
+        for k in range(3):
+            assert relmax(X[k], ref[k]) < tol
+        if kind == "hip-f32":
+            x, info = num.refine(v, B[0], maxSteps=8, tol=1e-15)
+            assert relmax(x, exact[0]) < 1e-12
+
+
 def test_cholesky_reports_indefinite_and_solve_refuses():
     pat, v, n = grid27(11, 11, 11, shift=26)
     num = amp.factorNumeric(amp.analyze(pat), v)

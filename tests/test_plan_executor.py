@@ -33,6 +33,7 @@ def sim():
     L.ps_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.ps_check_flow.argtypes = [C.c_void_p]
     L.ps_check_flow.restype = C.c_int64
+    L.ps_set_flow_width.argtypes = [C.c_void_p, C.c_int32]
     L.ps_create_params.restype = C.c_void_p
     L.ps_create_params.argtypes = [C.c_int32, i32p, i32p, C.c_void_p, C.c_int32, C.c_void_p]
     L.ps_check_streams.argtypes = [C.c_void_p, C.c_int32]
@@ -78,20 +79,24 @@ def test_plan_cholesky_and_ldlt(sim, name):
         assert not np.isnan(Lg[m]).any(), "a lower entry was never written"
         assert np.abs(Lg[m] - Lo[m]).max() / np.abs(Lo[m]).max() < 1e-12
         B = np.stack([2 * W.lcg_uniform(11 + k, n) - 1 for k in range(3)])
-        X = np.zeros_like(B)
-        sim.ps_solve(h, B.ravel(), X.ravel(), 3)
-        for k in range(3):
-            xo = O.chol_solve(S, Lo, B[k])
-            assert np.abs(X[k] - xo).max() / np.abs(xo).max() < 1e-12
+        for width in (4, 2):  # the task lists of the FP32 and FP64 factors' solve kernels
+            sim.ps_set_flow_width(h, width)
+            X = np.zeros_like(B)
+            sim.ps_solve(h, B.ravel(), X.ravel(), 3)
+            for k in range(3):
+                xo = O.chol_solve(S, Lo, B[k])
+                assert np.abs(X[k] - xo).max() / np.abs(xo).max() < 1e-12
         okl, Llo, Do, nneg, npert = O.ldlt_factor(S, v)
         pert = np.zeros(n, dtype=np.uint8)
         assert sim.ps_factor(h, v, 1, 1e-13, Lg, D, pert.ctypes.data) == 2**31 - 1
         assert np.abs(D - Do).max() / np.abs(Do).max() < 1e-12
         assert (D < 0).sum() == nneg and pert.sum() == npert
-        x = np.zeros(n)
-        sim.ps_solve(h, B[0].copy(), x, 1)
         xo = O.ldlt_solve(S, Llo, Do, B[0])
-        assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
+        for width in (4, 2):
+            sim.ps_set_flow_width(h, width)
+            x = np.zeros(n)
+            sim.ps_solve(h, B[0].copy(), x, 1)
+            assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
     finally:
         sim.ps_free(h)
 
@@ -121,7 +126,8 @@ def test_plan_failure_column(sim):
 
 
 @pytest.mark.parametrize("name", ["heat3d_14", "heat3d_20", "grid27_11", "batch2d_8"])
-@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128), (96, 32, 2**30, 64), (64, 16, 256, 128), (64, 32, 256, 128)])
+@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128, 160), (96, 32, 2**30, 64, 160), (64, 16, 256, 128, 0),
+                                 (64, 32, 256, 128, 2**30)])
 def test_plan_two_stream_ordering(sim, name, prm):
     """Every pair of factor launches on the two streams that the plan's waits leave unordered touches disjoint
     memory, and no two tasks of one launch conflict; with the waits removed the checker must find conflicts (so it
@@ -139,6 +145,32 @@ def test_plan_two_stream_ordering(sim, name, prm):
         assert sim.ps_factor(h, v, 0, 0.0, Lg, D, None) == 2**31 - 1
         m = lower_mask(S)
         assert np.abs(Lg[m] - Lo[m]).max() / np.abs(Lo[m]).max() < 1e-12
+    finally:
+        sim.ps_free(h)
+
+
+@pytest.mark.parametrize("name", ["heat3d_20", "grid27_11", "batch2d_8"])
+def test_plan_wide_solve_tasks(sim, name):
+    """Every forward solve launch with multi-block tasks (flowWideMin = 0): the ordering contract holds for both task
+    lists and the plan's solves, replayed task by task, match the oracle's."""
+    pat, v = CASES[name]()
+    n = pat["n"]
+    pa = np.array((96, 32, 512, 256, 0), np.int32)
+    h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
+    try:
+        assert sim.ps_check_flow(h) == 0
+        S = O.Sym(pat)
+        ok, Lo = O.chol_factor(S, v)
+        Lg, D = np.zeros(S.nnzLStored), np.zeros(n)
+        assert sim.ps_factor(h, v, 0, 0.0, Lg, D, None) == 2**31 - 1
+        B = np.stack([2 * W.lcg_uniform(21 + k, n) - 1 for k in range(2)])
+        for width in (4, 2):
+            sim.ps_set_flow_width(h, width)
+            X = np.zeros_like(B)
+            sim.ps_solve(h, B.ravel(), X.ravel(), 2)
+            for k in range(2):
+                xo = O.chol_solve(S, Lo, B[k])
+                assert np.abs(X[k] - xo).max() / np.abs(xo).max() < 1e-12
     finally:
         sim.ps_free(h)
 
