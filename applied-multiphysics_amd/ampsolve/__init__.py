@@ -6,6 +6,7 @@ from .sparse import (  # noqa: F401
     SupernodalLDLT,
     Symbolic,
     amdOrder,
+    ndOrder,
     analyze,
     checkSymbolic,
     factorNumeric,

@@ -79,6 +79,7 @@ f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 # every entry point of include/ampsolve.h with its ctypes signature
 SIGNATURES = {
     "amp_amd_order": ([C.c_int32, i32p, i32p, C.c_double, i32p], C.c_int),
+    "amp_nd_order": ([C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p], C.c_int),
     "amp_analyze": ([C.c_int32, i32p, i32p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
     "amp_sym_import": ([C.POINTER(SymDesc), C.POINTER(C.c_void_p)], C.c_int),
     "amp_pattern_fingerprint": ([C.c_int32, i32p, i32p, C.c_char_p], C.c_int),

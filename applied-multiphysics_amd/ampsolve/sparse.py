@@ -64,6 +64,20 @@ def amdOrder(pat, denseFactor=10.0):
     return perm[:n]
 
 
+def ndOrder(pat, x, y, leaf=48):
+    """Nested dissection by recursive coordinate bisection, perm new -> old (sparse.mjs:230-283)."""
+    n = int(pat["n"])
+    rp, ci = _pat_arrays(pat)
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    ys = np.ascontiguousarray(y, dtype=np.float64)
+    if len(xs) != n or len(ys) != n:
+        raise ValueError("ndOrder: node coordinates x and y must have length n")
+    perm = np.zeros(max(n, 1), dtype=np.int32)
+    A.check(A.lib().amp_nd_order(n, rp, ci if len(ci) else np.zeros(1, np.int32), xs if n else np.zeros(1),
+                                 ys if n else np.zeros(1), int(leaf), perm))
+    return perm[:n]
+
+
 class Symbolic:
     """Symbolic analysis (the object analyze() returns, sparse.mjs:873-901) owning the native handle and plan."""
 
@@ -134,24 +148,32 @@ def _copy(p, count, dtype):
 
 
 def analyze(pat, opts=None):
-    """Symbolic analysis of a full symmetric CSR pattern (sparse.mjs:666-902). opts: {'perm', 'relax'}; the ordering
-    is AMD unless a perm (new -> old) is given. ('nd' ordering of the reference is not provided.)"""
+    """Symbolic analysis of a full symmetric CSR pattern (sparse.mjs:666-902). opts: {'perm', 'ordering', 'x', 'y',
+    'relax'}: perm (new -> old) to start from; else `ordering` 'amd' (default) or 'nd' (node coordinates x and y of
+    length n). timing: {ordering_ms, symbolic_ms} (ordering_ms 0 when perm is given)."""
     opts = opts or {}
     n = int(pat["n"])
     perm = opts.get("perm")
-    if opts.get("ordering", "amd") != "amd" and perm is None:
-        raise ValueError(f"analyze: ordering {opts.get('ordering')} is not available (amd, or pass perm)")
     if perm is not None and len(perm) != n:
         raise ValueError(f"analyze: perm has length {len(perm)}, pattern has {n} rows")
+    if perm is None and n > 0 and opts.get("ordering") == "nd":
+        x, y = opts.get("x"), opts.get("y")
+        if x is None or y is None or len(x) != n or len(y) != n:
+            raise ValueError("analyze: ordering nd needs node coordinates x and y of length n")
     rp, ci = _pat_arrays(pat)
     t0 = time.perf_counter()
+    ordering_ms = 0.0
+    if perm is None and n > 0:
+        perm = ndOrder(pat, opts["x"], opts["y"]) if opts.get("ordering") == "nd" else amdOrder(pat)
+        ordering_ms = 1e3 * (time.perf_counter() - t0)
+    t1 = time.perf_counter()
     h = C.c_void_p()
     permArr = None if perm is None else np.ascontiguousarray(perm, dtype=np.int32)
     A.check(A.lib().amp_analyze(n, rp, ci if len(ci) else np.zeros(1, np.int32),
                                 None if permArr is None else permArr.ctypes.data_as(C.c_void_p),
                                 1 if opts.get("relax", True) else 0, C.byref(h)))
     sym = Symbolic(h.value)
-    sym.timing = {"symbolic_ms": 1e3 * (time.perf_counter() - t0)}
+    sym.timing = {"ordering_ms": ordering_ms, "symbolic_ms": 1e3 * (time.perf_counter() - t1)}
     sym.pattern = {"n": n, "rowptr": rp.copy(), "colidx": ci.copy(), "nnz": int(rp[n]) if n else 0}
     return sym
 
@@ -349,14 +371,14 @@ def solveNumericMulti(num, bs, outs=None):
 class SparseCholesky:
     """Supernodal multifrontal Cholesky for a fixed pattern (sparse.mjs:1312-1372)."""
 
-    def __init__(self, pat, perm=None, sym=None, ordering="amd", factor=None):
+    def __init__(self, pat, perm=None, sym=None, ordering="amd", x=None, y=None, factor=None):
         t0 = time.perf_counter()
         self.factorKind = factor
         if sym is not None:
             checkSymbolic(sym, pat)
             self.sym = sym
         else:
-            self.sym = analyze(pat, {"perm": perm} if perm is not None else {"ordering": ordering})
+            self.sym = analyze(pat, {"perm": perm} if perm is not None else {"ordering": ordering, "x": x, "y": y})
         self.n = int(pat["n"])
         self.perm = self.sym.perm
         self.nnzL = self.sym.nnzL
@@ -365,7 +387,9 @@ class SparseCholesky:
         reused = sym is not None
         self.timing = {"n": self.n, "nnzA": self.sym.nnzA, "nnzL": self.nnzL, "fill": self.sym.fill,
                        "flops": self.sym.flops, "nsuper": self.sym.nsuper, "symbolicReused": reused,
-                       "symbolic_ms": 0.0 if reused else 1e3 * (time.perf_counter() - t0), "factor_ms": 0.0,
+                       "ordering_ms": 0.0 if reused else self.sym.timing["ordering_ms"],
+                       "symbolic_ms": 0.0 if reused else
+                       1e3 * (time.perf_counter() - t0) - self.sym.timing["ordering_ms"], "factor_ms": 0.0,
                        "nFactor": 0, "solve_ms": 0.0, "nSolve": 0}
 
     def factor(self, values):

@@ -338,6 +338,18 @@ int amp_amd_order(int32_t n, const int32_t* rowptr, const int32_t* colidx, doubl
   return AMP_OK;
 }
 
+int amp_nd_order(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* x, const double* y, int32_t leaf,
+                 int32_t* perm) {
+  if (n < 0 || leaf < 1 || (n > 0 && (!rowptr || !colidx || !x || !y || !perm)))
+    return fail(AMP_ERR_ARG, "amp_nd_order: bad arguments");
+  for (int32_t i = 0; i < n; i++)
+    if (rowptr[i + 1] < rowptr[i]) return fail(AMP_ERR_ARG, "amp_nd_order: rowptr decreases");
+  for (int32_t p = 0; p < (n > 0 ? rowptr[n] : 0); p++)
+    if (colidx[p] < 0 || colidx[p] >= n) return fail(AMP_ERR_ARG, "amp_nd_order: column index out of range");
+  amp::ndOrder(n, rowptr, colidx, x, y, leaf, perm);
+  return AMP_OK;
+}
+
 int amp_pattern_fingerprint(int32_t n, const int32_t* rowptr, const int32_t* colidx, char* out) {
   if (n < 0 || !rowptr || (n > 0 && !colidx) || !out) return fail(AMP_ERR_ARG, "amp_pattern_fingerprint: bad arguments");
   const std::string k = amp::patternFingerprint(n, rowptr, colidx);

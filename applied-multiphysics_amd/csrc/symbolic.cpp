@@ -395,6 +395,55 @@ std::string patternFingerprint(int32_t n, const int32_t* rowptr, const int32_t* 
   return buf;
 }
 
+// ndOrder (sparse.mjs:230-283). A range of more than `leaf` nodes is sorted stably by its coordinate of larger extent
+// (x when the extents are equal) and split at half its length; a node of the first half with a neighbour in the second
+// half is a separator node. The order of the range is the order of its first half without the separators, then that
+// of its second half, then the separators (in sorted order); a range of at most `leaf` nodes keeps its order. Each
+// range is rearranged in place into [first half without separators | second half | separators] and its first two
+// parts are split in turn, so the array ends up holding the order itself.
+void ndOrder(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* x, const double* y, int32_t leaf,
+             int32_t* perm) {
+  for (int32_t i = 0; i < n; i++) perm[i] = i;
+  std::vector<int8_t> second(size_t(n), 0);
+  std::vector<int32_t> tmp;
+  std::vector<std::pair<int32_t, int32_t>> work{{0, n}};  // ranges [lo, hi) still to split
+  while (!work.empty()) {
+    const auto [lo, hi] = work.back();
+    work.pop_back();
+    if (hi - lo <= leaf) continue;
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int32_t i = lo; i < hi; i++) {
+      const int32_t v = perm[i];
+      x0 = std::min(x0, x[v]);
+      x1 = std::max(x1, x[v]);
+      y0 = std::min(y0, y[v]);
+      y1 = std::max(y1, y[v]);
+    }
+    const double* key = x1 - x0 >= y1 - y0 ? x : y;
+    std::stable_sort(perm + lo, perm + hi, [key](int32_t p, int32_t q) { return key[p] < key[q]; });
+    const int32_t half = (hi - lo) >> 1;
+    for (int32_t i = lo + half; i < hi; i++) second[perm[i]] = 1;
+    // first half: non-separators stay in front (in order), separators go to tmp; then the second half moves up
+    tmp.clear();
+    int32_t a = lo;
+    for (int32_t i = lo; i < lo + half; i++) {
+      const int32_t v = perm[i];
+      bool sep = false;
+      for (int32_t p = rowptr[v]; p < rowptr[v + 1] && !sep; p++) sep = colidx[p] != v && second[colidx[p]];
+      if (sep)
+        tmp.push_back(v);
+      else
+        perm[a++] = v;
+    }
+    for (int32_t i = lo + half; i < hi; i++) second[perm[i]] = 0;
+    const int32_t b = a + (hi - lo - half);
+    std::memmove(perm + a, perm + lo + half, sizeof(int32_t) * size_t(hi - lo - half));
+    std::copy(tmp.begin(), tmp.end(), perm + b);
+    work.push_back({a, b});   // second half
+    work.push_back({lo, a});  // first half without the separators
+  }
+}
+
 bool amdOrder(int32_t n, const int32_t* rowptr, const int32_t* colidx, double denseFactor, int32_t* perm) {
   AmdOrdering amd(n, rowptr, colidx, denseFactor);
   return amd.run(perm);

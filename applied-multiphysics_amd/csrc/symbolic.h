@@ -20,6 +20,9 @@ struct Symbolic {
 
 std::string patternFingerprint(int32_t n, const int32_t* rowptr, const int32_t* colidx);
 bool amdOrder(int32_t n, const int32_t* rowptr, const int32_t* colidx, double denseFactor, int32_t* perm);
+// Nested dissection by recursive coordinate bisection (ndOrder, sparse.mjs:230-283): perm new -> old. leaf >= 1.
+void ndOrder(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* x, const double* y, int32_t leaf,
+             int32_t* perm);
 bool analyzePattern(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm0, bool relax,
                     Symbolic& out);
 

@@ -384,9 +384,80 @@ static napi_value js_fingerprint(napi_env env, napi_callback_info cb) {
   return s;
 }
 
+/* the pattern arguments of an ordering call (argv[0..2]); NULL after throwing */
+static int order_pattern(napi_env env, napi_value *argv, const char *who, int32_t *n, const int32_t **rp,
+                         const int32_t **ci) {
+  void *r, *c;
+  size_t l1, l2;
+  if (napi_get_value_int32(env, argv[0], n) != napi_ok || !typed(env, argv[1], I32, &r, &l1) ||
+      !typed(env, argv[2], I32, &c, &l2)) {
+    throw_amp(env, AMP_ERR_ARG);
+    return 0;
+  }
+  if (*n < 0 || !r || l1 < (size_t)*n + 1 || ((const int32_t *)r)[*n] < 0 || l2 < (size_t)((const int32_t *)r)[*n]) {
+    char msg[96];
+    snprintf(msg, sizeof msg, "%s: rowptr / colidx shorter than the pattern", who);
+    throw_len(env, msg);
+    return 0;
+  }
+  static const int32_t none = 0;
+  *rp = (const int32_t *)r;
+  *ci = c ? (const int32_t *)c : &none;
+  return 1;
+}
+
+/* a new Int32Array of n entries and its storage */
+static napi_value new_i32(napi_env env, size_t n, int32_t **data) {
+  napi_value ab, arr;
+  void *dst;
+  if (napi_create_arraybuffer(env, n * 4, &dst, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, napi_int32_array, n, ab, 0, &arr) != napi_ok) return NULL;
+  *data = (int32_t *)dst;
+  return arr;
+}
+
+/* amdOrder(n, rowptr, colidx, denseFactor) -> Int32Array perm (new -> old) */
+static napi_value js_amd_order(napi_env env, napi_callback_info cb) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, cb, &argc, argv, NULL, NULL));
+  int32_t n;
+  const int32_t *rp, *ci;
+  if (!order_pattern(env, argv, "amdOrder", &n, &rp, &ci)) return NULL;
+  double dense = 10.0;
+  if (argc > 3) napi_get_value_double(env, argv[3], &dense);
+  int32_t *perm;
+  napi_value out = new_i32(env, (size_t)n, &perm);
+  if (!out) return throw_amp(env, AMP_ERR_ARG);
+  if (n > 0) CHECK_AMP(amp_amd_order(n, rp, ci, dense, perm));
+  return out;
+}
+
+/* ndOrder(n, rowptr, colidx, x, y, leaf) -> Int32Array perm (new -> old) */
+static napi_value js_nd_order(napi_env env, napi_callback_info cb) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CHECK_NAPI(napi_get_cb_info(env, cb, &argc, argv, NULL, NULL));
+  int32_t n, leaf = 48;
+  const int32_t *rp, *ci;
+  if (!order_pattern(env, argv, "ndOrder", &n, &rp, &ci)) return NULL;
+  void *x, *y;
+  size_t lx, ly;
+  if (!typed(env, argv[3], F64, &x, &lx) || !typed(env, argv[4], F64, &y, &ly)) return throw_amp(env, AMP_ERR_ARG);
+  if (lx != (size_t)n || ly != (size_t)n) return throw_len(env, "ndOrder: node coordinates x and y must have length n");
+  if (argc > 5) napi_get_value_int32(env, argv[5], &leaf);
+  int32_t *perm;
+  napi_value out = new_i32(env, (size_t)n, &perm);
+  if (!out) return throw_amp(env, AMP_ERR_ARG);
+  if (n > 0) CHECK_AMP(amp_nd_order(n, rp, ci, (const double *)x, (const double *)y, leaf, perm));
+  return out;
+}
+
 NAPI_MODULE_INIT() {
   napi_property_descriptor props[] = {
       {"analyze", NULL, js_analyze, NULL, NULL, NULL, napi_default, NULL},
+      {"amdOrder", NULL, js_amd_order, NULL, NULL, NULL, napi_default, NULL},
+      {"ndOrder", NULL, js_nd_order, NULL, NULL, NULL, napi_default, NULL},
       {"importSymbolic", NULL, js_import, NULL, NULL, NULL, napi_default, NULL},
       {"numCreate", NULL, js_num_create, NULL, NULL, NULL, napi_default, NULL},
       {"factor", NULL, js_factor, NULL, NULL, NULL, napi_default, NULL},

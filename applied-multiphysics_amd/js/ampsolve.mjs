@@ -13,22 +13,35 @@ import { createRequire } from 'module';
 const require = createRequire(import.meta.url);
 const native = require('./amp_native.node');
 
-export const REAL_FACTOR_KINDS = ['hip', 'hip-f32'];
+export const REAL_FACTOR_KINDS = ['hip', 'hip-f32', 'hip-bf16x3'];
 export let DEFAULT_REAL_FACTOR = 'hip';
 
-/** Select the default device factor: 'hip' (FP64) or 'hip-f32' (FP32 factor, refine with solveRefined). */
+/**
+ * Select the default device factor: 'hip' (FP64), 'hip-f32' (FP32 factor) or 'hip-bf16x3' (FP32 factor whose bulk
+ * updates run as three bf16 products); refine the reduced-precision ones with solveRefined.
+ */
 export const setDefaultRealFactor = (k) => {
-  if (!REAL_FACTOR_KINDS.includes(k)) throw new Error(`unknown real factor ${k} (hip | hip-f32)`);
+  if (!REAL_FACTOR_KINDS.includes(k)) throw new Error(`unknown real factor ${k} (hip | hip-f32 | hip-bf16x3)`);
   return (DEFAULT_REAL_FACTOR = k);
 };
 
 function resolveKind(k) {
   const kind = k === null || k === undefined ? DEFAULT_REAL_FACTOR : k;
-  if (!REAL_FACTOR_KINDS.includes(kind)) throw new Error(`unknown real factor ${kind} (hip | hip-f32)`);
+  if (!REAL_FACTOR_KINDS.includes(kind)) throw new Error(`unknown real factor ${kind} (hip | hip-f32 | hip-bf16x3)`);
   return kind;
 }
 
-const precisionOf = (kind) => (kind === 'hip-f32' ? 1 : 0);
+const precisionOf = (kind) => (kind === 'hip-f32' ? 1 : kind === 'hip-bf16x3' ? 2 : 0);
+
+/** Approximate minimum degree ordering, perm new -> old (sparse.mjs:297-592). */
+export function amdOrder(pat, denseFactor = 10) {
+  return native.amdOrder(pat.n, pat.rowptr, pat.colidx, denseFactor);
+}
+
+/** Nested dissection ordering by recursive coordinate bisection, perm new -> old (sparse.mjs:230-283). */
+export function ndOrder(pat, x, y, leaf = 48) {
+  return native.ndOrder(pat.n, pat.rowptr, pat.colidx, Float64Array.from(x), Float64Array.from(y), leaf);
+}
 
 /** "n:nnz:hash", 32-bit FNV-1a over rowptr and colidx. */
 export function patternFingerprint(pat) {
@@ -41,17 +54,22 @@ export function checkSymbolic(sym, pat) {
 }
 
 /**
- * Symbolic analysis (AMD ordering unless opts.perm is given; opts.relax default true). Returns the fields of the
- * reference's analysis object plus a native handle.
+ * Symbolic analysis. opts.perm: fill-reducing order (new -> old) to start from; else opts.ordering 'amd' (default) or
+ * 'nd' (needs node coordinates opts.x, opts.y of length n); opts.relax default true. Returns the fields of the
+ * reference's analysis object plus a native handle; timing {ordering_ms, symbolic_ms}.
  */
 export function analyze(pat, opts = {}) {
-  const t0 = Date.now();
   const n = pat.n;
   if (opts.perm && opts.perm.length !== n) throw new Error(`analyze: perm has length ${opts.perm.length}, pattern has ${n} rows`);
-  if (!opts.perm && opts.ordering && opts.ordering !== 'amd') throw new Error(`analyze: ordering ${opts.ordering} is not available here (amd, or pass perm)`);
-  const sym = native.analyze(n, pat.rowptr, pat.colidx, opts.perm ? Int32Array.from(opts.perm) : null, opts.relax !== false);
+  const nd = !opts.perm && n > 0 && opts.ordering === 'nd';
+  if (nd && !(opts.x && opts.x.length === n && opts.y && opts.y.length === n)) throw new Error('analyze: ordering nd needs node coordinates x and y of length n');
+  const t0 = Date.now();
+  let perm = opts.perm ? Int32Array.from(opts.perm) : null;
+  if (!perm && n > 0) perm = nd ? ndOrder(pat, opts.x, opts.y) : amdOrder(pat);
+  const t1 = Date.now();
+  const sym = native.analyze(n, pat.rowptr, pat.colidx, perm, opts.relax !== false);
   sym.nnzPattern = pat.rowptr[n];
-  sym.timing = { ordering_ms: 0, symbolic_ms: Date.now() - t0 };
+  sym.timing = { ordering_ms: opts.perm ? 0 : t1 - t0, symbolic_ms: Date.now() - t1 };
   return sym;
 }
 
@@ -128,10 +146,10 @@ export function solveNumericMulti(num, bs, outs = null) {
 
 /** Supernodal multifrontal Cholesky for a fixed pattern, factored and solved on the GPU. */
 export class SparseCholesky {
-  constructor(pat, perm = null, { sym = null, ordering = 'amd', factor = null } = {}) {
+  constructor(pat, perm = null, { sym = null, ordering = 'amd', x = null, y = null, factor = null } = {}) {
     const tStart = Date.now();
     this.factorKind = factor;
-    this.sym = sym || analyze(pat, perm ? { perm } : { ordering });
+    this.sym = sym || analyze(pat, perm ? { perm } : { ordering, x, y });
     if (sym) checkSymbolic(sym, pat);
     this.pat = pat;
     this.n = pat.n;
@@ -140,9 +158,10 @@ export class SparseCholesky {
     this.num = null;
     this.ok = false;
     const reused = !!sym;
+    const orderingMs = reused || !this.sym.timing ? 0 : this.sym.timing.ordering_ms;
     this.timing = {
       n: this.n, nnzA: this.sym.nnzA, nnzL: this.nnzL, fill: this.sym.fill, flops: this.sym.flops, nsuper: this.sym.nsuper,
-      symbolicReused: reused, ordering_ms: 0, symbolic_ms: reused ? 0 : Date.now() - tStart,
+      symbolicReused: reused, ordering_ms: orderingMs, symbolic_ms: reused ? 0 : Date.now() - tStart - orderingMs,
       factor_ms: 0, nFactor: 0, solve_ms: 0, nSolve: 0,
     };
   }

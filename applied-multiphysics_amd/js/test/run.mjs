@@ -1,6 +1,6 @@
 // Checks of the JavaScript host layer, run by tests/test_js_host.py: `--cpu` (no device: analysis, fingerprint,
 // argument errors) or `--gpu` (factor / solve / LDL^T / refinement through the addon). Prints one JSON line.
-import { analyze, patternFingerprint, checkSymbolic, factorNumeric, solveNumeric, solveNumericMulti, SparseCholesky, SupernodalLDLT, setDefaultRealFactor } from '../ampsolve.mjs';
+import { analyze, amdOrder, ndOrder, patternFingerprint, checkSymbolic, factorNumeric, solveNumeric, solveNumericMulti, SparseCholesky, SupernodalLDLT, setDefaultRealFactor } from '../ampsolve.mjs';
 
 /** 2D 5-point Laplacian on a k x k grid (full symmetric CSR, sorted rows). */
 function laplace(k, shift = 0) {
@@ -56,6 +56,16 @@ try { patternFingerprint({ n: pat.n, rowptr: Float64Array.from(pat.rowptr), coli
 try { patternFingerprint({ n: pat.n, rowptr: pat.rowptr, colidx: pat.colidx.subarray(0, 10) }); } catch (e) { addonErrors.push(e.message); }
 try { analyze({ n: pat.n + 5, rowptr: pat.rowptr, colidx: pat.colidx }); } catch (e) { addonErrors.push(e.message); }
 out.addon_errors = addonErrors;
+// orderings: ndOrder from the grid coordinates, analyze with ordering 'nd' (and without coordinates), timing split
+const gx = Float64Array.from({ length: pat.n }, (_, r) => r % 30), gy = Float64Array.from({ length: pat.n }, (_, r) => Math.floor(r / 30));
+out.nd = Array.from(ndOrder(pat, gx, gy));
+out.amd = Array.from(amdOrder(pat));
+const symNd = analyze(pat, { ordering: 'nd', x: gx, y: gy });
+out.nd_perm = Array.from(symNd.perm);
+out.nd_timing = Object.keys(symNd.timing).sort();
+try { analyze(pat, { ordering: 'nd' }); } catch (e) { out.nd_error = e.message; }
+out.kinds = ['hip', 'hip-f32', 'hip-bf16x3'].map((k) => setDefaultRealFactor(k));
+setDefaultRealFactor('hip');
 if (mode === '--gpu') {
   const b = Float64Array.from({ length: pat.n }, (_, i) => Math.sin(0.3 * i) + 0.1);
   const ch = new SparseCholesky(pat);

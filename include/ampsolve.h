@@ -87,6 +87,12 @@ typedef struct {
 /* amdOrder (sparse.mjs:297-592): approximate minimum degree, perm new -> old. denseFactor: 10 in the reference. */
 int amp_amd_order(int32_t n, const int32_t *rowptr, const int32_t *colidx, double denseFactor, int32_t *perm);
 
+/* ndOrder (sparse.mjs:230-283): nested dissection by recursive coordinate bisection with node coordinates x, y
+ * (ordering 'nd' of analyze, sparse.mjs:678-679), perm new -> old; ranges of at most `leaf` nodes (48 in the reference)
+ * keep their order. AMP_ERR_ARG for leaf < 1 or a malformed pattern. */
+int amp_nd_order(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *x, const double *y, int32_t leaf,
+                 int32_t *perm);
+
 /* analyze (sparse.mjs:666-902): ordering (AMD when perm is NULL), etree, postorder, column counts, relaxed supernodes
  * (relax != 0), row structures and block offsets, plus the device execution plan. AMP_ERR_ARG when the plan's index
  * ranges cannot hold the system (a tiled front with rows x super-panel width >= 2^31, or more than 2^31 extend-add

@@ -47,6 +47,13 @@ typedef struct {
 /* amdOrder (sparse.mjs:297-592). perm: new -> old, length n. Returns 0, or -1 on internal error. */
 int orc_amd(int32_t n, const int32_t *rowptr, const int32_t *colidx, double denseFactor, int32_t *perm);
 
+/* ndOrder (sparse.mjs:230-283): recursive coordinate bisection, perm new -> old. Pinned by a hand-derived order of a
+ * 100-node path (split at half the length, separator from the first half, first half / second half / separator
+ * order, leaves of `leaf` nodes), the axis rule on a vertical path and equal extents, the stability of ties and the
+ * fill of grids against the natural order (tests/test_oracle.py). Returns 0, or -1 on allocation failure. */
+int orc_nd_order(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *x, const double *y,
+                 int32_t leaf, int32_t *perm);
+
 /* analyze (sparse.mjs:666-902). perm0 may be NULL (AMD). relax: relaxed amalgamation. NULL on allocation failure. */
 orc_sym *orc_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const int32_t *perm0, int relax);
 void orc_sym_free(orc_sym *s);

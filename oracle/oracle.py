@@ -66,6 +66,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_amd.argtypes = [C.c_int32, i32p, i32p, C.c_double, i32p]
         L.orc_amd.restype = C.c_int
+        L.orc_nd_order.argtypes = [C.c_int32, i32p, i32p, f64p, f64p, C.c_int32, i32p]
+        L.orc_nd_order.restype = C.c_int
         L.orc_analyze.argtypes = [C.c_int32, i32p, i32p, C.c_void_p, C.c_int]
         L.orc_analyze.restype = C.POINTER(OrcSym)
         L.orc_sym_free.argtypes = [C.POINTER(OrcSym)]
@@ -124,6 +126,17 @@ def amd(pat, dense_factor=10.0):
     perm = np.zeros(n, dtype=np.int32)
     if lib().orc_amd(n, rp, ci, dense_factor, perm) != 0:
         raise RuntimeError("amdOrder: internal error")
+    return perm
+
+
+def nd_order(pat, x, y, leaf=48):
+    """ndOrder (sparse.mjs:230-283): perm new -> old from node coordinates x, y."""
+    n = pat["n"]
+    rp, ci = _pat(pat)
+    perm = np.zeros(n, dtype=np.int32)
+    if lib().orc_nd_order(n, rp, ci, np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64), leaf,
+                          perm) != 0:
+        raise RuntimeError("ndOrder: bad arguments or allocation failure")
     return perm
 
 

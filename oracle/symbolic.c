@@ -13,6 +13,98 @@
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
+/* ---- ndOrder, sparse.mjs:230-283 ----------------------------------------------------------------------------- */
+
+/* stable sort of v[0..m) by key (the reference sorts with Array.prototype.sort, which is stable), by merging */
+static void nd_sort(int32_t *v, int32_t *buf, int32_t m, const double *key) {
+  if (m < 2) return;
+  const int32_t h = m / 2;
+  nd_sort(v, buf, h, key);
+  nd_sort(v + h, buf, m - h, key);
+  int32_t i = 0, j = h, k = 0;
+  while (i < h && j < m) buf[k++] = key[v[j]] < key[v[i]] ? v[j++] : v[i++];
+  while (i < h) buf[k++] = v[i++];
+  while (j < m) buf[k++] = v[j++];
+  memcpy(v, buf, sizeof(int32_t) * (size_t)m);
+}
+
+typedef struct {
+  const int32_t *rowptr, *colidx;
+  const double *x, *y;
+  int32_t leaf;
+  int8_t *side; /* 0 unassigned, 1 first half, 2 second half */
+  int32_t *order, norder;
+} nd_ctx;
+
+/* rec(nodes) of sparse.mjs:236-275 */
+static int nd_rec(nd_ctx *c, const int32_t *nodes, int32_t m) {
+  if (m <= c->leaf) {
+    for (int32_t i = 0; i < m; i++) c->order[c->norder++] = nodes[i];
+    return 0;
+  }
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int32_t i = 0; i < m; i++) {
+    const int32_t v = nodes[i];
+    if (c->x[v] < x0) x0 = c->x[v];
+    if (c->x[v] > x1) x1 = c->x[v];
+    if (c->y[v] < y0) y0 = c->y[v];
+    if (c->y[v] > y1) y1 = c->y[v];
+  }
+  const double *key = x1 - x0 >= y1 - y0 ? c->x : c->y;
+  int32_t *sorted = malloc(sizeof(int32_t) * (size_t)m), *buf = malloc(sizeof(int32_t) * (size_t)m);
+  int32_t *A = malloc(sizeof(int32_t) * (size_t)m), *S = malloc(sizeof(int32_t) * (size_t)m);
+  int32_t *B = malloc(sizeof(int32_t) * (size_t)m);
+  if (!sorted || !buf || !A || !S || !B) {
+    free(sorted), free(buf), free(A), free(S), free(B);
+    return -1;
+  }
+  memcpy(sorted, nodes, sizeof(int32_t) * (size_t)m);
+  nd_sort(sorted, buf, m, key);
+  const int32_t half = m >> 1;
+  for (int32_t i = 0; i < m; i++) c->side[sorted[i]] = i < half ? 1 : 2;
+  int32_t na = 0, ns = 0, nb = 0;
+  for (int32_t i = 0; i < m; i++) {
+    const int32_t v = sorted[i];
+    if (i < half) {
+      int sep = 0;
+      for (int32_t p = c->rowptr[v]; p < c->rowptr[v + 1]; p++) {
+        const int32_t w = c->colidx[p];
+        if (w != v && c->side[w] == 2) {
+          sep = 1;
+          break;
+        }
+      }
+      if (sep)
+        S[ns++] = v;
+      else
+        A[na++] = v;
+    } else
+      B[nb++] = v;
+  }
+  for (int32_t i = 0; i < m; i++) c->side[sorted[i]] = 0;
+  int rc = nd_rec(c, A, na);
+  if (!rc) rc = nd_rec(c, B, nb);
+  if (!rc)
+    for (int32_t i = 0; i < ns; i++) c->order[c->norder++] = S[i];
+  free(sorted), free(buf), free(A), free(S), free(B);
+  return rc;
+}
+
+int orc_nd_order(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *x, const double *y,
+                 int32_t leaf, int32_t *perm) {
+  if (leaf < 1) return -1;
+  nd_ctx c = {rowptr, colidx, x, y, leaf, xcalloc((size_t)n, 1), perm, 0};
+  int32_t *all = xcalloc((size_t)n, sizeof(int32_t));
+  if (!c.side || !all) {
+    free(c.side), free(all);
+    return -1;
+  }
+  for (int32_t i = 0; i < n; i++) all[i] = i;
+  const int rc = nd_rec(&c, all, n);
+  free(c.side), free(all);
+  return rc;
+}
+
 /* ---- amdOrder, sparse.mjs:297-592 ---------------------------------------------------------------------------- */
 int orc_amd(int32_t n, const int32_t *rowptr, const int32_t *colidx, double denseFactor, int32_t *out) {
   if (n == 0) return 0;

@@ -49,6 +49,15 @@ def test_js_analysis_and_errors_cpu():
     assert len(out["addon_errors"]) == 3
     assert "shorter than the pattern" in out["addon_errors"][1]
     assert "shorter than the pattern" in out["addon_errors"][2]
+    # orderings: the dissection order equals the oracle's, analyze('nd') postorders it, AMD equals the oracle's
+    g = np.arange(pat["n"])
+    x, y = (g % 30).astype(float), (g // 30).astype(float)
+    nd = O.nd_order(pat, x, y)
+    assert out["nd"] == nd.tolist() and out["amd"] == O.amd(pat).tolist()
+    assert out["nd_perm"] == O.Sym(pat, perm=nd).perm.tolist()
+    assert out["nd_timing"] == ["ordering_ms", "symbolic_ms"]
+    assert out["nd_error"] == "analyze: ordering nd needs node coordinates x and y of length n"
+    assert out["kinds"] == ["hip", "hip-f32", "hip-bf16x3"]
 
 
 @needs_node

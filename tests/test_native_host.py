@@ -148,3 +148,28 @@ def test_sym_import_rejects_corrupted_descriptors(what):
     assert rc == _capi.AMP_ERR_ARG, (what, rc)
     assert not h.value
     assert _capi.lib().amp_last_error().decode().startswith("amp_sym_import")
+
+
+def test_analyze_nd_ordering():
+    """ordering 'nd' (sparse.mjs:678-679): needs node coordinates of length n (the reference's message), takes the
+    dissection order as its permutation, and reports the ordering time apart from the symbolic time."""
+    import ampsolve as amp
+    from test_oracle import grid_pattern
+
+    pat, x, y = grid_pattern(30, 20)
+    with pytest.raises(ValueError, match="needs node coordinates"):
+        amp.analyze(pat, {"ordering": "nd"})
+    with pytest.raises(ValueError, match="needs node coordinates"):
+        amp.analyze(pat, {"ordering": "nd", "x": x[:-1], "y": y})
+    sym = amp.analyze(pat, {"ordering": "nd", "x": x, "y": y})
+    given = amp.analyze(pat, {"perm": amp.ndOrder(pat, x, y)})  # the analysis postorders the given order
+    assert np.array_equal(sym.perm, given.perm) and sym.nnzL == given.nnzL
+    assert set(sym.timing) == {"ordering_ms", "symbolic_ms"} and sym.timing["ordering_ms"] >= 0
+    assert given.timing["ordering_ms"] == 0
+    So = O.Sym(pat, perm=O.nd_order(pat, x, y))
+    assert np.array_equal(sym.perm, So.perm) and np.array_equal(sym.Si, So.Si)
+    assert np.array_equal(amp.analyze(pat).perm, amp.analyze(pat, {"perm": amp.amdOrder(pat)}).perm)
+    ch = amp.SparseCholesky(pat, ordering="nd", x=x, y=y)
+    assert np.array_equal(ch.perm, sym.perm) and "ordering_ms" in ch.timing
+    with pytest.raises(Exception):
+        amp.ndOrder(pat, x, y, leaf=0)

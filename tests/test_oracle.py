@@ -447,3 +447,64 @@ def test_complex_ldlt_perturbed_pivots_and_refinement():
     assert np.abs(pr - xr).max() / np.abs(xr).max() > 1e-6
     rr, ri, rel, steps = O.cldlt_solve_refined(S, Lr, Li, Dr, Di, pat, re, im, br, bi, max_steps=30)
     assert np.abs(rr - xr).max() / np.abs(xr).max() < 1e-10 and np.abs(ri - xi).max() / np.abs(xi).max() < 1e-10
+
+
+# ---- ndOrder (sparse.mjs:230-283), restated in oracle/symbolic.c and csrc/symbolic.cpp -----------------------------
+
+def grid_pattern(nx, ny, nine=False):
+    """5-point (or 9-point) grid graph, node i + nx j at (x, y) = (i, j)."""
+    Ix, Iy = sp.identity(nx), sp.identity(ny)
+    Tx = sp.diags([1, 1, 1], [-1, 0, 1], shape=(nx, nx))
+    Ty = sp.diags([1, 1, 1], [-1, 0, 1], shape=(ny, ny))
+    M = sp.kron(Ty, Tx) if nine else sp.kron(Iy, Tx) + sp.kron(Ty, Ix)
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    g = np.arange(nx * ny)
+    return W.csr_dict(M), (g % nx).astype(float), (g // nx).astype(float)
+
+
+def test_nd_order_of_a_path_closed_form():
+    """A 100-node path at x = i: split at half the length (50), the first half's node next to the second half is the
+    separator, the first half (49 nodes) splits again at 24 with separator 23, the second (50) at 25 with separator
+    74; ranges of at most 48 nodes keep their order; the order is first half, second half, separator at every level.
+    Laid out vertically (x = 0, y = i) the bisection takes y and gives the same order."""
+    n = 100
+    M = sp.diags([1, 1, 1], [-1, 0, 1], shape=(n, n), format="csr")
+    pat = W.csr_dict(M)
+    want = list(range(0, 23)) + list(range(24, 49)) + [23] + list(range(50, 74)) + list(range(75, 100)) + [74, 49]
+    i = np.arange(n, dtype=float)
+    assert O.nd_order(pat, i, np.zeros(n)).tolist() == want
+    assert O.nd_order(pat, np.zeros(n), i).tolist() == want
+    # ranges of at most `leaf` nodes keep their order
+    assert O.nd_order(pat, i, np.zeros(n), leaf=100).tolist() == list(range(n))
+    p48 = W.csr_dict(sp.diags([1, 1, 1], [-1, 0, 1], shape=(48, 48), format="csr"))
+    assert O.nd_order(p48, np.arange(48.0), np.zeros(48)).tolist() == list(range(48))
+
+
+def test_nd_order_axis_rule_and_stable_ties():
+    """A 10 x 10 grid has equal extents, so the bisection takes x (>=); the sort is stable, so the first separator --
+    the last ten nodes of the order -- is the column x = 4 in node order (4, 14, ..., 94). Taking y, or an unstable
+    sort, would change it."""
+    pat, x, y = grid_pattern(10, 10)
+    perm = O.nd_order(pat, x, y)
+    assert perm[-10:].tolist() == [4 + 10 * j for j in range(10)]
+    assert sorted(perm.tolist()) == list(range(100))
+
+
+@pytest.mark.parametrize("nine", [False, True])
+def test_nd_order_reduces_fill_and_native_matches_oracle(nine):
+    """On grids the dissection order has far less fill than the natural order; the native ordering (amp_nd_order)
+    equals the oracle's on grids and on jittered coordinates with many ties broken by node order."""
+    import ampsolve as amp
+
+    pat, x, y = grid_pattern(40, 40, nine)
+    perm = O.nd_order(pat, x, y)
+    S_nd = O.Sym(pat, perm=perm)
+    S_nat = O.Sym(pat, perm=np.arange(pat["n"]))
+    assert S_nd.nnzL < (0.75 if nine else 0.6) * S_nat.nnzL
+    assert np.array_equal(amp.ndOrder(pat, x, y), perm)
+    rng = np.random.default_rng(3)
+    for leaf in (48, 5, 1):
+        xj = np.round(x + rng.uniform(-0.4, 0.4, pat["n"]), 1)
+        yj = np.round(y * 1.3 + rng.uniform(-0.4, 0.4, pat["n"]), 1)
+        assert np.array_equal(amp.ndOrder(pat, xj, yj, leaf), O.nd_order(pat, xj, yj, leaf))
