@@ -83,6 +83,8 @@ int symToDevice(amp_sym* h) {
   HIP_TRY(upload(h->eaRanges, F.eaRanges));
   HIP_TRY(upload(h->diag, F.diag));
   HIP_TRY(upload(h->gemm, F.gemm));
+  HIP_TRY(upload(h->splits, F.splits));
+  HIP_TRY(upload(h->srefs, F.srefs));
   HIP_TRY(upload(h->sFronts, h->SP.fronts));
   HIP_TRY(upload(h->sBlocks, h->SP.blocks));
   h->dfp.zeros = h->zeros.as<amp::ZeroTask>();
@@ -92,6 +94,8 @@ int symToDevice(amp_sym* h) {
   h->dfp.eaRanges = h->eaRanges.as<amp::EaRange>();
   h->dfp.diag = h->diag.as<amp::DiagTask>();
   h->dfp.gemm = h->gemm.p;
+  h->dfp.splits = h->splits.as<amp::SplitTask>();
+  h->dfp.srefs = h->srefs.as<amp::SplitRef>();
   h->dfp.launches = F.launches;
   h->dsp.fronts = h->sFronts.as<int32_t>();
   h->dsp.blocks = h->sBlocks.as<amp::BlockTask>();
@@ -100,6 +104,9 @@ int symToDevice(amp_sym* h) {
   h->onDevice = true;
   return AMP_OK;
 }
+
+// entries of each of the four split arrays (a multiple of 64, so every array starts 128-byte aligned)
+int64_t splitPitch(const amp::FactorPlan& F) { return (F.splitEntries + 63) / 64 * 64; }
 
 template <typename T>
 amp::FactorArgs<T> factorArgs(amp_num* num) {
@@ -128,6 +135,8 @@ amp::FactorArgs<T> factorArgs(amp_num* num) {
   a.bulkBf16 = num->prec == AMP_F32_BF16X3;
   a.growthLimit = std::is_same<T, float>::value ? num->growthLimit : 0.0;
   a.fallbacks = num->counts.as<int32_t>() + 2;
+  a.split = num->split.as<__bf16>();
+  a.splitPitch = splitPitch(h->F);
   return a;
 }
 
@@ -194,7 +203,8 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
   for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
     it = it->first.rfind("solve_", 0) == 0 ? std::next(it) : num->kernelMs.erase(it);
   if (num->profiling) {
-    static const char* names[] = {"zero", "assemble", "small_front", "extend_add", "diag_block", "gemm_tile"};
+    static const char* names[] = {"zero",       "assemble",  "small_front", "extend_add",
+                                  "diag_block", "gemm_tile", "split_panel"};
     for (size_t i = 0; i < num->prof.used; i++) {
       float k = 0;
       HIP_TRY(hipEventElapsedTime(&k, num->prof.events[2 * i], num->prof.events[2 * i + 1]));
@@ -564,6 +574,8 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   HIP_TRY(num->pert.alloc(size_t(std::max(h->S.n, 1))));
   HIP_TRY(num->failCol.alloc(4));
   HIP_TRY(num->counts.alloc(16));  // nNeg, nPerturbed, FP64 diagonal-block redos
+  if (precision == AMP_F32_BF16X3 && F.splitEntries > 0)
+    HIP_TRY(num->split.alloc(size_t(4 * splitPitch(F)) * sizeof(uint16_t)));
   // static pivoting (LDL^T) can grow the multipliers; a Cholesky factor of an SPD matrix cannot (|L|^2 <= |A|)
   num->growthLimit = precision == AMP_F64 || mode == AMP_CHOLESKY ? 0.0 : kDefaultGrowthLimit;
   HIP_TRY(num->values.alloc(size_t(std::max<int64_t>(h->S.nnzPattern, 1)) * 8));

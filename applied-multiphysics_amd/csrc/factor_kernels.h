@@ -18,6 +18,8 @@ struct DeviceFactorPlan {
   const EaRange* eaRanges = nullptr;
   const DiagTask* diag = nullptr;
   const void* gemm = nullptr;
+  const SplitTask* splits = nullptr;
+  const SplitRef* srefs = nullptr;
   std::vector<Launch> launches;
 };
 
@@ -87,8 +89,11 @@ constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 va
 
 // Bulk trailing-update launch of an FP32 factor on the bf16 matrix cores (three bf16 products per FP32 product; modes 0
 // and 1 of GemmTask, tiles of kGemmEdge).
-void launchGemmBf16x3(const GemmTask* tasks, unsigned grid, float* Lx, float* U, float* W, const float* D, bool ldlt,
+void launchGemmBf16x3(const GemmTask* tasks, const SplitRef* refs, unsigned grid, const FactorArgs<float>& a,
                       hipStream_t st);
+// Split launch in front of such a bulk launch: the panels of `splits` into the split arrays of `a`.
+void launchSplitPanels(const SplitTask* splits, int64_t count, int32_t maxRows, int32_t maxPitch,
+                       const FactorArgs<float>& a, hipStream_t st);
 
 // Dynamic LDS for a side-stream bulk launch of `kernel` (`threads` per workgroup, `staticLds` bytes of static LDS):
 // padding that lets one workgroup fewer fit on a CU than without it (0 when the kernel fits at most two), so that the

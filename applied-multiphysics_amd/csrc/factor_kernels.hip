@@ -543,6 +543,8 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
     const int si = (L.sync & kSyncSide) ? 1 : 0;
     hipStream_t st = si ? side : main;
     if ((L.sync & kSyncWait) && last[1 - si]) (void)hipStreamWaitEvent(st, last[1 - si], 0);
+    // panel splits serve the bf16 bulk products only (their wait, above, still orders the side stream)
+    if (L.kind == kSplit && !(std::is_same<T, float>::value && a.bulkBf16)) continue;
     if (prof && 2 * prof->used + 1 < prof->events.size()) {
       (void)hipEventRecord(prof->events[2 * prof->used], st);
       prof->kinds[prof->used] = L.kind;
@@ -583,12 +585,16 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
       case kDiag:
         launchDiagBlocks<T>(dp.diag + L.first, grid, a, st);
         break;
+      case kSplit:
+        if constexpr (std::is_same<T, float>::value)
+          launchSplitPanels(dp.splits + L.first, L.count, L.maxM, L.edge, a, st);
+        break;
       case kGemm: {
         const GemmTask* gt = reinterpret_cast<const GemmTask*>(dp.gemm) + L.first;
         if constexpr (std::is_same<T, float>::value) {
-          // bulk launches (side stream: modes 0 / 1 only) of a factor with bf16 bulk updates
+          // bulk launches (side stream: modes 0 / 1 only) of a factor with bf16 bulk updates, on their split panel
           if (a.bulkBf16 && L.edge == kGemmEdge && (L.sync & kSyncSide)) {
-            launchGemmBf16x3(gt, grid, a.Lx, a.U, a.W, a.D, a.ldlt != 0, st);
+            launchGemmBf16x3(gt, dp.srefs + L.first, grid, a, st);
             break;
           }
         }

@@ -7,12 +7,19 @@
 // super-panel) stays FP32; only the launches of the side stream (every later column and the Schur complement from a
 // factored super-panel) take this kernel, and FP64 iterative refinement restores the solution's accuracy.
 //
-// Launched with padding LDS (bulkLds) so that each CU keeps room for the panel chain's workgroups.
-// Tiles of 128 x 128 (modes 0: C -= A B^T and 1: C = -A B^T of plan.h, LDL^T: B scaled by the pivots of its depths
-// before the split); eight waves in 4 x 2, each 32 rows x 64 columns of 16 x 16 blocks. Depth chunks of 32: a thread
-// loads 8 consecutive depths of one row of A and of B (FP32, coalesced along rows), splits them and writes the heads
-// and tails to LDS row-major (depth contiguous, 16-byte groups XOR-swizzled by row so that the 16 rows a quarter-wave
-// reads fall in distinct banks). Two LDS buffers, the chunk after next in registers, one barrier per chunk.
+// Every operand row of a bulk launch is a row of its super-panel, which tens of tiles read. The panel is therefore
+// split once, by k_split_panel in front of the bulk launch: heads and tails in chunks of 32 depths, each chunk holding
+// its rows one after another, 32 depths (64 bytes) per row: the layout the matrix cores take an operand fragment in
+// (8 consecutive depths of one row per lane), zero-padded to the chunk; for LDL^T the B operand is split from the
+// pivot-scaled panel. A tile's chunk is then 8 KB of contiguous heads (and tails), which the product kernel loads
+// 16 bytes per lane, 1 KB per instruction, with no conversion.
+//
+// Tiles of 128 x 128 (modes 0: C -= A B^T and 1: C = -A B^T of plan.h); eight waves in 4 x 2, each 32 rows x 64
+// columns of 16 x 16 blocks. Depth chunks of 32: a thread loads the heads and tails of 8 consecutive depths of one row
+// of A and of B and writes them to LDS row-major (16-byte groups XOR-swizzled by row so that the 16 rows a
+// quarter-wave reads fall in distinct banks). Two LDS buffers, the chunk after next in registers, one barrier per
+// chunk.
+#include <algorithm>
 #include <type_traits>
 
 #include "factor_kernels.h"
@@ -24,8 +31,9 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((__vector_size__(16)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kG = 128, kKC = 32;
+constexpr int kG = 128, kKC = kSplitDepth;
 constexpr int kRowBytes = kKC * 2;  // bf16 depth row of a tile row in LDS (64 bytes, four 16-byte groups)
+static_assert(kKC == 32, "four 16-byte groups of depths per tile row");
 
 // byte offset of 16-byte group g (depths 8 g .. 8 g + 7) of tile row r
 __device__ inline int ldsOff(int r, int g) { return r * kRowBytes + 16 * (g ^ ((r >> 2) & 3)); }
@@ -41,61 +49,90 @@ __device__ inline void split8(const float* x, bf16x8& h, bf16x8& l) {
 
 }  // namespace
 
+// Panel split (SplitTask of plan.h): work items of 16 rows x one chunk (4 groups of 8 depths) per wave, lanes 0-15 on
+// consecutive rows, so that every FP32 load instruction reads four 64-byte row runs (rows are contiguous in the
+// column-major panel) and the wave's 16-byte stores cover 1 KB of the chunk contiguously. Depths at or past the panel
+// width are written as zeros.
 template <bool LDLT>
-__global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restrict__ tasks, float* Lx, float* U,
-                                                        float* W, const float* __restrict__ D) {
+__global__ __launch_bounds__(256) void k_split_panel(const SplitTask* __restrict__ tasks, const float* __restrict__ Lx,
+                                                     const float* __restrict__ D, __bf16* Ah, __bf16* Al, __bf16* Bh,
+                                                     __bf16* Bl) {
+  const SplitTask t = tasks[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int32_t ng4 = t.dp / 32;                  // quads of depth groups per row
+  const int64_t items = int64_t((t.rows + 15) / 16) * ng4;  // (16-row block, group quad) per wave
+  const int64_t waves = int64_t(gridDim.x) * 4;
+  for (int64_t it = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); it < items; it += waves) {
+    const int32_t r = int32_t(it / ng4) * 16 + (lane & 15);
+    const int32_t g = int32_t(it % ng4) * 4 + (lane >> 4);
+    if (r >= t.rows) continue;
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int32_t d = 8 * g + j;
+      x[j] = d < t.w ? Lx[t.src + r + int64_t(d) * t.m] : 0.0f;
+    }
+    const int64_t o = t.dst + int64_t(g >> 2) * t.rows * kKC + int64_t(r) * kKC + 8 * (g & 3);
+    bf16x8 h, l;
+    split8(x, h, l);
+    *reinterpret_cast<bf16x8*>(Ah + o) = h;
+    *reinterpret_cast<bf16x8*>(Al + o) = l;
+    if constexpr (LDLT) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int32_t d = 8 * g + j;
+        x[j] = d < t.w ? x[j] * D[t.d_off + d] : 0.0f;
+      }
+      split8(x, h, l);
+      *reinterpret_cast<bf16x8*>(Bh + o) = h;
+      *reinterpret_cast<bf16x8*>(Bl + o) = l;
+    }
+  }
+}
+
+__global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restrict__ tasks,
+                                                        const SplitRef* __restrict__ refs, float* Lx, float* U,
+                                                        const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
+                                                        const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl) {
   constexpr int WTR = 32, WTC = 64, NBI = WTR / 16, NBJ = WTC / 16;
   __shared__ __attribute__((aligned(16))) unsigned char lds[2][4][kG * kRowBytes];  // [buf][A head, A tail, B h, B t]
   const int nblk = gridDim.x, q8 = nblk / 8, r8 = nblk % 8, x8 = blockIdx.x % 8;
-  const GemmTask t = tasks[x8 * q8 + min(x8, r8) + blockIdx.x / 8];
-  auto base = [&](int32_t b) { return b == 0 ? Lx : b == 1 ? U : W; };
-  const float* A = base(t.a_buf) + t.a_off;
-  const float* B = base(t.b_buf) + t.b_off;
-  float* C = base(t.c_buf) + t.c_off;
+  const int ti = x8 * q8 + min(x8, r8) + blockIdx.x / 8;
+  const GemmTask t = tasks[ti];
+  const SplitRef sr = refs[ti];
+  float* C = (t.c_buf == 0 ? Lx : U) + t.c_off;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave & 3) * WTR, wc = (wave >> 2) * WTC;
   const bool active = wr < t.rows && wc < t.cols;  // uniform per wave
-  // loader: tile row li, depths k0 + 8 lg .. + 7
-  const int li = tid & (kG - 1), lg = tid >> 7;
+  // loader: tile row li, depths k0 + 8 lg .. + 7 (four lanes per row: each instruction reads 16 rows' 64 contiguous
+  // bytes); rows past the tile's edge read its last row and are zeroed
+  const int li = tid >> 2, lg = tid & 3;
   const bool rowA = li < t.rows, rowB = li < t.cols;
-  const float* Ar = A + min(li, t.rows - 1);
-  const float* Br = B + min(li, t.cols - 1);
+  const int64_t oa = sr.a + int64_t(min(li, t.rows - 1)) * kKC + 8 * lg;
+  const int64_t ob = sr.b + int64_t(min(li, t.cols - 1)) * kKC + 8 * lg;
+  const int64_t cp = sr.ld;  // chunk pitch
   f4 acc[NBI][NBJ];
 #pragma unroll
   for (int bi = 0; bi < NBI; bi++)
 #pragma unroll
     for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = f4{0, 0, 0, 0};
-  float pa[2][8], pb[2][8];
-  float pd[LDLT ? 2 : 1][LDLT ? 8 : 1];
+  bf16x8 ra[2][2], rb[2][2];  // [register set][head, tail]
   auto load = [&](auto set, int k0) {
     constexpr int S = decltype(set)::value;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int k = min(k0 + 8 * lg + j, t.k - 1);  // clamped: every load is issued, out-of-range depths zeroed later
-      pa[S][j] = Ar[int64_t(k) * t.lda];
-      pb[S][j] = Br[int64_t(k) * t.ldb];
-      if constexpr (LDLT) pd[S][j] = D[t.d_off + k];
-    }
+    const int64_t c = int64_t(k0 / kKC) * cp;
+    ra[S][0] = *reinterpret_cast<const bf16x8*>(Ah + oa + c);
+    ra[S][1] = *reinterpret_cast<const bf16x8*>(Al + oa + c);
+    rb[S][0] = *reinterpret_cast<const bf16x8*>(Bh + ob + c);
+    rb[S][1] = *reinterpret_cast<const bf16x8*>(Bl + ob + c);
   };
-  auto stash = [&](auto set, int buf, int k0) {
+  auto stash = [&](auto set, int buf) {
     constexpr int S = decltype(set)::value;
-    float xa[8], xb[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const bool kin = k0 + 8 * lg + j < t.k;
-      float bv = pb[S][j];
-      if constexpr (LDLT) bv *= pd[S][j];
-      xa[j] = (kin && rowA) ? pa[S][j] : 0.0f;
-      xb[j] = (kin && rowB) ? bv : 0.0f;
-    }
-    bf16x8 ah, al, bh, bl;
-    split8(xa, ah, al);
-    split8(xb, bh, bl);
+    const bf16x8 z = {};
     const int off = ldsOff(li, lg);
-    *reinterpret_cast<bf16x8*>(&lds[buf][0][off]) = ah;
-    *reinterpret_cast<bf16x8*>(&lds[buf][1][off]) = al;
-    *reinterpret_cast<bf16x8*>(&lds[buf][2][off]) = bh;
-    *reinterpret_cast<bf16x8*>(&lds[buf][3][off]) = bl;
+    *reinterpret_cast<bf16x8*>(&lds[buf][0][off]) = rowA ? ra[S][0] : z;
+    *reinterpret_cast<bf16x8*>(&lds[buf][1][off]) = rowA ? ra[S][1] : z;
+    *reinterpret_cast<bf16x8*>(&lds[buf][2][off]) = rowB ? rb[S][0] : z;
+    *reinterpret_cast<bf16x8*>(&lds[buf][3][off]) = rowB ? rb[S][1] : z;
   };
   auto compute = [&](int buf) {
     if (!active) return;
@@ -122,9 +159,10 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
   };
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
+  // the split rows hold whole depth chunks (zero past the panel width): every chunk below t.k is in bounds
   load(S0{}, 0);
-  load(S1{}, kKC);
-  stash(S0{}, 0, 0);
+  if (kKC < t.k) load(S1{}, kKC);
+  stash(S0{}, 0);
   __syncthreads();
   // chunk c is read from buffer c & 1 while chunk c + 1 (in registers) is written to the other buffer and chunk c + 2
   // is loaded into the register set chunk c left; `set` is the register set of chunk c + 1
@@ -134,7 +172,7 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
     asm volatile("" : "+s"(buf));
     if constexpr (decltype(ld)::value) load(std::integral_constant<int, S ^ 1>{}, k0 + 2 * kKC);
     compute(buf);
-    if (k0 + kKC < t.k) stash(set, buf ^ 1, k0 + kKC);
+    if (k0 + kKC < t.k) stash(set, buf ^ 1);
     __syncthreads();
   };
   int k0 = 0;
@@ -145,7 +183,8 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
   step(S1{}, k0, std::false_type{});
   if (k0 + kKC < t.k) step(S0{}, k0 + kKC, std::false_type{});
   if (!active) return;
-  // component q of acc[bi][bj]: C row wr + 16 bi + 4 (lane / 16) + q, column wc + 16 bj + lane % 16
+  // component q of acc[bi][bj]: C row wr + 16 bi + 4 (lane / 16) + q, column wc + 16 bj + lane % 16. Mode 0 reads a
+  // 16-column block's C entries, then writes them (reading all 32 per lane first measured slower)
   const int rowq = wr + 4 * (lane >> 4), col0 = wc + (lane & 15);
 #pragma unroll
   for (int bj = 0; bj < NBJ; bj++) {
@@ -177,16 +216,29 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
   }
 }
 
-void launchGemmBf16x3(const GemmTask* tasks, unsigned grid, float* Lx, float* U, float* W, const float* D, bool ldlt,
-                      hipStream_t st) {
-  constexpr size_t lds = sizeof(unsigned char[2][4][kG * kRowBytes]);
-  if (ldlt)
-    hipLaunchKernelGGL(k_gemm_bf16x3<true>, dim3(grid), dim3(512),
-                       bulkLds(reinterpret_cast<const void*>(&k_gemm_bf16x3<true>), 512, lds), st, tasks, Lx, U, W, D);
+void launchSplitPanels(const SplitTask* splits, int64_t count, int32_t maxRows, int32_t maxPitch,
+                       const FactorArgs<float>& a, hipStream_t st) {
+  if (count <= 0) return;
+  const int64_t p = a.splitPitch;
+  __bf16 *Ah = a.split, *Al = a.split + p, *Bh = a.split + 2 * p, *Bl = a.split + 3 * p;
+  // workgroups of four waves per task, enough for every (16-row block, group quad) of the largest task
+  const int64_t items = int64_t((maxRows + 15) / 16) * std::max(1, maxPitch / 32);
+  const unsigned gx = unsigned(std::min<int64_t>((items + 3) / 4, 1024));
+  if (a.ldlt)
+    hipLaunchKernelGGL(k_split_panel<true>, dim3(gx, unsigned(count)), dim3(256), 0, st, splits, a.Lx, a.D, Ah, Al,
+                       Bh, Bl);
   else
-    hipLaunchKernelGGL(k_gemm_bf16x3<false>, dim3(grid), dim3(512),
-                       bulkLds(reinterpret_cast<const void*>(&k_gemm_bf16x3<false>), 512, lds), st, tasks, Lx, U, W,
-                       D);
+    hipLaunchKernelGGL(k_split_panel<false>, dim3(gx, unsigned(count)), dim3(256), 0, st, splits, a.Lx, a.D, Ah, Al,
+                       Bh, Bl);
+}
+
+void launchGemmBf16x3(const GemmTask* tasks, const SplitRef* refs, unsigned grid, const FactorArgs<float>& a,
+                      hipStream_t st) {
+  const int64_t p = a.splitPitch;
+  const __bf16 *Ah = a.split, *Al = a.split + p;
+  const __bf16* Bh = a.ldlt ? a.split + 2 * p : Ah;  // Cholesky: B is split from the same (unscaled) panel
+  const __bf16* Bl = a.ldlt ? a.split + 3 * p : Al;
+  hipLaunchKernelGGL(k_gemm_bf16x3, dim3(grid), dim3(512), 0, st, tasks, refs, a.Lx, a.U, Ah, Al, Bh, Bl);
 }
 
 }  // namespace amp

@@ -88,7 +88,7 @@ struct amp_sym {
   bool onDevice = false;
   // device copies of the structure and of the task arrays
   amp::DevBuf superStart, Sp, Si, Lxp, Uoff, Woff, childHead, childNext, rel, lowerP, lowerSrc, amap, adst, perm, dDiagSrc;
-  amp::DevBuf zeros, smallFronts, ea, eaRanges, diag, gemm, sFronts, sBlocks;
+  amp::DevBuf zeros, smallFronts, ea, eaRanges, diag, gemm, splits, srefs, sFronts, sBlocks;
   amp::DeviceFactorPlan dfp;
   amp::DeviceSolvePlan dsp;
   amp::PlanParams params;  // the thresholds the plans were built with
@@ -103,6 +103,7 @@ struct amp_num {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evR0 = nullptr, evR1 = nullptr;
   hipEvent_t inEv = nullptr;  // afterCaller
   amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv, pz;
+  amp::DevBuf split;  // bf16 bulk products (AMP_F32_BF16X3): the split panel of the current bulk launch
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
   amp::DevBuf opw;  // shift-invert operator: the products M X (n x nvec, FP64)

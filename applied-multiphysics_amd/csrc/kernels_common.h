@@ -83,6 +83,10 @@ struct FactorArgs {
   double growthLimit;  // FP32 factor: diagonal blocks whose largest elimination multiplier exceeds it are redone in
                        // FP64 arithmetic (0: never)
   int32_t* fallbacks;  // count of such blocks in the current factorization
+  // bf16 bulk products: the split panel of the current bulk launch, four arrays of splitPitch entries (A heads, A
+  // tails, B heads, B tails; for Cholesky the B arrays are the A arrays)
+  __bf16* split;
+  int64_t splitPitch;
 };
 
 }  // namespace amp

@@ -239,6 +239,47 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
           out.push_back(t);
         }
     };
+    // In front of a bulk launch of kGemmEdge tiles (the last launch so far), the split of its panel for the bf16
+    // products: rows [r0, m) of columns [c0, c0 + w) of every front the launch updates, r0 = c2 when it updates
+    // later columns, else nc (the Schur complement's rows only). The split launch takes over the bulk launch's wait
+    // (it reads the factored super-panel); the bulk launch follows it on the side stream. The bulk tasks' split
+    // operands are their A and B rows in the split panel.
+    auto addSplit = [&](int32_t L, int32_t c0, int32_t c2, const std::vector<int32_t>& fronts) {
+      Launch bulk = P.launches.back();
+      P.launches.pop_back();
+      std::vector<int64_t> dst(size_t(ns), -1);
+      std::vector<int32_t> row0(size_t(ns), 0);
+      const int64_t first = int64_t(P.splits.size());
+      int64_t total = 0;
+      int32_t maxRows = 0, maxDp = 0;
+      for (int32_t s : fronts) {
+        const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+        if (nc <= c0 || (nc <= c2 && m == nc)) continue;
+        const int32_t r0 = nc > c2 ? c2 : nc, w = std::min(SPW, nc - c0);
+        const int32_t dp = (w + kSplitDepth - 1) / kSplitDepth * kSplitDepth;
+        P.splits.push_back(SplitTask{S.Lxp[s] + r0 + int64_t(c0) * m, total, m, m - r0, w, dp, S.superStart[s] + c0, 0});
+        dst[s] = total;
+        row0[s] = r0;
+        total += int64_t(m - r0) * dp;
+        maxRows = std::max(maxRows, m - r0);
+        maxDp = std::max(maxDp, dp);
+      }
+      P.splitEntries = std::max(P.splitEntries, total);
+      P.srefs.resize(P.gemm.size(), SplitRef{0, 0, 0, 0});
+      for (int64_t i = bulk.first; i < bulk.first + bulk.count; i++) {
+        const GemmTask& g = P.gemm[size_t(i)];
+        const int32_t s = int32_t(std::upper_bound(S.Lxp.begin(), S.Lxp.end(), g.a_off) - S.Lxp.begin()) - 1;
+        const int32_t m = frontRows(S, s);
+        const int64_t ra = (g.a_off - S.Lxp[s]) % m, rb = (g.b_off - S.Lxp[s]) % m;
+        if (dst[s] < 0 || ra < row0[s] || rb < row0[s]) throw std::logic_error("bulk task outside its split panel");
+        const int32_t cp = (m - row0[s]) * kSplitDepth;  // chunk pitch of the front's split panel
+        P.srefs[size_t(i)] = SplitRef{dst[s] + (ra - row0[s]) * kSplitDepth, dst[s] + (rb - row0[s]) * kSplitDepth, cp, 0};
+      }
+      P.launches.push_back(Launch{kSplit, L, first, int64_t(P.splits.size()) - first, maxRows, 0, maxDp,
+                                  kSyncSide | (bulk.sync & kSyncWait)});
+      bulk.sync &= ~kSyncWait;
+      P.launches.push_back(bulk);
+    };
     for (int32_t c0 = 0; c0 < maxNc; c0 += SPW) {
       for (int32_t k0 = c0; k0 < std::min(maxNc, c0 + SPW); k0 += kTileEdge) {
         const int64_t dFirst = int64_t(P.diag.size());
@@ -308,8 +349,10 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
               if (nc > c2) lowerUpdate(out, edge, s, c2, nc, c0, SPW);
               if (nc > c0) schurUpdate(out, edge, s, c0, std::min(SPW, nc - c0));
             }
-          }, kSyncSide | kSyncWait | kSyncSignal))
+          }, kSyncSide | kSyncWait | kSyncSignal)) {
         sideUsed = true;
+        if (P.launches.back().edge == kGemmEdge) addSplit(L, c0, c2, big);
+      }
     }
     // the next main-stream launch (the update-matrix extend-add, or the next level) joins the side stream: also when
     // the level's last super-panel had no bulk launch (no front with a Schur share or columns left there), since an
@@ -340,6 +383,7 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     }
     P.gemmFlops += 2.0 * entries * t.k;
   }
+  P.srefs.resize(P.gemm.size(), SplitRef{0, 0, 0, 0});
   return P;
 }
 

@@ -36,6 +36,26 @@ struct GemmTask {
   int8_t a_buf, b_buf, c_buf, mode;
 };
 
+// Split of a panel for the bf16 bulk products of FP32 factors (k_gemm_bf16x3): rows [r0, m) of columns
+// [c0, c0 + w) of a tiled front (FP32 at Lx[src], src = Lxp[s] + r0 + c0 m, leading dimension m) into bf16 heads and
+// tails (x = head + tail + O(2^-16 x)), in chunks of kSplitDepth depths: depth d of row r0 + i at
+// dst + (d / kSplitDepth) rows kSplitDepth + i kSplitDepth + d % kSplitDepth, so that a chunk of consecutive rows is
+// contiguous; depths [w, dp) are zero (dp: w rounded up to the chunk). LDL^T also splits the pivot-scaled copy (the B
+// operand, pivots at D[d_off ..]).
+struct SplitTask {
+  int64_t src, dst;
+  int32_t m, rows, w, dp, d_off, pad;
+};
+
+// Split-buffer operands of a bulk product task (aligned with FactorPlan::gemm; unused for the tasks of other
+// launches): element offsets of the first rows of the A and B tiles in their panel's first chunk, and the chunk pitch
+// of that panel (rows kSplitDepth).
+struct SplitRef {
+  int64_t a, b;
+  int32_t ld, pad;
+};
+constexpr int kSplitDepth = 32;  // depth chunk of k_gemm_bf16x3: split rows are padded to a multiple of it
+
 // Factor the nb x nb diagonal block at Lx[f_off] (leading dimension ld) in place and write the inverse of its lower
 // factor (unit lower for LDL^T) to W[w_off] (leading dimension kTileEdge). col0: global index of its first column.
 struct DiagTask {
@@ -65,15 +85,17 @@ enum LaunchKind : int32_t {
   kExtendAdd = 3,
   kDiag = 4,
   kGemm = 5,
+  kSplit = 6,
 };
 
 struct Launch {
   LaunchKind kind;
   int32_t level;
   int64_t first, count;  // range in the task array of this kind (kSmall: in smallFronts; kAssemble: lower entries)
-  int32_t maxM;          // kSmall: largest front of the launch (LDS sizing)
+  int32_t maxM;          // kSmall: largest front of the launch (LDS sizing); kSplit: most rows of a task
   int32_t threads;       // kSmall: workgroup size, or 0 for one wave per front (maxM <= kWaveFrontMax)
-  int32_t edge;          // kGemm: output-tile edge of the launch's tasks (kTileEdge or kGemmEdge)
+  int32_t edge;          // kGemm: output-tile edge of the launch's tasks (kTileEdge or kGemmEdge); kSplit: largest
+                         // row pitch of the launch's tasks
   int32_t sync;          // kSyncSide | kSyncWait | kSyncSignal
 };
 
@@ -102,6 +124,9 @@ struct FactorPlan {
   std::vector<EaRange> eaRanges;
   std::vector<DiagTask> diag;
   std::vector<GemmTask> gemm;
+  std::vector<SplitTask> splits;  // kSplit launches: the panels of bulk launches with kGemmEdge tiles
+  std::vector<SplitRef> srefs;    // aligned with gemm
+  int64_t splitEntries = 0;       // largest split buffer a kSplit launch writes (entries of one array)
   std::vector<Launch> launches;
   int32_t signals = 0;  // launches with kSyncSignal
 };

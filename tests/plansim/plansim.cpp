@@ -298,6 +298,8 @@ int32_t ps_factor(void* p, const double* values, int32_t ldlt, double pivotTol, 
         case kGemm:
           runGemm(c, P.gemm[t]);
           break;
+        case kSplit:  // bf16 operands of the bulk products: no effect on an FP64 replay
+          break;
       }
     }
   }
@@ -359,11 +361,12 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
       }
     }
   };
-  Sh sh[4];
+  Sh sh[5];
   sh[0].init(size_t(std::max<int64_t>(P.nnzLStored, 1)));
   sh[1].init(size_t(std::max<int64_t>(P.uEntries, 1)));
   sh[2].init(size_t(std::max<int64_t>(P.wEntries, 1)));
   sh[3].init(size_t(std::max(S.n, 1)));
+  sh[4].init(size_t(std::max<int64_t>(P.splitEntries, 1)));  // the split panel (one array stands for its four)
   int64_t bad = 0;
   int32_t curL = 0, curT = 0;
   auto conflict = [&](int32_t ol, int32_t ot) {
@@ -446,6 +449,18 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
           for (int32_t k = 0; k < d.nb; k++) acc(3, d.col0 + k, true);
           break;
         }
+        case kSplit: {
+          const SplitTask& sp = P.splits[t];
+          for (int32_t d = 0; d < sp.w; d++) {
+            acc(3, sp.d_off + d, false);
+            for (int32_t r = 0; r < sp.rows; r++) acc(0, sp.src + r + int64_t(d) * sp.m, false);
+          }
+          for (int32_t r = 0; r < sp.rows; r++)
+            for (int32_t d = 0; d < sp.dp; d++)
+              acc(4, sp.dst + int64_t(d / kSplitDepth) * sp.rows * kSplitDepth + int64_t(r) * kSplitDepth + d % kSplitDepth,
+                  true);
+          break;
+        }
         case kGemm: {
           const GemmTask& g = P.gemm[t];
           for (int32_t k = 0; k < g.k; k++) {
@@ -455,6 +470,13 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
           }
           if (g.mode == 2)
             for (int32_t j = 0; j < g.cols; j++) acc(3, g.d_off + j, false);
+          const SplitRef& r = P.srefs[t];
+          if (L.kind == kGemm && L.edge == kGemmEdge && (L.sync & kSyncSide) && r.ld > 0)
+            for (int32_t k = 0; k < g.k; k++) {  // the bf16 kernel reads its operands from the split panel
+              const int64_t c = int64_t(k / kSplitDepth) * r.ld + k % kSplitDepth;
+              for (int32_t i = 0; i < g.rows; i++) acc(4, r.a + c + int64_t(i) * kSplitDepth, false);
+              for (int32_t j = 0; j < g.cols; j++) acc(4, r.b + c + int64_t(j) * kSplitDepth, false);
+            }
           for (int32_t j = 0; j < g.cols; j++)
             for (int32_t i = 0; i < g.rows; i++) {
               const int64_t d = g.c_off + i + int64_t(j) * g.ldc;
