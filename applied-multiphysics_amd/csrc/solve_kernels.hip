@@ -102,10 +102,11 @@ __global__ __launch_bounds__(256) void k_solve_small_fwd(const int32_t* __restri
     const T lnext = (own && col + 1 < nc) ? L[r + int64_t(col + 1) * m] : T(0);
     const int b = col & 1;
     if (r == col) {
+      const S dinv = a.ldlt ? S(1) : S(1) / S(lcur);  // one division per column for every right-hand side
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         if (k >= q) continue;
-        const S x = a.ldlt ? vr[k] : vr[k] / S(lcur);
+        const S x = a.ldlt ? vr[k] : vr[k] * dinv;
         vr[k] = x;
         xc[b][k] = x;
       }
@@ -182,11 +183,12 @@ __global__ __launch_bounds__(256) void k_solve_wave_fwd(const int32_t* __restric
       const int32_t col = c0 + j;
       if (col >= nc) break;
       const S l = lane > col ? S(cur[j]) : S(0);
-      const S d = a.ldlt ? S(1) : S(laneValue(cur[j], col));
+      // the pivot's reciprocal, once per column for every right-hand side
+      const S dinv = a.ldlt ? S(1) : S(1) / S(laneValue(cur[j], col));
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         if (k >= q) continue;
-        const S x = a.ldlt ? laneValue(xr[k], col) : laneValue(xr[k], col) / d;
+        const S x = a.ldlt ? laneValue(xr[k], col) : laneValue(xr[k], col) * dinv;
         xr[k] = lane == col ? x : xr[k] - l * x;
       }
     }
@@ -522,10 +524,11 @@ __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restri
     const T lnext = (own && col > 0) ? Li[col - 1] : T(0);
     const int b = col & 1;
     if (i == col) {
+      const S dinv = a.ldlt ? S(1) : S(1) / S(lcur);  // one division per column for every right-hand side
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         if (k >= q) continue;
-        const S x = a.ldlt ? zr[k] : zr[k] / S(lcur);
+        const S x = a.ldlt ? zr[k] : zr[k] * dinv;
         zr[k] = x;
         xc[b][k] = x;
       }
@@ -592,11 +595,11 @@ __global__ __launch_bounds__(256) void k_solve_wave_bwd(const int32_t* __restric
       const int32_t col = c0 - j;
       if (col < 0) break;
       const S l = lane < col ? S(cur[j]) : S(0);
-      const S d = a.ldlt ? S(1) : S(laneValue(cur[j], col));
+      const S dinv = a.ldlt ? S(1) : S(1) / S(laneValue(cur[j], col));
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         if (k >= q) continue;
-        const S x = a.ldlt ? laneValue(z[k], col) : laneValue(z[k], col) / d;
+        const S x = a.ldlt ? laneValue(z[k], col) : laneValue(z[k], col) * dinv;
         z[k] = lane == col ? x : z[k] - l * x;
       }
     }
