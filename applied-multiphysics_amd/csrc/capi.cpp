@@ -748,9 +748,7 @@ int enqueueSequence(amp_graph* g, const double* d_values, const double* d_b, dou
   double* r = g->r.as<double>();
   double* d = g->d.as<double>();
   for (int32_t it = 0; it < irSteps; it++) {
-    for (int32_t k = 0; k < q; k++)
-      amp::launchResidual(h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_values, d_x + int64_t(k) * n,
-                          d_b + int64_t(k) * n, r + int64_t(k) * n, n, st);
+    amp::launchResidualMulti(h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_values, d_x, d_b, r, n, q, st);
     HIP_TRY(solve(r, d));
     amp::launchAxpby(1.0, d, 1.0, d_x, n * q, st);
   }
@@ -853,7 +851,7 @@ int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
 bool validParams(const amp::PlanParams& p) {
   return p.smallMax >= 1 && p.smallMax <= amp::kSmallMax && p.smallNarrow >= 1 && p.smallNarrow <= p.smallMax &&
          p.wideLaunch >= 1 && p.superPanel >= amp::kTileEdge && p.superPanel % amp::kTileEdge == 0 &&
-         p.flowWideMin >= 0;
+         p.flowWideMin >= 0 && p.packMin >= 0;
 }
 
 // factor + one solve (zero right-hand side), device milliseconds of the second of two runs
@@ -889,6 +887,7 @@ int amp_sym_set_params(amp_sym* h, const amp_tune_info* t) {
   p.wideLaunch = t->wideLaunch;
   p.superPanel = t->superPanel;
   p.flowWideMin = t->flowWideMin;
+  p.packMin = t->packMin;
   if (!validParams(p)) return fail(AMP_ERR_ARG, "amp_sym_set_params: thresholds out of range");
   return rebuildPlans(h, p);
 }
@@ -911,6 +910,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
   // one threshold at a time, from the defaults
   const int32_t superPanels[] = {128, 512}, wides[] = {256, 1024, 1 << 30}, smalls[] = {64}, narrows[] = {16, 64};
   const int32_t flowWides[] = {64, 1 << 30};  // forward solve launches: wide tasks from 64 tasks on, or never
+  const int32_t packs[] = {128, 1 << 30};     // whole fronts of at most 8 / 16 rows packed from 128 on, or never
   auto attempt = [&](amp::PlanParams p) -> int {
     if (!validParams(p)) return AMP_OK;
     int r = rebuildPlans(h, p);
@@ -951,6 +951,11 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     p.flowWideMin = v;
     if ((rc = attempt(p))) return rc;
   }
+  for (int32_t v : packs) {
+    amp::PlanParams p = best;
+    p.packMin = v;
+    if ((rc = attempt(p))) return rc;
+  }
   rc = rebuildPlans(h, best);
   if (rc) return rc;
   if (info) {
@@ -959,6 +964,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     info->wideLaunch = best.wideLaunch;
     info->superPanel = best.superPanel;
     info->flowWideMin = best.flowWideMin;
+    info->packMin = best.packMin;
     info->candidates = tried;
     info->defaultMs = defaultMs;
     info->bestMs = bestMs;

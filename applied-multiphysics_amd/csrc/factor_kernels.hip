@@ -156,6 +156,81 @@ __global__ __launch_bounds__(256) void k_small_front_wave(const int32_t* __restr
   }
 }
 
+// Fronts with m <= G (8 or 16), 64 / G per wave: segment g (lanes G g .. G g + G - 1) takes one front through the
+// steps of k_small_front_wave, lane j of the segment holding column j, the pivot column's entries read from the
+// segment's lane k with ds_bpermute. The elimination loop runs in step across the wave (its exit is taken when no
+// segment has a column left) and a segment's own sizes enter through selects, so all lanes of a segment are active at
+// every cross-lane read; the assembly and store loops may diverge between segments.
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_small_front_seg(const int32_t* __restrict__ fronts, int32_t count,
+                                                         FactorArgs<T> a, const double* __restrict__ values) {
+  constexpr int FW = 64 / G;  // fronts per wave
+  __shared__ T Fs[4][FW][G * G];
+  chainPriority();
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int sl = lane % G;  // lane within the segment
+  const int64_t idx0 = (int64_t(blockIdx.x) * 4 + w) * FW;
+  if (idx0 >= count) return;  // whole waves only: no workgroup barrier below
+  const int64_t idx = idx0 + lane / G;
+  T* F = Fs[w][lane / G];
+  int32_t s = -1, f = 0, nc = 0, m = 0;
+  if (idx < count) {  // segments past the end keep an empty front
+    s = fronts[idx];
+    f = a.superStart[s];
+    nc = a.superStart[s + 1] - f;
+    m = a.Sp[s + 1] - a.Sp[s];
+  }
+  const int32_t mu = m - nc;
+  for (int i = sl; i < m * m; i += G) F[i] = T(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (s >= 0)
+    for (int32_t q = a.lowerP[f] + sl; q < a.lowerP[f + nc]; q += G) F[a.amap[q]] = T(0.0 + values[a.lowerSrc[q]]);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int32_t c = s >= 0 ? a.childHead[s] : -1; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t* rel = a.rel + a.Sp[c] + ncc;
+    const T* Uc = a.U + a.Uoff[c];
+    for (int i = sl; i < muc * muc; i += G) {
+      const int r = i % muc, b = i / muc;
+      if (r >= b) F[rel[r] + rel[b] * m] += Uc[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  const bool own = sl < m;
+  T cv[G];
+#pragma unroll
+  for (int i = 0; i < G; i++) cv[i] = (own && i < m) ? (i >= sl ? F[i + sl * m] : F[sl + i * m]) : T(0);
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    if (!waveAny(k < nc)) break;
+    const bool act = k < nc;
+    T inv, mult;
+    const T piv = pivotRule<T>(segValue<G>(cv[k], k), f + k, a, &inv, &mult, act && sl == 0);
+    const T ljm = sl > k ? cv[k] * inv * mult : T(0);  // l_j d_k (row k of column j is F[j][k] by symmetry)
+    if (act && sl == k) cv[k] = piv;
+#pragma unroll
+    for (int i = k + 1; i < G; i++) {
+      const T li = segValue<G>(cv[i], k) * inv;
+      cv[i] = !act ? cv[i] : sl == k ? li : cv[i] - li * ljm;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+  for (int i = 0; i < G; i++)
+    if (own && i < m) F[i + sl * m] = i >= sl ? cv[i] : T(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (s < 0) return;
+  T* L = a.Lx + a.Lxp[s];
+  for (int i = sl; i < m * nc; i += G) L[i] = F[i];
+  if (mu > 0) {
+    T* Us = a.U + a.Uoff[s];
+    for (int i = sl; i < mu * mu; i += G) {
+      const int r = i % mu, b = i / mu;
+      Us[i] = F[(nc + r) + (nc + b) * m];
+    }
+  }
+}
+
 // Extend-add into parent columns [j0, j1) of a tiled front. The plan lists, per work item, the column range of
 // every child that lands there (each child column maps to one parent column through the increasing relative map).
 // Children are added one after another (reference order); a child's entries land on distinct parent entries, so the
@@ -560,6 +635,16 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         break;
       }
       case kSmall: {
+        if (L.threads == 0 && L.edge > 0) {  // 64 / edge fronts per wave
+          const unsigned wg = unsigned((L.count + 4 * (64 / L.edge) - 1) / (4 * (64 / L.edge)));
+          if (L.edge == 8)
+            hipLaunchKernelGGL((k_small_front_seg<T, 8>), dim3(wg), dim3(256), 0, st, dp.smallFronts + L.first,
+                               int32_t(L.count), a, values);
+          else
+            hipLaunchKernelGGL((k_small_front_seg<T, 16>), dim3(wg), dim3(256), 0, st, dp.smallFronts + L.first,
+                               int32_t(L.count), a, values);
+          break;
+        }
         if (L.threads == 0) {  // one wave per front
           const unsigned wg = unsigned((L.count + 3) / 4);
           if (L.maxM <= 16)

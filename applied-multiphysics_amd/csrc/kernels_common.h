@@ -56,6 +56,27 @@ __device__ inline float laneValue(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// Lane l's value within each segment of G lanes (lanes G g .. G g + G - 1 of the wave read lane G g + l), through
+// ds_bpermute, which reads nothing of LDS memory. l may differ between segments; the source lane must be active (an
+// inactive one gives 0).
+template <int G>
+__device__ inline float segValue(float v, int l) {
+  static_assert(G > 0 && G < 64 && (G & (G - 1)) == 0, "segments are a power of two below the wave");
+  const int src = ((int(threadIdx.x & 63) & ~(G - 1)) + l) << 2;
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v)));
+}
+template <int G>
+__device__ inline double segValue(double v, int l) {
+  static_assert(G > 0 && G < 64 && (G & (G - 1)) == 0, "segments are a power of two below the wave");
+  const int src = ((int(threadIdx.x & 63) & ~(G - 1)) + l) << 2;
+  const int lo = __builtin_amdgcn_ds_bpermute(src, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(src, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// Whether any lane of the wave has `p` (a wave-uniform answer, for loop exits of segment-varying bounds).
+__device__ inline bool waveAny(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
 // Kernel-side view of the factor storage and of the plan's per-supernode arrays.
 template <typename T>
 struct FactorArgs {

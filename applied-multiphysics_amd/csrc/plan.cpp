@@ -149,19 +149,29 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
 
   for (int32_t L = 0; L < P.nlevels; L++) {
     const std::vector<int32_t>& fronts = byLevel[L];
-    // fronts factored in LDS, four size classes: one wave per front up to 16 rows and up to kWaveFrontMax rows, then
-    // a workgroup of 64 threads up to smallNarrow rows, else 256
-    for (int cls = 0; cls < 4; cls++) {
+    // fronts factored in LDS, five size classes: up to 8 rows eight to a wave when the level has at least packMin of
+    // them (else with the next class), up to 16 rows four to a wave when the level has at least packMin of them (else
+    // one wave each), one wave per front up to kWaveFrontMax rows, then a workgroup of 64 threads up to smallNarrow
+    // rows, else 256
+    int64_t n8 = 0, n16 = 0;
+    for (int32_t s : fronts)
+      if (!P.isBig[s]) {
+        n8 += frontRows(S, s) <= 8;
+        n16 += frontRows(S, s) <= 16;
+      }
+    const bool pack8 = n8 >= prm.packMin, pack16 = n16 - (pack8 ? n8 : 0) >= prm.packMin;
+    for (int cls = 0; cls < 5; cls++) {
       const int64_t first = int64_t(P.smallFronts.size());
       int32_t maxM = 0;
       for (int32_t s : fronts) {
         const int32_t m = frontRows(S, s);
-        const int c = m <= 16 ? 0 : m <= kWaveFrontMax ? 1 : m <= prm.smallNarrow ? 2 : 3;
+        const int c = (m <= 8 && pack8) ? 0 : m <= 16 ? 1 : m <= kWaveFrontMax ? 2 : m <= prm.smallNarrow ? 3 : 4;
         if (P.isBig[s] || c != cls) continue;
         P.smallFronts.push_back(s);
         maxM = std::max(maxM, m);
       }
-      push(kSmall, L, first, int64_t(P.smallFronts.size()) - first, maxM, cls < 2 ? 0 : cls == 2 ? 64 : 256);
+      const int32_t seg = cls == 0 ? 8 : (cls == 1 && pack16) ? 16 : 0;
+      push(kSmall, L, first, int64_t(P.smallFronts.size()) - first, maxM, cls < 3 ? 0 : cls == 3 ? 64 : 256, seg);
     }
     std::vector<int32_t> big;
     for (int32_t s : fronts)
@@ -406,17 +416,28 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F, const PlanParam
   };
   auto nblocks = [&S](int32_t s) { return (frontCols(S, s) + kTileEdge - 1) / kTileEdge; };
   // the whole-front supernodes of a level: those with m <= kSolveWaveMax (one wave each) and the rest, one launch each
+  // (packed several to a wave: up to 8 rows when the level has at least packMin of them, then up to 16 rows when it
+  // has at least packMin of those left)
   auto pushSmall = [&](SolveKind kind, int32_t L) {
-    for (int pass = 0; pass < 2; pass++) {
+    int64_t n8 = 0, n16 = 0;
+    for (int32_t s : byLevel[L])
+      if (!F.isBig[s]) {
+        n8 += frontRows(S, s) <= 8;
+        n16 += frontRows(S, s) <= 16;
+      }
+    const bool pack8 = n8 >= prm.packMin, pack16 = n16 - (pack8 ? n8 : 0) >= prm.packMin;
+    for (int cls = 0; cls < 4; cls++) {
       const int64_t first = int64_t(P.fronts.size());
       int32_t maxM = 0;
       for (int32_t s : byLevel[L]) {
         const int32_t m = frontRows(S, s);
-        if (F.isBig[s] || (m <= kSolveWaveMax) != (pass == 0)) continue;
+        const int c = (m <= 8 && pack8) ? 0 : (m <= 16 && pack16) ? 1 : m <= kSolveWaveMax ? 2 : 3;
+        if (F.isBig[s] || c != cls) continue;
         P.fronts.push_back(s);
         maxM = std::max(maxM, m);
       }
       push(kind, L, first, int64_t(P.fronts.size()) - first, maxM);
+      if (cls < 2 && int64_t(P.fronts.size()) > first) P.launches.back().seg = cls == 0 ? 8 : 16;
     }
   };
   // a flow launch with its task lists for FP32 and FP64 factors: `emit(nq)` appends the tasks of nq blocks to P.blocks;

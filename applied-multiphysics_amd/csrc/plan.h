@@ -95,7 +95,8 @@ struct Launch {
   int32_t maxM;          // kSmall: largest front of the launch (LDS sizing); kSplit: most rows of a task
   int32_t threads;       // kSmall: workgroup size, or 0 for one wave per front (maxM <= kWaveFrontMax)
   int32_t edge;          // kGemm: output-tile edge of the launch's tasks (kTileEdge or kGemmEdge); kSplit: largest
-                         // row pitch of the launch's tasks
+                         // row pitch of the launch's tasks; kSmall with threads 0: lanes per front when a wave takes
+                         // several (8 or 16, at least maxM), else 0
   int32_t sync;          // kSyncSide | kSyncWait | kSyncSignal
 };
 
@@ -132,6 +133,10 @@ struct FactorPlan {
 };
 
 constexpr int kFlowWideMinTasks = 160;  // default of PlanParams::flowWideMin (see the solve plan below)
+// Whole fronts of at most 8 (or 16) rows of a level are packed 8 (or 4) to a wave, in launches of their own, when the
+// level has at least PlanParams::packMin of them (levels of many tiny fronts, where one wave per front is bound by
+// the latency of its handful of loads); fewer go with the fronts of the next size class.
+constexpr int kPackMinFronts = 512;
 
 // Thresholds that pick kernels by supernode shape; the defaults are the constants above, and a device autotuner may
 // choose others (each at most its constant, which sizes the kernels' static storage).
@@ -143,6 +148,7 @@ struct PlanParams {
   int32_t superPanel = kSuperPanel;   // columns of a super-panel (a multiple of kTileEdge)
   int32_t flowWideMin = kFlowWideMinTasks;  // wide tasks a forward solve launch needs to use them (else one block per
                                             // task)
+  int32_t packMin = kPackMinFronts;         // fronts of a size class a level needs to pack them several to a wave
 };
 
 FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams());
@@ -203,6 +209,7 @@ struct SolveLaunch {
                          // front
   int64_t first2 = 0, count2 = 0;  // flow kinds: range in `blocks` of the same launch for FP64 factors
   int32_t nq = 1, nq2 = 1;          // flow kinds: blocks per task of the two lists (1, or kFlowQuadsF32 / kFlowQuadsF64)
+  int32_t seg = 0;                  // small kinds: lanes per front when a wave takes several (8 or 16), else 0
 };
 
 struct SolvePlan {

@@ -417,9 +417,8 @@ int runShiftInvert(amp_num* num, const double* d_M, const double* d_X, double* d
   if (rc) return rc;
   if ((rc = ensure(num->opw, size_t(n) * nvec * 8))) return rc;
   double* w = num->opw.as<double>();
-  for (int32_t k = 0; k < nvec; k++)
-    amp::launchResidual(h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_M, d_X + int64_t(k) * n, nullptr,
-                        w + int64_t(k) * n, n, num->stream);
+  amp::launchResidualMulti(h->dRowptr.as<int32_t>(), h->dColidx.as<int32_t>(), d_M, d_X, nullptr, w, n, nvec,
+                           num->stream);
   HIP_TRY(hipGetLastError());
   return amp::solveOnDevice(num, w, d_Y, nvec);
 }

@@ -10,6 +10,9 @@ constexpr int kMaxRestart = 64;
 
 void launchResidual(const int32_t* rowptr, const int32_t* colidx, const double* val, const double* x, const double* b,
                     double* r, int32_t n, hipStream_t st);
+// R_k = B_k - A X_k for q right-hand sides n apart (B == nullptr: R_k = A X_k), the matrix read once per eight
+void launchResidualMulti(const int32_t* rowptr, const int32_t* colidx, const double* val, const double* X,
+                         const double* B, double* R, int32_t n, int32_t q, hipStream_t st);
 // out[j] = V_j . w for j < nv (deterministic two-stage reduction through `partial`)
 void launchMultiDot(const double* V, int64_t ldv, int32_t nv, const double* w, int32_t n, double* partial, double* out,
                     hipStream_t st);

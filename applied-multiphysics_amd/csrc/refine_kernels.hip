@@ -40,6 +40,33 @@ __global__ __launch_bounds__(256) void k_csr_residual(const int32_t* __restrict_
   }
 }
 
+// R_k = B_k - A X_k for k < q <= Q right-hand sides (each n apart), one row per thread: the row's entries are read once
+// for all of them, in the order of k_csr_residual (so each R_k equals its result bit for bit); B == nullptr gives
+// R_k = A X_k
+template <int Q>
+__global__ __launch_bounds__(256) void k_csr_residual_multi(const int32_t* __restrict__ rowptr,
+                                                            const int32_t* __restrict__ colidx,
+                                                            const double* __restrict__ val,
+                                                            const double* __restrict__ X,
+                                                            const double* __restrict__ B, double* __restrict__ R,
+                                                            int32_t n, int32_t q) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) s[k] = (B && k < q) ? B[int64_t(k) * n + i] : 0.0;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; p++) {
+      const double a = val[p];
+      const int64_t c = colidx[p];
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) s[k] -= a * X[int64_t(k) * n + c];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; k++)
+      if (k < q) R[int64_t(k) * n + i] = B ? s[k] : -s[k];
+  }
+}
+
 // partial[j * gridDim.x + block] = sum over the block's rows of V_j . w, for j < nv (V_j = V + j ldv)
 __global__ __launch_bounds__(kRedThreads) void k_multi_dot(const double* __restrict__ V, int64_t ldv, int32_t nv,
                                                             const double* __restrict__ w, int32_t n,
@@ -138,6 +165,16 @@ static unsigned gridFor(int32_t n) { return unsigned(std::min<int64_t>((int64_t(
 void launchResidual(const int32_t* rowptr, const int32_t* colidx, const double* val, const double* x, const double* b,
                     double* r, int32_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_csr_residual, dim3(gridFor(n)), dim3(256), 0, st, rowptr, colidx, val, x, b, r, n);
+}
+
+void launchResidualMulti(const int32_t* rowptr, const int32_t* colidx, const double* val, const double* X,
+                         const double* B, double* R, int32_t n, int32_t q, hipStream_t st) {
+  constexpr int Q = 8;
+  for (int32_t k0 = 0; k0 < q; k0 += Q) {
+    const int64_t o = int64_t(k0) * n;
+    hipLaunchKernelGGL(k_csr_residual_multi<Q>, dim3(gridFor(n)), dim3(256), 0, st, rowptr, colidx, val, X + o,
+                       B ? B + o : nullptr, R + o, n, std::min(Q, q - k0));
+  }
 }
 
 void launchMultiDot(const double* V, int64_t ldv, int32_t nv, const double* w, int32_t n, double* partial, double* out,

@@ -203,13 +203,82 @@ __global__ __launch_bounds__(256) void k_solve_wave_fwd(const int32_t* __restric
   }
 }
 
+// Whole fronts of at most G (8 or 16) rows, 64 / G per wave: segment g of the wave (lanes G g .. G g + G - 1) solves
+// one front by the steps of k_solve_wave_fwd, lane r of the segment keeping row r, each column's solution values read
+// from the segment's pivot lane with ds_bpermute. All G columns of a lane's row of L are loaded at once. The column
+// loop runs in step across the wave (it ends when no segment has a column left) and a segment's own sizes enter
+// through selects, so all lanes of a segment are active at every cross-lane read.
+template <typename T, typename S, int Q, int G>
+__global__ __launch_bounds__(256) void k_solve_seg_fwd(const int32_t* __restrict__ fronts, int32_t count,
+                                                       SolveArgs<T> a, S* y, S* fv, int32_t q, int64_t fvN) {
+  constexpr int FW = 64 / G;  // fronts per wave
+  __shared__ S v[4][Q][64];   // the wave's front vectors: segment g at G g
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int sl = lane % G, base = lane - sl;
+  const int64_t idx0 = (int64_t(blockIdx.x) * 4 + w) * FW;
+  if (idx0 >= count) return;  // whole waves only: no workgroup barrier below
+  const int64_t idx = idx0 + lane / G;
+  int32_t s = -1, f = 0, nc = 0, m = 0;
+  if (idx < count) {  // segments past the end keep an empty front
+    s = fronts[idx];
+    f = a.superStart[s];
+    nc = a.superStart[s + 1] - f;
+    m = a.Sp[s + 1] - a.Sp[s];
+  }
+  const T* L = a.Lx + (s >= 0 ? a.Lxp[s] : 0);
+  T lr[G];  // this lane's row of L: lr[c] = L[sl][c]
+#pragma unroll
+  for (int c = 0; c < G; c++) lr[c] = (sl < m && c < nc) ? L[sl + int64_t(c) * m] : T(0);
+#pragma unroll
+  for (int k = 0; k < Q; k++)
+    if (k < q) v[w][k][lane] = sl < nc ? y[int64_t(k) * a.n + f + sl] : S(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int32_t c = s >= 0 ? a.childHead[s] : -1; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t o = a.Sp[c] + ncc;
+    if (sl < muc) {
+      const int32_t r = a.rel[o + sl];
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (k < q) v[w][k][base + r] += fv[k * fvN + o + sl];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  S xr[Q];
+#pragma unroll
+  for (int k = 0; k < Q; k++) xr[k] = k < q ? v[w][k][lane] : S(0);
+#pragma unroll
+  for (int col = 0; col < G; col++) {
+    if (!waveAny(col < nc)) break;
+    const bool act = col < nc;
+    const S l = sl > col ? S(lr[col]) : S(0);
+    // the pivot's reciprocal, once per column for every right-hand side
+    const S dinv = a.ldlt ? S(1) : S(1) / S(segValue<G>(lr[col], col));
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      if (k >= q) continue;
+      const S xc = segValue<G>(xr[k], col);
+      const S x = a.ldlt ? xc : xc * dinv;
+      xr[k] = !act ? xr[k] : sl == col ? x : xr[k] - l * x;
+    }
+  }
+  if (s < 0) return;
+#pragma unroll
+  for (int k = 0; k < Q; k++) {
+    if (k >= q) continue;
+    if (sl < nc) y[int64_t(k) * a.n + f + sl] = xr[k];
+    if (sl < m) fv[k * fvN + a.Sp[s] + sl] = xr[k];
+  }
+}
+
 // Front vectors of the tiled supernodes of a level, rows [kGatherRows y, kGatherRows (y + 1)) of front fronts[x] per
 // workgroup (x, y), so that the large fronts near the root are gathered by many workgroups. Every workgroup walks each
-// child's whole relative map and keeps the rows that land in its slice (the map increases, so a lane stops at the first
-// row past the slice); per entry the order is the single-workgroup one: own value or zero, then the children in order.
+// child's whole relative map once for all right-hand sides and keeps the rows that land in its slice (the map
+// increases, so a lane stops at the first row past the slice); per entry the order is the single-workgroup one: own
+// value or zero, then the children in order.
 constexpr int kGatherRows = 512;
 
-template <typename T, typename S>
+template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y, S* fv,
                                                       int32_t q, int64_t fvN) {
   const int32_t s = fronts[blockIdx.x];
@@ -217,41 +286,45 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
   const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(m, i0 + kGatherRows);
   if (i0 >= m) return;  // whole workgroup
   // the slice of v is accumulated in LDS (children in order, one barrier each) and stored once
-  __shared__ S vs[kGatherRows];
-  for (int32_t k = 0; k < q; k++) {
-    const S* yk = y + int64_t(k) * a.n;
-    const S* fk = fv + k * fvN;
-    S* v = fv + k * fvN + a.Sp[s];
-    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) vs[i - i0] = i < nc ? yk[f + i] : S(0);
-    __syncthreads();
-    for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
-      const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
-      const int32_t o = a.Sp[c] + ncc;
-      // eight entries per lane are read before any is used, so walking a long map to the slice costs a round trip
-      // per 8 x 256 entries, not per 256
-      constexpr int U = 8;
-      for (int32_t r0 = threadIdx.x; r0 < muc; r0 += U * int32_t(blockDim.x)) {
-        int32_t iv[U];
-        S vv[U];
+  __shared__ S vs[Q][kGatherRows];
+  for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int32_t r = r0 + u * int32_t(blockDim.x);
-          iv[u] = r < muc ? a.rel[o + r] : INT32_MAX;
-          vv[u] = r < muc ? fk[o + r] : S(0);
-        }
-        bool past = false;
+    for (int k = 0; k < Q; k++)
+      if (k < q) vs[k][i - i0] = i < nc ? y[int64_t(k) * a.n + f + i] : S(0);
+  __syncthreads();
+  for (int32_t c = a.childHead[s]; c != -1; c = a.childNext[c]) {
+    const int32_t ncc = a.superStart[c + 1] - a.superStart[c], muc = a.Sp[c + 1] - a.Sp[c] - ncc;
+    const int32_t o = a.Sp[c] + ncc;
+    // U entries per lane are read before any is used, so walking a long map to the slice costs a round trip per
+    // U x 256 entries, not per 256
+    constexpr int U = Q == 1 ? 8 : 2;
+    for (int32_t r0 = threadIdx.x; r0 < muc; r0 += U * int32_t(blockDim.x)) {
+      int32_t iv[U];
+      S vv[U][Q];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          past = past || iv[u] >= i1;
-          if (!past && iv[u] >= i0) vs[iv[u] - i0] += vv[u];
-        }
-        if (past) break;
+      for (int u = 0; u < U; u++) {
+        const int32_t r = r0 + u * int32_t(blockDim.x);
+        iv[u] = r < muc ? a.rel[o + r] : INT32_MAX;
+#pragma unroll
+        for (int k = 0; k < Q; k++) vv[u][k] = (r < muc && k < q) ? fv[k * fvN + o + r] : S(0);
       }
-      __syncthreads();
+      bool past = false;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        past = past || iv[u] >= i1;
+        if (!past && iv[u] >= i0)
+#pragma unroll
+          for (int k = 0; k < Q; k++)
+            if (k < q) vs[k][iv[u] - i0] += vv[u][k];
+      }
+      if (past) break;
     }
-    for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) v[i] = vs[i - i0];
     __syncthreads();
   }
+  for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < Q; k++)
+      if (k < q) fv[k * fvN + a.Sp[s] + i] = vs[k][i - i0];
 }
 
 namespace {
@@ -611,52 +684,149 @@ __global__ __launch_bounds__(256) void k_solve_wave_bwd(const int32_t* __restric
     if (k < q && lane < nc) y[int64_t(k) * a.n + f + lane] = z[k];
 }
 
+// Backward counterpart of k_solve_seg_fwd: lane i of segment g keeps solution row i of the segment's supernode and
+// loads column i of L whole; the below rows' values are held by the segment's lanes 0..mu-1 and read with ds_bpermute.
+template <typename T, typename S, int Q, int G>
+__global__ __launch_bounds__(256) void k_solve_seg_bwd(const int32_t* __restrict__ fronts, int32_t count,
+                                                       SolveArgs<T> a, S* y, int32_t q) {
+  constexpr int FW = 64 / G;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int sl = lane % G;
+  const int64_t idx0 = (int64_t(blockIdx.x) * 4 + w) * FW;
+  if (idx0 >= count) return;
+  const int64_t idx = idx0 + lane / G;
+  int32_t s = -1, f = 0, nc = 0, m = 0;
+  if (idx < count) {
+    s = fronts[idx];
+    f = a.superStart[s];
+    nc = a.superStart[s + 1] - f;
+    m = a.Sp[s + 1] - a.Sp[s];
+  }
+  const int32_t mu = m - nc;
+  const bool colValid = sl < nc;
+  const T* Li = a.Lx + (s >= 0 ? a.Lxp[s] : 0) + int64_t(colValid ? sl : 0) * m;  // column sl of L
+  T lc[G], lb[G];  // lc[c] = L[c][sl] (in-block rows), lb[r] = L[nc + r][sl] (below rows)
+#pragma unroll
+  for (int r = 0; r < G; r++) {
+    lc[r] = (colValid && r < nc) ? Li[r] : T(0);
+    lb[r] = (colValid && r < mu) ? Li[nc + r] : T(0);
+  }
+  const int32_t br = (s >= 0 && sl < mu) ? a.Si[a.Sp[s] + nc + sl] : 0;
+  S xb[Q], z[Q];
+#pragma unroll
+  for (int k = 0; k < Q; k++) {
+    xb[k] = (k < q && sl < mu) ? y[int64_t(k) * a.n + br] : S(0);
+    z[k] = (k < q && colValid) ? y[int64_t(k) * a.n + f + sl] : S(0);
+  }
+  // z_i -= sum_r L[nc + r][i] xb_r
+#pragma unroll
+  for (int r = 0; r < G; r++) {
+    if (!waveAny(r < mu)) break;
+    const S l = S(lb[r]);
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      if (k >= q) continue;
+      const S xv = segValue<G>(xb[k], r);
+      z[k] = r < mu ? z[k] - l * xv : z[k];
+    }
+  }
+  // in-block backward substitution, last column first: lane i < col applies L[col][i]
+#pragma unroll
+  for (int col = G - 1; col >= 0; col--) {
+    if (!waveAny(col < nc)) continue;
+    const bool act = col < nc;
+    const S l = sl < col ? S(lc[col]) : S(0);
+    const S dinv = a.ldlt ? S(1) : S(1) / S(segValue<G>(lc[col], col));
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      if (k >= q) continue;
+      const S zc = segValue<G>(z[k], col);
+      const S x = a.ldlt ? zc : zc * dinv;
+      z[k] = !act ? z[k] : sl == col ? x : z[k] - l * x;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < Q; k++)
+    if (k < q && colValid) y[int64_t(k) * a.n + f + sl] = z[k];
+}
+
+// Sums over the 64 lanes of a wave of V values per lane (V a power of two, at most 64) with V - 1 + log2(64 / V)
+// exchanges instead of 6 V. The lanes hold their values permuted: lane l keeps value v in slot v ^ spreadMask<V>(l)
+// (the low log2 V lane bits in reverse order). Step j pairs lane l with l ^ 2^j, which differs in the slot bit of
+// weight h = V / 2^(j+1): each lane adds the partner's slot i + h to its own slot i (i < h), both the same value, and
+// keeps the lower half. Plain butterflies over the remaining lane bits finish; lane l then holds in a[0] the whole-wave
+// sum of value spreadMask<V>(l). Every register index is a constant, so no step selects between slots at run time.
+template <int V>
+__host__ __device__ constexpr int spreadMask(int l) {
+  int r = 0;
+  for (int w = V >> 1, b = 1; w > 0; w >>= 1, b <<= 1)
+    if (l & b) r |= w;
+  return r;
+}
+template <int V, typename S>
+__device__ inline S waveSumSpread(S (&a)[V]) {
+  static_assert(V > 0 && V <= 64 && (V & (V - 1)) == 0, "a power of two of values, at most one per lane");
+  constexpr int LV = __builtin_ctz(V);
+#pragma unroll
+  for (int j = 0; j < LV; j++)
+#pragma unroll
+    for (int i = 0; i < (V >> (j + 1)); i++) a[i] += __shfl_xor(a[i + (V >> (j + 1))], 1 << j, 64);
+  S v = a[0];
+#pragma unroll
+  for (int o = V; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // Partial products of the backward initialisation: for the task's column block and below rows [r0, r1) (at most
 // kInitRows), pz[slot][jj][c] = sum_r L[nc + r, c] y[below r] (y[below] gathered into the front vector). Wave w handles
-// columns 16 w .. 16 w + 15, four at a time; lane i takes rows r0 + i, r0 + 64 + i, ..., so every load instruction
-// reads 64 consecutive rows of one column, and the column sums are reduced across the wave at the end.
+// columns 16 w .. 16 w + 15, CG at a time; lane i takes rows r0 + i, r0 + 64 + i, ..., so every load instruction reads
+// 64 consecutive rows of one column. Each lane gathers its rows' y values once. The CG Q column sums of a group are
+// reduced across the wave together (waveSumSpread): lane l accumulates value v = c Q + jj (column c, right-hand side
+// jj) in slot v ^ spreadMask(l), which it arranges by loading column c ^ (mask / Q) into slot c and right-hand side
+// jj ^ (mask % Q) into slot jj.
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a,
                                                         const S* y, S* pz, int32_t q) {
-  constexpr int E = kTileEdge, CG = 4, NR = kInitRows / E;
+  constexpr int E = kTileEdge, NR = kInitRows / E;
+  constexpr int CG = (Q > 1 && sizeof(S) == 8) ? 2 : 4;  // columns per group (registers: CG Q sums per lane)
+  constexpr int V = CG * Q;
+  static_assert((Q & (Q - 1)) == 0, "right-hand sides per pass: a power of two");
   const BlockTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int mask = spreadMask<V>(lane & (V - 1)), mc = mask / Q, mj = mask % Q;
   const int32_t s = t.s, nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s];
   const T* Lb = a.Lx + a.Lxp[s] + nc;
   const int32_t* rows = a.Si + a.Sp[s] + nc;  // y[below] read through the row structure
+  S yv[NR][Q];  // slot js: right-hand side js ^ mj
+#pragma unroll
+  for (int u = 0; u < NR; u++) {
+    const int32_t i = t.r0 + E * u + lane;
+    const int32_t yi = i < t.r1 ? rows[i] : 0;
+#pragma unroll
+    for (int js = 0; js < Q; js++) {
+      const int jj = js ^ mj;
+      yv[u][js] = (i < t.r1 && jj < q) ? y[int64_t(jj) * a.n + yi] : S(0);
+    }
+  }
 #pragma unroll 1
   for (int g = 0; g < 16 / CG; g++) {
     const int cj0 = 16 * w + CG * g;  // first column (in the block) of this group
-    S acc[CG][Q];
+    S acc[V];                         // slot cs Q + js: column cs ^ mc, right-hand side js ^ mj
 #pragma unroll
-    for (int c = 0; c < CG; c++)
-#pragma unroll
-      for (int jj = 0; jj < Q; jj++) acc[c][jj] = S(0);
+    for (int v = 0; v < V; v++) acc[v] = S(0);
 #pragma unroll
     for (int u = 0; u < NR; u++) {
       const int32_t i = t.r0 + E * u + lane;
-      const bool row = i < t.r1;
-      const int32_t yi = row ? rows[i] : 0;
-      S yv[Q];
 #pragma unroll
-      for (int jj = 0; jj < Q; jj++) yv[jj] = (row && jj < q) ? y[int64_t(jj) * a.n + yi] : S(0);
+      for (int cs = 0; cs < CG; cs++) {
+        const int32_t col = t.blk * E + cj0 + (cs ^ mc);
+        const S lv = (i < t.r1 && col < nc) ? S(Lb[int64_t(col) * m + i]) : S(0);
 #pragma unroll
-      for (int c = 0; c < CG; c++) {
-        const int32_t col = t.blk * E + cj0 + c;
-        const S lv = (row && col < nc) ? S(Lb[int64_t(col) * m + i]) : S(0);
-#pragma unroll
-        for (int jj = 0; jj < Q; jj++) acc[c][jj] += lv * yv[jj];
+        for (int js = 0; js < Q; js++) acc[cs * Q + js] += lv * yv[u][js];
       }
     }
-#pragma unroll
-    for (int c = 0; c < CG; c++)
-#pragma unroll
-      for (int jj = 0; jj < Q; jj++) {
-        S v = acc[c][jj];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (jj < q && lane == c) pz[(int64_t(t.p0) * q + jj) * E + cj0 + c] = v;
-      }
+    const S sum = waveSumSpread<V>(acc);  // lane l < V: value `mask` (column mask / Q, right-hand side mask % Q)
+    if (lane < V && mj < q) pz[(int64_t(t.p0) * q + mj) * E + cj0 + mc] = sum;
   }
 }
 
@@ -852,7 +1022,17 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
     }
     switch (L.kind) {
       case kSolveSmallFwd:
-        if (L.maxM <= kSolveWaveMax) {
+        if (L.seg > 0) {  // 64 / seg fronts per wave
+          const unsigned wg = unsigned((L.count + 4 * (64 / L.seg) - 1) / (4 * (64 / L.seg)));
+          auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(wg), dim3(256), 0, st, dp.fronts + L.first, int32_t(L.count), a, y, fv, q,
+                               dp.fvEntries);
+          };
+          if (L.seg == 8)
+            q == 1 ? go(k_solve_seg_fwd<T, S, 1, 8>) : go(k_solve_seg_fwd<T, S, kFlowRhs, 8>);
+          else
+            q == 1 ? go(k_solve_seg_fwd<T, S, 1, 16>) : go(k_solve_seg_fwd<T, S, kFlowRhs, 16>);
+        } else if (L.maxM <= kSolveWaveMax) {
           const unsigned wg = unsigned((L.count + 3) / 4);
           if (q == 1)
             hipLaunchKernelGGL((k_solve_wave_fwd<T, S, 1>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,
@@ -867,10 +1047,16 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
           hipLaunchKernelGGL((k_solve_small_fwd<T, S, kFlowRhs>), dim3(grid), dim3(kSmallSolveThreads), 0, st,
                              dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
         break;
-      case kSolveGather:
-        hipLaunchKernelGGL((k_solve_gather<T, S>), dim3(grid, unsigned((L.maxM + kGatherRows - 1) / kGatherRows)),
-                           dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q, dp.fvEntries);
+      case kSolveGather: {
+        const dim3 gg(grid, unsigned((L.maxM + kGatherRows - 1) / kGatherRows));
+        if (q == 1)
+          hipLaunchKernelGGL((k_solve_gather<T, S, 1>), gg, dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q,
+                             dp.fvEntries);
+        else
+          hipLaunchKernelGGL((k_solve_gather<T, S, kFlowRhs>), gg, dim3(256), 0, st, dp.fronts + L.first, a, y, fv, q,
+                             dp.fvEntries);
         break;
+      }
       case kSolveFwdFlow: {
         // the launch's task list for this factor precision, with its blocks per task
         constexpr int NQ = sizeof(T) == 4 ? kFlowQuadsF32 : kFlowQuadsF64;
@@ -889,7 +1075,16 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
         break;
       }
       case kSolveSmallBwd:
-        if (L.maxM <= kSolveWaveMax) {
+        if (L.seg > 0) {
+          const unsigned wg = unsigned((L.count + 4 * (64 / L.seg) - 1) / (4 * (64 / L.seg)));
+          auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(wg), dim3(256), 0, st, dp.fronts + L.first, int32_t(L.count), a, y, q);
+          };
+          if (L.seg == 8)
+            q == 1 ? go(k_solve_seg_bwd<T, S, 1, 8>) : go(k_solve_seg_bwd<T, S, kFlowRhs, 8>);
+          else
+            q == 1 ? go(k_solve_seg_bwd<T, S, 1, 16>) : go(k_solve_seg_bwd<T, S, kFlowRhs, 16>);
+        } else if (L.maxM <= kSolveWaveMax) {
           const unsigned wg = unsigned((L.count + 3) / 4);
           if (q == 1)
             hipLaunchKernelGGL((k_solve_wave_bwd<T, S, 1>), dim3(wg), dim3(256), 0, st, dp.fronts + L.first,

@@ -199,12 +199,13 @@ int amp_num_release_workspace(amp_num *num);
  * m <= smallNarrow, else 256); product launches use 128 x 128 tiles when they have at least wideLaunch of them (else
  * 64 x 64); tiled fronts are factored in super-panels of superPanel columns; a forward solve launch of the tiled
  * fronts of a level gives each workgroup several 64-row blocks (passed on inside the workgroup) when it has at least
- * flowWideMin such tasks, else one block each. amp_sym_autotune times the factor and one
+ * flowWideMin such tasks, else one block each; whole fronts of at most 8 (16) rows are packed 8 (4) to a wave when a
+ * level has at least packMin of them. amp_sym_autotune times the factor and one
  * solve of d_values (device, pattern order) on the device for a set of candidate thresholds, starting from the
  * defaults and changing one threshold at a time, and keeps the fastest; amp_sym_set_params applies given thresholds.
  * Both rebuild the plans, so they refuse (AMP_ERR_STATE) while numeric handles of the analysis exist. */
 typedef struct {
-  int32_t smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin;
+  int32_t smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin;
   int32_t candidates;        /* out: plans timed */
   double defaultMs, bestMs;  /* out: factor + solve device time with the default and with the chosen thresholds */
 } amp_tune_info;

@@ -171,7 +171,7 @@ struct Handle {
 
 extern "C" {
 
-// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin
+// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin
 void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax,
                        const int32_t* prm) {
   Handle* h = new Handle();
@@ -186,6 +186,7 @@ void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, 
     pp.wideLaunch = prm[2];
     pp.superPanel = prm[3];
     pp.flowWideMin = prm[4];
+    pp.packMin = prm[5];
   }
   h->F = buildFactorPlan(h->S, pp);
   h->SP = buildSolvePlan(h->S, h->F, pp);
@@ -583,6 +584,28 @@ int64_t ps_check_flow(void* p) {
 
 // Task lists of the flow launches that ps_solve simulates: those of FP32 factors (kFlowQuadsF32) or FP64 factors.
 void ps_set_flow_width(void* p, int32_t nq) { static_cast<Handle*>(p)->flowWidth = nq; }
+
+// Whole fronts in launches that pack several to a wave: out[0] / out[1] factor fronts packed by 8 / 16 lanes,
+// out[2] / out[3] forward-solve fronts packed by 8 / 16, out[4] / out[5] the same for the backward solve. Returns the
+// number of packed fronts taller than their segment (must be 0).
+int64_t ps_packed(void* p, int64_t* out) {
+  Handle* h = static_cast<Handle*>(p);
+  const Symbolic& S = h->S;
+  int64_t bad = 0;
+  std::fill(out, out + 6, 0);
+  auto rows = [&S](int32_t s) { return S.Sp[s + 1] - S.Sp[s]; };
+  for (const Launch& L : h->F.launches) {
+    if (L.kind != kSmall || L.threads != 0 || L.edge == 0) continue;
+    out[L.edge == 8 ? 0 : 1] += L.count;
+    for (int64_t i = L.first; i < L.first + L.count; i++) bad += rows(h->F.smallFronts[size_t(i)]) > L.edge;
+  }
+  for (const SolveLaunch& L : h->SP.launches) {
+    if ((L.kind != kSolveSmallFwd && L.kind != kSolveSmallBwd) || L.seg == 0) continue;
+    out[(L.kind == kSolveSmallFwd ? 2 : 4) + (L.seg == 8 ? 0 : 1)] += L.count;
+    for (int64_t i = L.first; i < L.first + L.count; i++) bad += rows(h->SP.fronts[size_t(i)]) > L.seg;
+  }
+  return bad;
+}
 
 // solve q right-hand sides (one after another, original ordering) with the last factor
 void ps_solve(void* p, const double* b, double* x, int32_t q) {

@@ -134,7 +134,11 @@ def test_ldlt_indefinite_matches_oracle(dims, opt, pivot_tol):
 def quasi_definite(ncell, c=0.5):
     """[[K, c I], [c I, -K]] with K the SPD 3D heat matrix: symmetric quasi-definite, so LDL^T exists in any order
     without perturbation, its inertia is (n, n) (Sylvester), and its conditioning is that of K (well conditioned)."""
-    pat, v, _ = W.heat3d(ncell)
+    return quasi_definite_of(*W.heat3d(ncell)[:2], c)
+
+
+def quasi_definite_of(pat, v, c=0.5):
+    """[[K, c I], [c I, -K]] for the SPD matrix K given by (pat, v)."""
     n = pat["n"]
     K = sp.csr_matrix((v, pat["colidx"], pat["rowptr"]), shape=(n, n))
     I = sp.identity(n, format="csr")
@@ -215,6 +219,56 @@ def test_solve_task_widths_match_oracle(kind, flow_wide):
         if kind == "hip-f32":
             x, info = num.refine(v, B[0], maxSteps=8, tol=1e-15)
             assert relmax(x, exact[0]) < 1e-12
+
+
+@pytest.mark.parametrize("pack", [0, 2**30])
+@pytest.mark.parametrize("kind", ["hip", "hip-f32"])
+def test_packed_small_fronts_match_oracle(kind, pack):
+    """Whole fronts of at most 8 / 16 rows factored and solved 8 / 4 to a wave (packMin = 0: packed on every level)
+    and one per wave (packMin = 2^30): Cholesky of a batch of 2D heat systems and LDL^T of a quasi-definite system
+    built on it, one, three and eight right-hand sides. FP64 factors: within 1e-12 of the oracle's solutions. FP32
+    factors: within 1e-5 of the oracle's FP64 solves with the same (downloaded) FP32 factor. FP64 LDL^T of the
+    quasi-definite system with pivotTol 0.05 (1,583 perturbed pivots): nNeg and nPerturbed equal the oracle's."""
+    prm = {"smallMax": 96, "smallNarrow": 32, "wideLaunch": 512, "superPanel": 256, "flowWideMin": 160,
+           "packMin": pack}
+    K2 = W.batch_heat2d(nsys=8, lo=30, hi=90, nrhs=1)[:2]
+    for mode, (pat, v) in ((0, K2), (1, quasi_definite_of(*K2)[:2])):
+        n = pat["n"]
+        print(f"packed kind {kind} packMin {pack} mode {mode} n {n}", flush=True)
+        sym = amp.analyze(pat)
+        sym.set_params(prm)
+        So = O.Sym(pat)
+        if mode == 0:
+            _, Lo = O.chol_factor(So, v)
+            ref_solve = lambda L, D, b: O.chol_solve(So, L, b)  # noqa: E731
+            Do = None
+        else:
+            _, Lo, Do, _, _ = O.ldlt_factor(So, v)
+            ref_solve = lambda L, D, b: O.ldlt_solve(So, L, D, b)  # noqa: E731
+        num = amp.NumericFactor(sym, mode, kind)
+        assert num.factor(v)
+        B = np.stack([2 * W.lcg_uniform(41 + k, n) - 1 for k in range(8)])
+        if kind == "hip":
+            Lr, Dr, tol = Lo, Do, 1e-12
+        else:
+            Lr, Dr = num.download()
+            tol = 1e-5
+        ref = [ref_solve(Lr, Dr if mode else None, B[k]) for k in range(8)]
+        assert relmax(num.solve(B[0]), ref[0]) < tol
+        for q in (3, 8):
+            X = num.solve(B[:q].ravel(), nrhs=q).reshape(q, n)
+            for k in range(q):
+                assert relmax(X[k], ref[k]) < tol
+    if kind == "hip":
+        pat, v = quasi_definite_of(*K2)[:2]
+        print(f"packed ldlt perturbed packMin {pack}", flush=True)
+        So = O.Sym(pat)
+        ok_o, _, _, nneg_o, npert_o = O.ldlt_factor(So, v, 0.05)
+        sym = amp.analyze(pat)
+        sym.set_params(prm)
+        num = amp.NumericFactor(sym, 1, kind)
+        assert ok_o and num.factor(v, pivotTol=0.05)
+        assert num.info["nNeg"] == nneg_o > 0 and num.info["nPerturbed"] == npert_o > 0
 
 
 def test_cholesky_reports_indefinite_and_solve_refuses():

@@ -19,10 +19,9 @@ f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
 
 @pytest.fixture(scope="module")
 def sim():
-    if not os.path.exists(LIB):
-        import subprocess
+    import subprocess
 
-        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "plansim")], check=True)
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "plansim")], check=True)  # rebuilt when a source changed
     L = C.CDLL(LIB)
     L.ps_create.restype = C.c_void_p
     L.ps_create.argtypes = [C.c_int32, i32p, i32p, C.c_void_p, C.c_int32]
@@ -38,6 +37,8 @@ def sim():
     L.ps_create_params.argtypes = [C.c_int32, i32p, i32p, C.c_void_p, C.c_int32, C.c_void_p]
     L.ps_check_streams.argtypes = [C.c_void_p, C.c_int32]
     L.ps_check_streams.restype = C.c_int64
+    L.ps_packed.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.ps_packed.restype = C.c_int64
     return L
 
 
@@ -126,8 +127,8 @@ def test_plan_failure_column(sim):
 
 
 @pytest.mark.parametrize("name", ["heat3d_14", "heat3d_20", "grid27_11", "batch2d_8"])
-@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128, 160), (96, 32, 2**30, 64, 160), (64, 16, 256, 128, 0),
-                                 (64, 32, 256, 128, 2**30)])
+@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128, 160, 512), (96, 32, 2**30, 64, 160, 0),
+                                 (64, 16, 256, 128, 0, 2**30), (64, 32, 256, 128, 2**30, 0)])
 def test_plan_two_stream_ordering(sim, name, prm):
     """Every pair of factor launches on the two streams that the plan's waits leave unordered touches disjoint
     memory, and no two tasks of one launch conflict; with the waits removed the checker must find conflicts (so it
@@ -155,7 +156,7 @@ def test_plan_wide_solve_tasks(sim, name):
     lists and the plan's solves, replayed task by task, match the oracle's."""
     pat, v = CASES[name]()
     n = pat["n"]
-    pa = np.array((96, 32, 512, 256, 0), np.int32)
+    pa = np.array((96, 32, 512, 256, 0, 512), np.int32)
     h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
     try:
         assert sim.ps_check_flow(h) == 0
@@ -173,6 +174,46 @@ def test_plan_wide_solve_tasks(sim, name):
                 assert np.abs(X[k] - xo).max() / np.abs(xo).max() < 1e-12
     finally:
         sim.ps_free(h)
+
+
+@pytest.mark.parametrize("name", ["batch2d_8", "heat3d_14"])
+def test_plan_packed_small_fronts(sim, name):
+    """Whole fronts of at most 8 / 16 rows packed several to a wave: with packMin = 0 every such front of the factor and
+    of both solve directions is in a launch packed by 8 (m <= 8) or 16 lanes (8 < m <= 16), none taller than its
+    segment; with packMin = 2^30 none is packed; with the default the packed fronts are a subset, by the same rule.
+    The factor and solves of the packed plan still match the oracle's."""
+    pat, v = CASES[name]()
+    n = pat["n"]
+    S = O.Sym(pat)
+    m = np.diff(S.Sp)
+    small = m <= 96
+    want = [int(((m <= 8) & small).sum()), int(((m > 8) & (m <= 16) & small).sum())]
+    assert want[0] > 0 and want[1] > 0
+    for pack, expect in ((0, want), (2**30, [0, 0]), (512, None)):
+        pa = np.array((96, 32, 512, 256, 160, pack), np.int32)
+        h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
+        try:
+            out = (C.c_int64 * 6)()
+            assert sim.ps_packed(h, out) == 0
+            got = list(out)
+            if expect is not None:
+                assert got == expect * 3, (pack, got)
+            else:
+                assert got[2:4] == got[4:6] and got[0] <= want[0] and got[2] <= want[0]
+                assert got[0] + got[1] <= sum(want) and got[2] + got[3] <= sum(want)
+            if pack == 0:
+                ok, Lo = O.chol_factor(S, v)
+                Lg, D = np.zeros(S.nnzLStored), np.zeros(n)
+                assert sim.ps_factor(h, v, 0, 0.0, Lg, D, None) == 2**31 - 1
+                mk = lower_mask(S)
+                assert np.abs(Lg[mk] - Lo[mk]).max() / np.abs(Lo[mk]).max() < 1e-12
+                b = 2 * W.lcg_uniform(5, n) - 1
+                x = np.zeros(n)
+                sim.ps_solve(h, b, x, 1)
+                xo = O.chol_solve(S, Lo, b)
+                assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
+        finally:
+            sim.ps_free(h)
 
 
 def test_tiled_front_map_past_2_to_31_entries(sim):
