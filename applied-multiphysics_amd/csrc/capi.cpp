@@ -39,8 +39,22 @@ constexpr double kDefaultGrowthLimit = 1e4;
 int buildPlans(amp_sym* h, const amp::PlanParams& p) {
   try {
     amp::FactorPlan F = amp::buildFactorPlan(h->S, p);
+    amp::PlanParams q = p;
+    q.fusedPanel = 0;
+    if (F.panelMaxTasks > amp::panelResidentTasks())  // a front's panel step must fit the device at once
+      F = amp::buildFactorPlan(h->S, q);
+    amp::FactorPlan F3 = F.panelCounters > 0 ? amp::buildFactorPlan(h->S, q) : amp::FactorPlan();
+    if (!F3.launches.empty()) {  // the task arrays the two forms share on the device must be the same
+      auto same = [](const auto& x, const auto& y) {
+        return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
+      };
+      if (!same(F.zeros, F3.zeros) || !same(F.adst, F3.adst) || !same(F.smallFronts, F3.smallFronts) ||
+          !same(F.ea, F3.ea) || !same(F.eaRanges, F3.eaRanges))
+        return fail(AMP_ERR_INTERNAL, "analyze: the two forms of the factor plan disagree outside the panel chain");
+    }
     amp::SolvePlan SP = amp::buildSolvePlan(h->S, F, p);
     h->F = std::move(F);
+    h->F3 = std::move(F3);
     h->SP = std::move(SP);
   } catch (const std::length_error& e) {
     return fail(AMP_ERR_ARG, std::string("analyze: ") + e.what());
@@ -82,6 +96,7 @@ int symToDevice(amp_sym* h) {
   HIP_TRY(upload(h->ea, F.ea));
   HIP_TRY(upload(h->eaRanges, F.eaRanges));
   HIP_TRY(upload(h->diag, F.diag));
+  HIP_TRY(upload(h->panels, F.panels));
   HIP_TRY(upload(h->gemm, F.gemm));
   HIP_TRY(upload(h->splits, F.splits));
   HIP_TRY(upload(h->srefs, F.srefs));
@@ -93,10 +108,27 @@ int symToDevice(amp_sym* h) {
   h->dfp.ea = h->ea.as<amp::EaTask>();
   h->dfp.eaRanges = h->eaRanges.as<amp::EaRange>();
   h->dfp.diag = h->diag.as<amp::DiagTask>();
+  h->dfp.panels = h->panels.as<amp::PanelTask>();
+  h->dfp.panelCounters = F.panelCounters;
   h->dfp.gemm = h->gemm.p;
   h->dfp.splits = h->splits.as<amp::SplitTask>();
   h->dfp.srefs = h->srefs.as<amp::SplitRef>();
   h->dfp.launches = F.launches;
+  h->dfp3 = h->dfp;
+  if (!h->F3.launches.empty()) {
+    // the three-launch form shares every task array but those of the panel chain and the bulk launches
+    HIP_TRY(upload(h->diag3, h->F3.diag));
+    HIP_TRY(upload(h->gemm3, h->F3.gemm));
+    HIP_TRY(upload(h->splits3, h->F3.splits));
+    HIP_TRY(upload(h->srefs3, h->F3.srefs));
+    h->dfp3.diag = h->diag3.as<amp::DiagTask>();
+    h->dfp3.panels = nullptr;
+    h->dfp3.panelCounters = 0;
+    h->dfp3.gemm = h->gemm3.p;
+    h->dfp3.splits = h->splits3.as<amp::SplitTask>();
+    h->dfp3.srefs = h->srefs3.as<amp::SplitRef>();
+    h->dfp3.launches = h->F3.launches;
+  }
   h->dsp.fronts = h->sFronts.as<int32_t>();
   h->dsp.blocks = h->sBlocks.as<amp::BlockTask>();
   h->dsp.launches = h->SP.launches;
@@ -136,7 +168,9 @@ amp::FactorArgs<T> factorArgs(amp_num* num) {
   a.growthLimit = std::is_same<T, float>::value ? num->growthLimit : 0.0;
   a.fallbacks = num->counts.as<int32_t>() + 2;
   a.split = num->split.as<__bf16>();
-  a.splitPitch = splitPitch(h->F);
+  a.splitPitch = splitPitch(factorPlan(num));
+  a.panelCtr = num->panelCtr.as<int32_t>();
+  a.panelErr = num->counts.as<int32_t>() + 3;
   return a;
 }
 
@@ -172,7 +206,7 @@ template <typename T>
 int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_info* info) {
   amp_sym* h = num->sym;
   const int32_t n = h->S.n;
-  if (!num->U.p && h->F.uEntries > 0) HIP_TRY(num->U.alloc(size_t(h->F.uEntries) * sizeof(T)));  // released before
+  if (!num->U.p && factorPlan(num).uEntries > 0) HIP_TRY(num->U.alloc(size_t(factorPlan(num).uEntries) * sizeof(T)));  // released before
   amp::FactorArgs<T> a = factorArgs<T>(num);
   if (num->mode == AMP_LDLT && n > 0) {
     // tau = pivotTol * max |a_jj| needs the diagonal on the host before the launch sequence
@@ -187,24 +221,28 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
   }
   HIP_TRY(hipEventRecord(num->ev0, num->stream));
   if (n > 0)
-    amp::launchFactorPlan<T>(h->dfp, a, d_values, num->stream, num->side, num->syncEv,
+    amp::launchFactorPlan<T>(deviceFactorPlan(num), a, d_values, num->stream, num->side, num->syncEv,
                              num->profiling ? &num->prof : nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(num->ev1, num->stream));
-  int32_t counts[3] = {0, 0, 0}, failCol = INT32_MAX;
+  int32_t counts[4] = {0, 0, 0, 0}, failCol = INT32_MAX;
   if (n > 0) {
     if (num->mode == AMP_LDLT) amp::launchLdltCounts<T>(a, n, num->counts.as<int32_t>(), num->stream);
     HIP_TRY(hipMemcpyAsync(&failCol, num->failCol.p, 4, hipMemcpyDeviceToHost, num->stream));
-    HIP_TRY(hipMemcpyAsync(counts, num->counts.p, 12, hipMemcpyDeviceToHost, num->stream));
+    HIP_TRY(hipMemcpyAsync(counts, num->counts.p, 16, hipMemcpyDeviceToHost, num->stream));
   }
   HIP_TRY(hipStreamSynchronize(num->stream));
+  if (counts[3]) {
+    amp::launchClearFlag(num->counts.as<int32_t>() + 3, num->stream);  // reports this factor only
+    return fail(AMP_ERR_INTERNAL, "factor: a panel-step wait for the launch's workgroups timed out");
+  }
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, num->ev0, num->ev1));
   for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
     it = it->first.rfind("solve_", 0) == 0 ? std::next(it) : num->kernelMs.erase(it);
   if (num->profiling) {
     static const char* names[] = {"zero",       "assemble",  "small_front", "extend_add",
-                                  "diag_block", "gemm_tile", "split_panel"};
+                                  "diag_block", "gemm_tile", "split_panel", "panel_step"};
     for (size_t i = 0; i < num->prof.used; i++) {
       float k = 0;
       HIP_TRY(hipEventElapsedTime(&k, num->prof.events[2 * i], num->prof.events[2 * i + 1]));
@@ -559,7 +597,7 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   num->mode = mode;
   num->prec = precision;
   const size_t es = precision == AMP_F64 ? 8 : 4;
-  const amp::FactorPlan& F = h->F;
+  const amp::FactorPlan& F = factorPlan(num.get());
   HIP_TRY(hipStreamCreateWithFlags(&num->stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&num->side, hipStreamNonBlocking));
   num->syncEv.resize(size_t(F.signals) + 2, nullptr);  // signals, the final join, the initial fork
@@ -573,7 +611,9 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   HIP_TRY(num->D.alloc(size_t(std::max(h->S.n, 1)) * es));
   HIP_TRY(num->pert.alloc(size_t(std::max(h->S.n, 1))));
   HIP_TRY(num->failCol.alloc(4));
-  HIP_TRY(num->counts.alloc(16));  // nNeg, nPerturbed, FP64 diagonal-block redos
+  HIP_TRY(num->counts.alloc(16));  // nNeg, nPerturbed, FP64 diagonal-block redos, panel-step wait failure
+  HIP_TRY(hipMemset(num->counts.p, 0, 16));
+  if (F.panelCounters > 0) HIP_TRY(num->panelCtr.alloc(size_t(F.panelCounters) * 4));
   if (precision == AMP_F32_BF16X3 && F.splitEntries > 0)
     HIP_TRY(num->split.alloc(size_t(4 * splitPitch(F)) * sizeof(uint16_t)));
   // static pivoting (LDL^T) can grow the multipliers; a Cholesky factor of an SPD matrix cannot (|L|^2 <= |A|)
@@ -672,7 +712,7 @@ int amp_num_set_profiling(amp_num* num, int32_t enable) {
   if (!num) return fail(AMP_ERR_ARG, "amp_num_set_profiling: bad arguments");
   num->profiling = enable != 0;
   if (num->profiling && num->prof.events.empty()) {
-    const size_t nl = num->sym->dfp.launches.size();
+    const size_t nl = deviceFactorPlan(num).launches.size();
     num->prof.events.resize(2 * nl + 2, nullptr);
     num->prof.kinds.resize(nl + 1, 0);
     num->prof.levels.resize(nl + 1, 0);
@@ -738,7 +778,7 @@ int enqueueSequence(amp_graph* g, const double* d_values, const double* d_b, dou
   amp_sym* h = num->sym;
   const int32_t n = h->S.n;
   hipStream_t st = num->stream;
-  amp::launchFactorPlan<T>(h->dfp, factorArgs<T>(num), d_values, st, num->side, num->syncEv, nullptr);
+  amp::launchFactorPlan<T>(deviceFactorPlan(num), factorArgs<T>(num), d_values, st, num->side, num->syncEv, nullptr);
   const amp::SolveArgs<T> sa = solveArgs<T>(num);
   auto solve = [&](const double* b, double* x) {
     return amp::launchSolvePlan<T, T>(h->dsp, sa, b, x, q, num->y.as<T>(), num->fv.as<T>(), num->pz.as<T>(), num->fc,
@@ -808,6 +848,7 @@ int amp_graph_launch(amp_graph* g) {
   if (!g) return fail(AMP_ERR_ARG, "amp_graph_launch: bad arguments");
   HIP_TRY(amp::afterCaller(g->num->stream, g->num->inEv));
   amp::launchClearFlag(g->num->fc.err, g->num->stream);  // the wait-failure flag reports this replay only
+  amp::launchClearFlag(g->num->counts.as<int32_t>() + 3, g->num->stream);  // and the panel-step one
   HIP_TRY(hipEventRecord(g->e0, g->num->stream));
   HIP_TRY(hipGraphLaunch(g->exec, g->num->stream));
   HIP_TRY(hipEventRecord(g->e1, g->num->stream));
@@ -817,11 +858,13 @@ int amp_graph_launch(amp_graph* g) {
 int amp_graph_sync(amp_graph* g, amp_factor_info* info) {
   if (!g) return fail(AMP_ERR_ARG, "amp_graph_sync: bad arguments");
   amp_num* num = g->num;
-  int32_t failCol = INT32_MAX, waitErr = 0;
+  int32_t failCol = INT32_MAX, waitErr = 0, panelErr = 0;
   HIP_TRY(hipMemcpyAsync(&failCol, num->failCol.p, 4, hipMemcpyDeviceToHost, num->stream));
   HIP_TRY(hipMemcpyAsync(&waitErr, num->fc.err, 4, hipMemcpyDeviceToHost, num->stream));
+  HIP_TRY(hipMemcpyAsync(&panelErr, num->counts.as<int32_t>() + 3, 4, hipMemcpyDeviceToHost, num->stream));
   HIP_TRY(hipStreamSynchronize(num->stream));
   if (waitErr) return fail(AMP_ERR_INTERNAL, "graph replay: a block dependency wait timed out");
+  if (panelErr) return fail(AMP_ERR_INTERNAL, "graph replay: a panel-step wait for the launch's workgroups timed out");
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, g->e0, g->e1));
   num->ok = failCol == INT32_MAX;
@@ -851,7 +894,7 @@ int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
 bool validParams(const amp::PlanParams& p) {
   return p.smallMax >= 1 && p.smallMax <= amp::kSmallMax && p.smallNarrow >= 1 && p.smallNarrow <= p.smallMax &&
          p.wideLaunch >= 1 && p.superPanel >= amp::kTileEdge && p.superPanel % amp::kTileEdge == 0 &&
-         p.flowWideMin >= 0 && p.packMin >= 0;
+         p.flowWideMin >= 0 && p.packMin >= 0 && (p.fusedPanel == 0 || p.fusedPanel == 1);
 }
 
 // factor + one solve (zero right-hand side), device milliseconds of the second of two runs
@@ -888,6 +931,7 @@ int amp_sym_set_params(amp_sym* h, const amp_tune_info* t) {
   p.superPanel = t->superPanel;
   p.flowWideMin = t->flowWideMin;
   p.packMin = t->packMin;
+  p.fusedPanel = t->fusedPanel;
   if (!validParams(p)) return fail(AMP_ERR_ARG, "amp_sym_set_params: thresholds out of range");
   return rebuildPlans(h, p);
 }
@@ -956,6 +1000,11 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     p.packMin = v;
     if ((rc = attempt(p))) return rc;
   }
+  {  // the panel chain as three launches per step instead of one
+    amp::PlanParams p = best;
+    p.fusedPanel = 0;
+    if ((rc = attempt(p))) return rc;
+  }
   rc = rebuildPlans(h, best);
   if (rc) return rc;
   if (info) {
@@ -965,6 +1014,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     info->superPanel = best.superPanel;
     info->flowWideMin = best.flowWideMin;
     info->packMin = best.packMin;
+    info->fusedPanel = best.fusedPanel;
     info->candidates = tried;
     info->defaultMs = defaultMs;
     info->bestMs = bestMs;

@@ -17,6 +17,8 @@ struct DeviceFactorPlan {
   const EaTask* ea = nullptr;
   const EaRange* eaRanges = nullptr;
   const DiagTask* diag = nullptr;
+  const PanelTask* panels = nullptr;
+  int32_t panelCounters = 0;
   const void* gemm = nullptr;
   const SplitTask* splits = nullptr;
   const SplitRef* srefs = nullptr;
@@ -99,6 +101,14 @@ void launchSplitPanels(const SplitTask* splits, int64_t count, int32_t maxRows, 
 // padding that lets one workgroup fewer fit on a CU than without it (0 when the kernel fits at most two), so that the
 // panel chain's kernels find room beside the bulk updates on every CU. Cached per kernel and device.
 size_t bulkLds(const void* kernel, int threads, size_t staticLds);
+
+// Fused panel-step launch (kPanel) and the kernel that clears the launches' counters (n ints) before a factor.
+template <typename T>
+void launchPanelSteps(const PanelTask* tasks, unsigned count, const FactorArgs<T>& a, hipStream_t st);
+void launchClearInts(int32_t* p, int32_t n, hipStream_t st);
+// Workgroups of the fused panel-step kernels resident at once on the current device (the fewest over their instances;
+// INT32_MAX when the device cannot be queried).
+int32_t panelResidentTasks();
 
 // Clear one int32 device flag with a kernel (a kernel node, not a memset node, when the caller captures the stream).
 void launchClearFlag(int32_t* flag, hipStream_t st);

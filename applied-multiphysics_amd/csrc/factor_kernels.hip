@@ -606,6 +606,7 @@ template <typename T>
 void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const double* values, hipStream_t main,
                       hipStream_t side, const std::vector<hipEvent_t>& sync, ProfileSink* prof) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, main, a.failCol, INT32_MAX);
+  if (a.panelCtr) launchClearInts(a.panelCtr, dp.panelCounters, main);
   if (a.fallbacks) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, main, a.fallbacks, 0);
   if (prof) prof->used = 0;
   hipEvent_t last[2] = {nullptr, nullptr};  // most recent signal of each stream
@@ -669,6 +670,9 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
         break;
       case kDiag:
         launchDiagBlocks<T>(dp.diag + L.first, grid, a, st);
+        break;
+      case kPanel:
+        launchPanelSteps<T>(dp.panels + L.first, grid, a, st);
         break;
       case kSplit:
         if constexpr (std::is_same<T, float>::value)

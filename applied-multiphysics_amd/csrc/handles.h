@@ -81,6 +81,9 @@ int solveWithScratch(amp_num* num, const double* d_b, double* d_x, int32_t nrhs,
 struct amp_sym {
   amp::Symbolic S;
   amp::FactorPlan F;
+  // F with each fused panel step as three launches, built when F fuses them: FP64 factors run this form (the fused
+  // kernel's FP64 instance cannot share a CU with the FP64 bulk updates); empty otherwise
+  amp::FactorPlan F3;
   amp::SolvePlan SP;
   std::vector<int32_t> diagSrc;  // CSR slot of each diagonal entry (new column order), -1 when absent
   std::vector<int32_t> rowptr, colidx;  // the analysed CSR pattern (for the refinement residual), empty if unknown
@@ -88,8 +91,10 @@ struct amp_sym {
   bool onDevice = false;
   // device copies of the structure and of the task arrays
   amp::DevBuf superStart, Sp, Si, Lxp, Uoff, Woff, childHead, childNext, rel, lowerP, lowerSrc, amap, adst, perm, dDiagSrc;
-  amp::DevBuf zeros, smallFronts, ea, eaRanges, diag, gemm, splits, srefs, sFronts, sBlocks;
+  amp::DevBuf zeros, smallFronts, ea, eaRanges, diag, panels, gemm, splits, srefs, sFronts, sBlocks;
   amp::DeviceFactorPlan dfp;
+  amp::DevBuf diag3, gemm3, splits3, srefs3;  // task arrays of F3 (the others are F's)
+  amp::DeviceFactorPlan dfp3;
   amp::DeviceSolvePlan dsp;
   amp::PlanParams params;  // the thresholds the plans were built with
   int32_t liveNums = 0;    // numeric handles on this analysis (their storage is laid out by the plans)
@@ -103,6 +108,7 @@ struct amp_num {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evR0 = nullptr, evR1 = nullptr;
   hipEvent_t inEv = nullptr;  // afterCaller
   amp::DevBuf Lx, U, W, D, pert, failCol, counts, tau, values, b, x, y, fv, pz;
+  amp::DevBuf panelCtr;  // counters of the kPanel launches
   amp::DevBuf split;  // bf16 bulk products (AMP_F32_BF16X3): the split panel of the current bulk launch
   // refinement work space (FP64): r, d, w, Krylov bases V (restart + 1 vectors) and Z (restart), reductions
   amp::DevBuf r, dvec, w, V, Z, partial, small;
@@ -131,4 +137,12 @@ struct amp_num {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+// The factor plan a numeric handle runs (host and device forms): F3 for FP64 factors when the analysis has one.
+inline const amp::FactorPlan& factorPlan(const amp_num* num) {
+  return num->prec == AMP_F64 && !num->sym->F3.launches.empty() ? num->sym->F3 : num->sym->F;
+}
+inline const amp::DeviceFactorPlan& deviceFactorPlan(const amp_num* num) {
+  return num->prec == AMP_F64 && !num->sym->F3.launches.empty() ? num->sym->dfp3 : num->sym->dfp;
+}
 

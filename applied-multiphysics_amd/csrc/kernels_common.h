@@ -108,6 +108,10 @@ struct FactorArgs {
   // tails, B heads, B tails; for Cholesky the B arrays are the A arrays)
   __bf16* split;
   int64_t splitPitch;
+  // fused panel steps: one counter per kPanel launch (workgroups whose inputs are read), and the flag a wait on one
+  // sets when it gives up
+  int32_t* panelCtr;
+  int32_t* panelErr;
 };
 
 }  // namespace amp

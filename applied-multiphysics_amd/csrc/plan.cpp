@@ -292,6 +292,33 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     };
     for (int32_t c0 = 0; c0 < maxNc; c0 += SPW) {
       for (int32_t k0 = c0; k0 < std::min(maxNc, c0 + SPW); k0 += kTileEdge) {
+        if (prm.fusedPanel && SPW <= kSuperPanel) {
+          // one launch: every row tile of the panel (or the diagonal block alone) is a task
+          const int64_t pFirst = int64_t(P.panels.size());
+          for (int32_t s : big) {
+            const int32_t nc = frontCols(S, s), m = frontRows(S, s);
+            if (nc <= k0) continue;
+            const int32_t nb = std::min(kTileEdge, nc - k0);
+            const int32_t ucols = std::max(0, std::min(nc, c0 + SPW) - (k0 + nb));
+            const int32_t ntask = std::max(1, (m - k0 - nb + kTileEdge - 1) / kTileEdge);
+            PanelTask t{S.Lxp[s] + k0 + int64_t(k0) * m,
+                        P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge, m, nb, S.superStart[s] + k0,
+                        0, 0, ucols, P.panelCounters++, ntask, 1, 0};
+            P.panelMaxTasks = std::max(P.panelMaxTasks, ntask);
+            if (m - k0 == nb) {
+              P.panels.push_back(t);
+              continue;
+            }
+            for (int32_t ro = nb; ro < m - k0; ro += kTileEdge) {
+              t.ro = ro;
+              t.rows = std::min(kTileEdge, m - k0 - ro);
+              t.owner = ro == nb;
+              P.panels.push_back(t);
+            }
+          }
+          push(kPanel, L, pFirst, int64_t(P.panels.size()) - pFirst);
+          continue;
+        }
         const int64_t dFirst = int64_t(P.diag.size());
         for (int32_t s : big) {
           const int32_t nc = frontCols(S, s), m = frontRows(S, s);

@@ -78,6 +78,22 @@ struct ZeroTask {
   int64_t off, len;  // Lx entries
 };
 
+// One workgroup of a fused panel step (kPanel launch): the 64-column step whose diagonal block is at Lx[f_off]
+// (leading dimension ld, nb columns, first global column col0) of a tiled front, for one row tile of its panel. Every
+// task factors the diagonal block itself; the owner also stores its factor, the inverse of its lower factor (W[w_off])
+// and the pivot records. Own rows [ro, ro + rows) below the block (relative to its first row; rows == 0: none) become
+// L = A L11^-T (LDL^T: times D^-1); then the same rows of the ucols columns that follow the block inside its
+// super-panel are updated, C -= L D Lc^T, with Lc (the panel rows [nb, nb + ucols)) formed by the task from A. Tasks
+// that overwrite inputs of the front's step (the owner, and the row tiles that hold Lc) wait until all ntask tasks of
+// the front's step have read them: counter ctr (one per front and launch) counts the tasks that have. A front's tasks
+// are consecutive in the launch, so with workgroups dispatched in order the writers of every front whose tasks are
+// all resident complete, and no wait can hold the launch (ntask must not exceed the resident workgroups).
+struct PanelTask {
+  int64_t f_off, w_off;
+  int32_t ld, nb, col0, ro, rows, ucols, ctr, ntask, owner, pad;
+};
+constexpr int kPanelUpdateTiles = kSuperPanel / kTileEdge - 1;  // column tiles a panel task updates, at most
+
 enum LaunchKind : int32_t {
   kZero = 0,
   kAssemble = 1,
@@ -86,6 +102,7 @@ enum LaunchKind : int32_t {
   kDiag = 4,
   kGemm = 5,
   kSplit = 6,
+  kPanel = 7,
 };
 
 struct Launch {
@@ -124,6 +141,9 @@ struct FactorPlan {
   std::vector<EaTask> ea;
   std::vector<EaRange> eaRanges;
   std::vector<DiagTask> diag;
+  std::vector<PanelTask> panels;  // kPanel launches (fused panel steps)
+  int32_t panelCounters = 0;      // counters of the kPanel launches (one per front and launch, PanelTask::ctr)
+  int32_t panelMaxTasks = 0;      // most tasks of one front in a kPanel launch
   std::vector<GemmTask> gemm;
   std::vector<SplitTask> splits;  // kSplit launches: the panels of bulk launches with kGemmEdge tiles
   std::vector<SplitRef> srefs;    // aligned with gemm
@@ -149,6 +169,9 @@ struct PlanParams {
   int32_t flowWideMin = kFlowWideMinTasks;  // wide tasks a forward solve launch needs to use them (else one block per
                                             // task)
   int32_t packMin = kPackMinFronts;         // fronts of a size class a level needs to pack them several to a wave
+  int32_t fusedPanel = 1;  // a panel step of tiled fronts is one kPanel launch (else diagonal blocks, panel products
+                           // and the super-panel's update as three launches); super-panels of at most kSuperPanel
+                           // columns only
 };
 
 FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams());

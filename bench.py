@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "applied-multiphysics_amd"))
 METRIC = "factor+solve s & TFLOP/s vs peak, 3D P1 heat 61³ nodes, FP64 vs mixed @1e-10 resid"
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (TFLOP/s): FP64, FP32
 PEAK_HBM = 8000.0  # MI355X HBM3E (GB/s)
-KERNEL_CLASSES = ("zero", "assemble", "small_front", "extend_add", "diag_block", "gemm_tile", "solve_small_fwd",
+KERNEL_CLASSES = ("zero", "assemble", "small_front", "extend_add", "diag_block", "panel_step", "gemm_tile", "solve_small_fwd",
                   "solve_gather", "solve_fwd_flow", "solve_small_bwd", "solve_bwd_gather", "solve_bwd_init", "solve_bwd_flow", "solve_total")
 
 

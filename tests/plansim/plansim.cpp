@@ -135,6 +135,85 @@ void runDiag(Ctx& c, const DiagTask& t) {
   }
 }
 
+// A kPanel launch: every task reads the diagonal block and its panel rows as they were when the launch started (the
+// device's writers of those inputs wait until every workgroup has read them), so the writes of all tasks are
+// collected first and applied after. Per task: the block factored and inverted as runDiag does, the own rows
+// L = A W^T (LDL^T: / d) as a mode-2 product, the column tiles' rows the same way, and the update of the own rows in
+// the column tiles' columns, C -= L d Lc^T, as a mode-0 product.
+void runPanelLaunch(Ctx& c, const Launch& L) {
+  const FactorPlan& P = *c.P;
+  std::vector<std::pair<int64_t, double>> lxw, ww;
+  // counter contract: a front's tasks are consecutive, share a counter no other front of any launch uses, and number
+  // ntask; exactly one of them is the owner (else a plan error, reported as failCol -2)
+  static thread_local std::vector<int32_t> seen;
+  if (L.first == 0 || seen.size() < size_t(P.panelCounters)) seen.assign(size_t(P.panelCounters), 0);
+  for (int64_t q = L.first; q < L.first + L.count;) {
+    const PanelTask& t0 = P.panels[size_t(q)];
+    int64_t e = q;
+    int32_t owners = 0;
+    while (e < L.first + L.count && P.panels[size_t(e)].ctr == t0.ctr) owners += P.panels[size_t(e++)].owner;
+    if (t0.ctr < 0 || t0.ctr >= P.panelCounters || seen[size_t(t0.ctr)]++ || e - q != t0.ntask || owners != 1 ||
+        t0.ntask > P.panelMaxTasks)
+      c.failCol = -2;
+    q = e;
+  }
+  for (int64_t q = L.first; q < L.first + L.count; q++) {
+    const PanelTask& t = P.panels[size_t(q)];
+    const int32_t nb = t.nb, E = kTileEdge;
+    const int64_t ld = t.ld;
+    const double* F0 = c.Lx.data() + t.f_off;
+    std::vector<double> F(size_t(nb) * nb, 0.0), W(size_t(E) * E, 0.0);
+    for (int32_t j = 0; j < nb; j++)
+      for (int32_t i = j; i < nb; i++) F[i + size_t(j) * nb] = F0[i + j * ld];
+    partialFactor(c, F.data(), nb, nb, nb, t.col0);
+    for (int32_t j = 0; j < nb; j++)
+      for (int32_t i = j; i < nb; i++) {
+        double s = i == j ? 1.0 : 0.0;
+        for (int32_t k = j; k < i; k++) s -= F[i + size_t(k) * nb] * W[k + size_t(j) * E];
+        W[i + size_t(j) * E] = c.ldlt ? s : s / F[i + size_t(i) * nb];
+      }
+    if (t.owner) {
+      for (int32_t j = 0; j < nb; j++)
+        for (int32_t i = j; i < nb; i++) lxw.emplace_back(t.f_off + i + j * ld, F[i + size_t(j) * nb]);
+      for (int32_t i = 0; i < E * E; i++) ww.emplace_back(t.w_off + i, W[size_t(i)]);
+    }
+    if (t.rows == 0) continue;
+    // rows r0 .. r0 + rows - 1 of the panel (relative to the block's first row): L = A W^T (/ d)
+    auto solveRows = [&](int32_t r0, int32_t rows) {
+      std::vector<double> Lr(size_t(rows) * nb);
+      for (int32_t j = 0; j < nb; j++)
+        for (int32_t i = 0; i < rows; i++) {
+          double s = 0;
+          for (int32_t k = 0; k < nb; k++) s += F0[r0 + i + k * ld] * W[j + size_t(k) * E];
+          if (c.ldlt) s /= c.D[t.col0 + j];
+          Lr[i + size_t(j) * rows] = s;
+        }
+      return Lr;
+    };
+    const std::vector<double> Lo = solveRows(t.ro, t.rows);
+    for (int32_t j = 0; j < nb; j++)
+      for (int32_t i = 0; i < t.rows; i++) lxw.emplace_back(t.f_off + t.ro + i + j * ld, Lo[i + size_t(j) * t.rows]);
+    const int32_t own = (t.ro - nb) / E;
+    for (int32_t ct = 0; ct * E < t.ucols && ct <= own; ct++) {
+      const int32_t wc = std::min(E, t.ucols - E * ct);
+      const std::vector<double> Lc = solveRows(nb + E * ct, wc);
+      for (int32_t j = 0; j < wc; j++)
+        for (int32_t i = 0; i < t.rows; i++) {
+          double s = 0;
+          for (int32_t k = 0; k < nb; k++) {
+            double b = Lc[j + size_t(k) * wc];
+            if (c.ldlt) b *= c.D[t.col0 + k];
+            s += Lo[i + size_t(k) * t.rows] * b;
+          }
+          const int64_t d = t.f_off + t.ro + i + (nb + E * ct + j) * ld;
+          lxw.emplace_back(d, c.Lx[size_t(d)] - s);
+        }
+    }
+  }
+  for (const auto& e : lxw) c.Lx[size_t(e.first)] = e.second;
+  for (const auto& e : ww) c.W[size_t(e.first)] = e.second;
+}
+
 void runGemm(Ctx& c, const GemmTask& t) {
   const double* A = buf(c, t.a_buf) + t.a_off;
   const double* B = buf(c, t.b_buf) + t.b_off;
@@ -171,7 +250,7 @@ struct Handle {
 
 extern "C" {
 
-// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin
+// prm (may be NULL for the defaults): smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin, fusedPanel
 void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, const int32_t* perm, int32_t relax,
                        const int32_t* prm) {
   Handle* h = new Handle();
@@ -187,6 +266,7 @@ void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, 
     pp.superPanel = prm[3];
     pp.flowWideMin = prm[4];
     pp.packMin = prm[5];
+    pp.fusedPanel = prm[6];
   }
   h->F = buildFactorPlan(h->S, pp);
   h->SP = buildSolvePlan(h->S, h->F, pp);
@@ -279,6 +359,10 @@ int32_t ps_factor(void* p, const double* values, int32_t ldlt, double pivotTol, 
     c.tau = pivotTol * (amax > 0 ? amax : 1);
   }
   for (const Launch& L : P.launches) {
+    if (L.kind == kPanel) {
+      runPanelLaunch(c, L);
+      continue;
+    }
     for (int64_t t = L.first; t < L.first + L.count; t++) {
       switch (L.kind) {
         case kZero:
@@ -300,6 +384,8 @@ int32_t ps_factor(void* p, const double* values, int32_t ldlt, double pivotTol, 
           runGemm(c, P.gemm[t]);
           break;
         case kSplit:  // bf16 operands of the bulk products: no effect on an FP64 replay
+          break;
+        case kPanel:  // run per launch above
           break;
       }
     }
@@ -370,6 +456,7 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
   sh[4].init(size_t(std::max<int64_t>(P.splitEntries, 1)));  // the split panel (one array stands for its four)
   int64_t bad = 0;
   int32_t curL = 0, curT = 0;
+  bool readsOrdered = false;  // kPanel writes: the launch's own reads are ordered before them (launch counter)
   auto conflict = [&](int32_t ol, int32_t ot) {
     if (ol < 0) return false;
     if (ol == curL) return ot != curT;
@@ -385,7 +472,7 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
     if (conflict(x.wl[i], x.wt[i])) bad++;
     if (write) {
       for (int o = 0; o < 2; o++)
-        if (conflict(x.rl[o][i], x.rt[o][i])) bad++;
+        if (!(readsOrdered && x.rl[o][i] == curL) && conflict(x.rl[o][i], x.rt[o][i])) bad++;
       x.wl[i] = curL;
       x.wt[i] = curT;
     } else {
@@ -396,6 +483,41 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
   for (int32_t l = 0; l < N; l++) {
     const Launch& L = P.launches[l];
     curL = l;
+    if (L.kind == kPanel) {
+      // pass 0: every task's reads (the block, its A rows and the column tiles' A rows, its C entries); pass 1: the
+      // writes (owner: the block, W, D; every task: its L rows and C entries)
+      for (int pass = 0; pass < 2; pass++) {
+        readsOrdered = pass == 1;
+        for (int64_t q = L.first; q < L.first + L.count; q++) {
+          curT = int32_t(q - L.first);
+          const PanelTask& t = P.panels[size_t(q)];
+          const int32_t E = kTileEdge, nb = t.nb;
+          const int64_t ld = t.ld;
+          const bool wr = pass == 1;
+          if (!wr || t.owner)
+            for (int32_t j = 0; j < nb; j++)
+              for (int32_t i = j; i < nb; i++) acc(0, t.f_off + i + j * ld, wr);
+          if (wr && t.owner) {
+            for (int64_t i = 0; i < int64_t(E) * E; i++) acc(2, t.w_off + i, true);
+            for (int32_t k = 0; k < nb; k++) acc(3, t.col0 + k, true);
+          }
+          if (t.rows == 0) continue;
+          for (int32_t j = 0; j < nb; j++)
+            for (int32_t i = 0; i < t.rows; i++) acc(0, t.f_off + t.ro + i + j * ld, wr);
+          const int32_t own = (t.ro - nb) / E;
+          for (int32_t ct = 0; ct * E < t.ucols && ct <= own; ct++) {
+            const int32_t wc = std::min(E, t.ucols - E * ct);
+            if (!wr)
+              for (int32_t j = 0; j < nb; j++)
+                for (int32_t i = 0; i < wc; i++) acc(0, t.f_off + nb + E * ct + i + j * ld, false);
+            for (int32_t j = 0; j < wc; j++)
+              for (int32_t i = 0; i < t.rows; i++) acc(0, t.f_off + t.ro + i + (nb + E * ct + j) * ld, wr);
+          }
+        }
+      }
+      readsOrdered = false;
+      continue;
+    }
     for (int64_t t = L.first; t < L.first + L.count; t++) {
       curT = int32_t(t - L.first);
       switch (L.kind) {
@@ -450,6 +572,8 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
           for (int32_t k = 0; k < d.nb; k++) acc(3, d.col0 + k, true);
           break;
         }
+        case kPanel:  // checked per launch above
+          break;
         case kSplit: {
           const SplitTask& sp = P.splits[t];
           for (int32_t d = 0; d < sp.w; d++) {

@@ -413,3 +413,55 @@ def test_inputs_ordered_after_the_callers_stream(device):
     x2 = num.solve(db).numpy()
     L2, _ = num.download()
     assert np.array_equal(L1, L2) and np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("kind", ["hip", "hip-f32", "hip-bf16x3"])
+def test_fused_panel_steps_match_the_three_launch_form(kind):
+    """A panel step of the tiled fronts as one fused launch (every workgroup eliminates the diagonal block itself,
+    forms its L rows and updates its rows of the super-panel's later columns as (L W) A_c^T on the matrix cores)
+    against the same step as three launches (diagonal block, L = A W^T, C -= L D L_c^T): the same factor up to the
+    rounding of the reordered update product. FP32 factors: within 1e-5 (relative, max norm), equal pivot records
+    (inertia, perturbations). FP64 factors run the three-launch form of the plan whatever fusedPanel says (the fused
+    kernel's FP64 instance cannot share a CU with the FP64 bulk updates), so theirs are bitwise equal.
+    Cholesky of the 3D heat system (several super-panels per front) and LDL^T of a quasi-definite system. For FP32
+    also the growth-checked instance with a limit that redoes blocks in FP64, on the nearly singular shifted 27-point
+    grid, where rounding differences are amplified and the two factors are not close: there the fused factor must
+    redo blocks, have the oracle's inertia, and bring GMRES-IR to 1e-10."""
+    heat = W.heat3d(20)[:2]
+    qd = quasi_definite(14)[:2]
+    near = grid27(14, 14, 14, shift=26.49, symmetric=True)[:2]
+    cases = [(0, heat, None), (1, qd, None)]
+    if kind == "hip-f32":
+        cases.append((1, near, 100.0))
+    tol = 0.0 if kind == "hip" else 1e-5
+    for mode, (pat, v), limit in cases:
+        n = pat["n"]
+        b = 2 * W.lcg_uniform(5, n) - 1
+        out = {}
+        for fused in (1, 0):
+            print(f"fused panel steps: kind {kind} mode {mode} n {n} growth limit {limit} fused {fused}", flush=True)
+            sym = amp.analyze(pat)
+            sym.set_params({"smallMax": 96, "smallNarrow": 32, "wideLaunch": 512, "superPanel": 256,
+                            "flowWideMin": 160, "fusedPanel": fused})
+            num = amp.NumericFactor(sym, mode, kind)
+            if limit is not None:
+                num.set_growth_limit(limit)
+            assert num.factor(v)
+            L, D = num.download()
+            out[fused] = (L, D, num.info["nNeg"], num.info["nPerturbed"], num.kernel_ms("diag_fp64_redo")[1],
+                          num.solve(b), lower_mask(sym))
+            if limit is not None:
+                _, info = num.refine(v, b, "gmres", maxSteps=200, restart=30, tol=1e-10)
+                assert info["converged"] and info["relResidual"] <= 1e-10
+            del num
+        (L1, D1, nn1, np1, r1, x1, m), (L0, D0, nn0, np0, r0, x0, _) = out[1], out[0]
+        print(f"  L {relmax(L1[m], L0[m]):.2e}  x {relmax(x1, x0):.2e}  redone {r1} / {r0}", flush=True)
+        if limit is not None:
+            _, _, _, nneg, _ = O.ldlt_factor(O.Sym(pat), v)
+            assert r1 > 0 and nn1 == nneg
+            continue
+        assert relmax(L1[m], L0[m]) <= tol
+        if mode == 1:
+            assert relmax(D1, D0) <= tol and nn1 == nn0 and np1 == np0
+        if kind == "hip":
+            assert np.array_equal(x1, x0)

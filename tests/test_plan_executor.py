@@ -62,12 +62,20 @@ CASES = {
 }
 
 
+DEFAULT_PRM = (96, 32, 512, 256, 160, 512, 1)  # smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin,
+                                               # fusedPanel
+
+
 @pytest.mark.parametrize("name", list(CASES))
-def test_plan_cholesky_and_ldlt(sim, name):
+@pytest.mark.parametrize("fused", [1, 0])
+def test_plan_cholesky_and_ldlt(sim, name, fused):
+    """The plan replayed on the CPU, panel steps as one fused launch each (fused = 1) or as three launches, against
+    the oracle's Cholesky and LDL^T factors and solves."""
     pat, v = CASES[name]()
     n = pat["n"]
     S = O.Sym(pat)
-    h = sim.ps_create(n, pat["rowptr"], pat["colidx"], None, 1)
+    pa = np.array(DEFAULT_PRM[:6] + (fused,), np.int32)
+    h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
     try:
         assert sim.ps_check_flow(h) == 0  # every solve task waits only on tasks listed before it
         stats = (C.c_int64 * 6)()
@@ -127,8 +135,9 @@ def test_plan_failure_column(sim):
 
 
 @pytest.mark.parametrize("name", ["heat3d_14", "heat3d_20", "grid27_11", "batch2d_8"])
-@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128, 160, 512), (96, 32, 2**30, 64, 160, 0),
-                                 (64, 16, 256, 128, 0, 2**30), (64, 32, 256, 128, 2**30, 0)])
+@pytest.mark.parametrize("prm", [None, (96, 32, 512, 128, 160, 512, 1), (96, 32, 2**30, 64, 160, 0, 1),
+                                 (64, 16, 256, 128, 0, 2**30, 0), (64, 32, 256, 128, 2**30, 0, 1),
+                                 (96, 32, 512, 256, 160, 512, 0), (96, 32, 512, 512, 160, 512, 1)])
 def test_plan_two_stream_ordering(sim, name, prm):
     """Every pair of factor launches on the two streams that the plan's waits leave unordered touches disjoint
     memory, and no two tasks of one launch conflict; with the waits removed the checker must find conflicts (so it
@@ -156,7 +165,7 @@ def test_plan_wide_solve_tasks(sim, name):
     lists and the plan's solves, replayed task by task, match the oracle's."""
     pat, v = CASES[name]()
     n = pat["n"]
-    pa = np.array((96, 32, 512, 256, 0, 512), np.int32)
+    pa = np.array((96, 32, 512, 256, 0, 512, 1), np.int32)
     h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
     try:
         assert sim.ps_check_flow(h) == 0
@@ -190,7 +199,7 @@ def test_plan_packed_small_fronts(sim, name):
     want = [int(((m <= 8) & small).sum()), int(((m > 8) & (m <= 16) & small).sum())]
     assert want[0] > 0 and want[1] > 0
     for pack, expect in ((0, want), (2**30, [0, 0]), (512, None)):
-        pa = np.array((96, 32, 512, 256, 160, pack), np.int32)
+        pa = np.array((96, 32, 512, 256, 160, pack, 1), np.int32)
         h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
         try:
             out = (C.c_int64 * 6)()
