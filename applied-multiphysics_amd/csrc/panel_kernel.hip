@@ -5,7 +5,9 @@
 // each step's critical path is three dependent kernels. Here every workgroup owns one 64-row tile of the panel and
 // eliminates the diagonal block itself (the elimination is deterministic, so every workgroup holds the same factor and
 // inverse), forms its rows of L on the matrix cores, forms the rows of L that head the super-panel's later columns
-// (the column tiles of its update) the same way, and updates its rows of those columns. The reference's operations per
+// (the column tiles of its update) the same way, and updates its rows of those columns. The workgroup of the first
+// row tile updates the next step's diagonal block and eliminates it at once, so the later steps of a super-panel
+// start from its stored inverse instead of eliminating the block again. The reference's operations per
 // entry are those of sparse.mjs:992-1098 (Cholesky) and ldlt.mjs:490-560 (LDL^T); the products are the ones
 // k_gemm_tile computes (same operands, same depth order), so the factor is the three-launch one, bit for bit.
 #include <algorithm>
@@ -60,13 +62,15 @@ __device__ inline void waitLaunchCount(int32_t* ctr, int32_t target, int32_t* er
 // One workgroup per PanelTask (eight waves). Steps:
 //  1. issue the loads of the own row tile and of the column tiles that precede it (their rows of A, depths < nb);
 //  2. eliminate the diagonal block (diagCore: factored columns and the rows of W = L11^-1 in registers; GROWTH as
-//     k_diag_block: an FP64 redo when the largest multiplier exceeds the limit);
+//     k_diag_block: an FP64 redo when the largest multiplier exceeds the limit); kPanelPre: load its W and pivots;
 //  3. count the workgroup on the front's counter (the block and its A rows are read); the owner waits for every task
 //     of the front, then stores L11, W and the pivot records;
 //  4. with W in LDS (the B operand), P = A W^T for the own tile and each column tile (L = P, LDL^T: L = P D^-1);
 //  5. with the own L in LDS (the A operand) and each column tile's L (times D for LDL^T) as the B operand, the update
 //     C -= L_own D L_c^T of the own rows in that tile's columns (read, subtract, write);
-//  6. store the own L rows (in place; a tile that is also a column tile first waits for every task of the front).
+//  6. kPanelNext: eliminate the next block (updated in step 5, from registers through LDS) and store its L11, W and
+//     pivot records;
+//  7. store the own L rows (in place; a tile that is also a column tile first waits for every task of the front).
 // The column tiles of the update are the panel's first row tiles after the block (tile c: panel rows
 // [nb + 64 c, nb + 64 c + min(64, ucols - 64 c))), c <= the own tile's index: every entry written is at or below the
 // front's diagonal in the columns of the super-panel (the diagonal tile also writes its upper part, which no kernel
@@ -114,6 +118,35 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
     for (int c = 0; c < NCT; c++)
       if (c < nred) loadTile(pre[1 + c], c);
   }
+  const bool pivotLane = lane < nb && w == (lane / B) % NW;  // the lane that records column `lane`'s pivot
+  T wT[J];  // rows of W (stored precision): slot s holds row B (NW (s / B) + w) + s % B, column lane
+  T dT = T(1);
+  // the A rows and the block are read (every load of this workgroup has returned): count the workgroup
+  auto arrive = [&]() {
+#pragma unroll
+    for (int c = 0; c < 1 + NCT; c++)
+#pragma unroll
+      for (int q = 0; q < 8; q++) asm volatile("" ::"v"(pre[c][q]));
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(a.panelCtr + t.ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  if (t.flags & kPanelPre) {
+    // 2'. the block was factored by the launch before: its W and pivots
+    const T* Wt = a.W + t.w_off;
+#pragma unroll
+    for (int s = 0; s < J; s++) {
+      const int j = B * (NW * (s / B) + w) + s % B;
+      wT[s] = Wt[j + E * lane];
+    }
+    if (pivotLane && LDLT) dT = a.D[t.col0 + lane];
+    if constexpr (!kEarly) {
+      if (hasRows) loadTile(pre[0], -1);
+#pragma unroll
+      for (int c = 0; c < NCT; c++)
+        if (c < nred) loadTile(pre[1 + c], c);
+    }
+    arrive();
+  } else {
   // 2. the diagonal block
   const DiagTask dt{t.f_off, t.w_off, t.ld, t.nb, t.col0, 0};
   T recD = T(0), gmax = T(0);
@@ -121,9 +154,6 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
   int32_t recFail = INT32_MAX;
   T rA[J], rW[J];
   diagCore<T, LDLT>(dt, F, T(a.tau), w, lane, rA, rW, reinterpret_cast<T*>(ldsRaw), recD, recP, recFail, gmax);
-  const bool pivotLane = lane < nb && w == (lane / B) % NW;  // the lane that records column `lane`'s pivot
-  T wT[J];  // rows of W (stored precision)
-  T dT = T(1);
   bool redo = false;
   if constexpr (GROWTH && std::is_same<T, float>::value) {
     float g = gmax;
@@ -143,13 +173,8 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
 #pragma unroll
       for (int s = 0; s < J; s++) wT[s] = T(dW[s]);
       if (pivotLane) dT = T(dD);
-      // 3. (the redone block) the block and the A rows are read: count, then the owner stores
-#pragma unroll
-      for (int c = 0; c < 1 + NCT; c++)
-#pragma unroll
-        for (int q = 0; q < 8; q++) asm volatile("" ::"v"(pre[c][q]));
-      __syncthreads();
-      if (tid == 0) __hip_atomic_fetch_add(a.panelCtr + t.ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // 3. (the redone block) count, then the owner stores
+      arrive();
       if (t.owner) {
         waitLaunchCount(a.panelCtr + t.ctr, t.ntask, a.panelErr);
         if (tid == 0) atomicAdd(a.fallbacks, 1);
@@ -167,17 +192,13 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
       for (int c = 0; c < NCT; c++)
         if (c < nred) loadTile(pre[1 + c], c);
     }
-    // 3. the block and the A rows are read (every load of this workgroup has returned): count, then the owner stores
-#pragma unroll
-    for (int c = 0; c < 1 + NCT; c++)
-#pragma unroll
-      for (int q = 0; q < 8; q++) asm volatile("" ::"v"(pre[c][q]));
-    __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(a.panelCtr + t.ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // 3. count, then the owner stores
+    arrive();
     if (t.owner) {
       waitLaunchCount(a.panelCtr + t.ctr, t.ntask, a.panelErr);
       diagStore<T, LDLT>(dt, a, w, lane, rA, rW, recD, recP, recFail);
     }
+  }
   }
   if (!hasRows) return;  // a block with no rows below it
   T* opA = reinterpret_cast<T*>(ldsRaw);
@@ -220,6 +241,9 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
   for (int bj = 0; bj < 2; bj++)
 #pragma unroll
     for (int r = 0; r < 4; r++) lOwn[bj][r] = LDLT ? acc[bj][r] * (T(1) / dsh[colOf(bj, r)]) : acc[bj][r];
+  const bool next = t.flags & kPanelNext;
+  const int nbn = min(E, t.ucols);  // (next) the next block's columns
+  T nxt[2][4];                      // (next) this lane's entries of the updated next block
   if (nup > 0) {
     // 5. Q = L_own W (L_own as the A operand, W^T as the B operand: opB[k P + j] = W[k][j], W's row k = lane's slot)
     __syncthreads();  // the product has read opA and opB
@@ -238,7 +262,7 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
       for (int r = 0; r < 4; r++) qv[bj][r] = acc[bj][r];
     __syncthreads();
     stageAcc(opA, qv);  // Q as the A operand of every update
-    // 6. per column tile c: C -= Q A_c^T (A_c: the panel rows heading the tile's columns, as the B operand; for LDL^T
+    // per column tile c: C -= Q A_c^T (A_c: the panel rows heading the tile's columns, as the B operand; for LDL^T
     // L D L_c^T = L D D^-1 W A_c^T, so the pivots cancel)
 #pragma unroll
     for (int c = 0; c < NCT; c++) {
@@ -255,14 +279,60 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
       stage(opB, c == own ? pre[0] : pre[1 + (c < NCT ? c : 0)], wcols);
       __syncthreads();
       tileProduct<T>(opA, opB, w, lane, acc);
+      // (next: tile 0 is the next block; its lower part is stored once factored)
+      const bool blockTile = next && c == 0;
 #pragma unroll
       for (int bj = 0; bj < 2; bj++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int col = colOf(bj, r);
-          if (row < t.rows && col < wcols) C[row + int64_t(col) * ld] = cv[bj][r] - acc[bj][r];
+          const T v = cv[bj][r] - acc[bj][r];
+          if (blockTile) nxt[bj][r] = v;
+          if (row < t.rows && col < wcols && !(blockTile && row < nbn && row >= col)) C[row + int64_t(col) * ld] = v;
         }
     }
+  }
+  // 6. (kPanelNext: the first row tile, whose first update tile is the next step's diagonal block) the updated block,
+  // from the registers into LDS (column-major, pitch P, its lower part), factored and stored as that step's owner
+  // would store it
+  if (next) {
+    T* blk = opB;
+    __syncthreads();  // the last product has read opB
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = colOf(bj, r);
+        blk[row + col * P] = (row < nbn && row >= col) ? nxt[bj][r] : T(0);
+      }
+    __syncthreads();
+    const DiagTask rd{0, 0, P, nbn, t.col0 + E, 0};                                 // read from LDS
+    const DiagTask st{t.f_off + E + int64_t(E) * ld, t.w_off + E * E, t.ld, nbn, t.col0 + E, 0};  // store
+    T nD = T(0), ng = T(0);
+    uint8_t nP = 0;
+    int32_t nFail = INT32_MAX;
+    T nA[J], nW[J];
+    diagCore<T, LDLT>(rd, blk, T(a.tau), w, lane, nA, nW, opA, nD, nP, nFail, ng);
+    bool nredo = false;
+    if constexpr (GROWTH && std::is_same<T, float>::value) {
+      float g = ng;
+      for (int o = 32; o > 0; o >>= 1) g = fmaxf(g, __shfl_xor(g, o, 64));
+      if (lane == 0) gred[w] = double(g);
+      __syncthreads();
+      double gb = 0;
+#pragma unroll
+      for (int i = 0; i < NW; i++) gb = fmax(gb, gred[i]);
+      if (gb > a.growthLimit) {
+        nredo = true;
+        double dA[J], dW[J], dD = 0, dg = 0;
+        uint8_t dP = 0;
+        int32_t dFail = INT32_MAX;
+        diagCore<double, LDLT>(rd, blk, a.tau, w, lane, dA, dW, reinterpret_cast<double*>(opA), dD, dP, dFail, dg);
+        if (tid == 0) atomicAdd(a.fallbacks, 1);
+        diagStore<double, LDLT>(st, a, w, lane, dA, dW, dD, dP, dFail);
+      }
+    }
+    if (!nredo) diagStore<T, LDLT>(st, a, w, lane, nA, nW, nD, nP, nFail);
   }
   // 7. the own rows of L, in place (a column tile: once every task of the front has read it)
   if (own < nct) waitLaunchCount(a.panelCtr + t.ctr, t.ntask, a.panelErr);

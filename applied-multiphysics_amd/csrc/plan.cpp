@@ -301,9 +301,13 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
             const int32_t nb = std::min(kTileEdge, nc - k0);
             const int32_t ucols = std::max(0, std::min(nc, c0 + SPW) - (k0 + nb));
             const int32_t ntask = std::max(1, (m - k0 - nb + kTileEdge - 1) / kTileEdge);
+            // a later step of the super-panel: its block was factored by the previous step's first row tile (which
+            // exists: the previous step had these rows below it and this block's columns to update)
+            const bool pre = k0 > c0;
+            if (pre && m - k0 == nb) continue;  // factored already, and no rows below it: nothing left to do
             PanelTask t{S.Lxp[s] + k0 + int64_t(k0) * m,
                         P.Woff[s] + int64_t(k0 / kTileEdge) * kTileEdge * kTileEdge, m, nb, S.superStart[s] + k0,
-                        0, 0, ucols, P.panelCounters++, ntask, 1, 0};
+                        0, 0, ucols, P.panelCounters++, ntask, pre ? 0 : 1, pre ? kPanelPre : 0};
             P.panelMaxTasks = std::max(P.panelMaxTasks, ntask);
             if (m - k0 == nb) {
               P.panels.push_back(t);
@@ -312,7 +316,8 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
             for (int32_t ro = nb; ro < m - k0; ro += kTileEdge) {
               t.ro = ro;
               t.rows = std::min(kTileEdge, m - k0 - ro);
-              t.owner = ro == nb;
+              t.owner = !pre && ro == nb;
+              t.flags = (pre ? kPanelPre : 0) | (ro == nb && ucols > 0 ? kPanelNext : 0);
               P.panels.push_back(t);
             }
           }

@@ -88,10 +88,15 @@ struct ZeroTask {
 // the front's step have read them: counter ctr (one per front and launch) counts the tasks that have. A front's tasks
 // are consecutive in the launch, so with workgroups dispatched in order the writers of every front whose tasks are
 // all resident complete, and no wait can hold the launch (ntask must not exceed the resident workgroups).
+// flags: kPanelPre: the block was factored by the launch before (the next block of its super-panel's previous step),
+// so its tasks load W and D instead, and none is the owner; kPanelNext (the first row tile, when the super-panel has
+// a next step): after the update of its first column tile, which is the next step's diagonal block, the task
+// factors that block and stores it as that step's owner would.
 struct PanelTask {
   int64_t f_off, w_off;
-  int32_t ld, nb, col0, ro, rows, ucols, ctr, ntask, owner, pad;
+  int32_t ld, nb, col0, ro, rows, ucols, ctr, ntask, owner, flags;
 };
+constexpr int32_t kPanelPre = 1, kPanelNext = 2;
 constexpr int kPanelUpdateTiles = kSuperPanel / kTileEdge - 1;  // column tiles a panel task updates, at most
 
 enum LaunchKind : int32_t {

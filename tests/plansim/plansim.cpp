@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <vector>
 
@@ -137,41 +138,67 @@ void runDiag(Ctx& c, const DiagTask& t) {
 
 // A kPanel launch: every task reads the diagonal block and its panel rows as they were when the launch started (the
 // device's writers of those inputs wait until every workgroup has read them), so the writes of all tasks are
-// collected first and applied after. Per task: the block factored and inverted as runDiag does, the own rows
-// L = A W^T (LDL^T: / d) as a mode-2 product, the column tiles' rows the same way, and the update of the own rows in
-// the column tiles' columns, C -= L d Lc^T, as a mode-0 product.
+// collected first and applied after. Per task: the block factored and inverted as runDiag does (kPanelPre: its W
+// and pivots as the launch before stored them), the own rows L = A W^T (LDL^T: / d) as a mode-2 product, the column
+// tiles' rows the same way, and the update of the own rows in the column tiles' columns, C -= L d Lc^T, as a mode-0
+// product; kPanelNext: the updated first column tile's block factored and stored as the next step's owner would.
 void runPanelLaunch(Ctx& c, const Launch& L) {
   const FactorPlan& P = *c.P;
-  std::vector<std::pair<int64_t, double>> lxw, ww;
+  constexpr int32_t E = kTileEdge;
+  std::vector<std::pair<int64_t, double>> lxw, ww, nextL;
   // counter contract: a front's tasks are consecutive, share a counter no other front of any launch uses, and number
-  // ntask; exactly one of them is the owner (else a plan error, reported as failCol -2)
+  // ntask; one of them is the owner, none for a kPanelPre front, whose W must come from a kPanelNext task of an
+  // earlier launch (the first row tile of a step with columns to update); else a plan error, reported as failCol -2
   static thread_local std::vector<int32_t> seen;
-  if (L.first == 0 || seen.size() < size_t(P.panelCounters)) seen.assign(size_t(P.panelCounters), 0);
+  static thread_local std::set<int64_t> ahead;  // W offsets stored by kPanelNext tasks
+  if (L.first == 0 || seen.size() < size_t(P.panelCounters)) {
+    seen.assign(size_t(P.panelCounters), 0);
+    ahead.clear();
+  }
   for (int64_t q = L.first; q < L.first + L.count;) {
     const PanelTask& t0 = P.panels[size_t(q)];
+    const bool pre = t0.flags & kPanelPre;
     int64_t e = q;
     int32_t owners = 0;
-    while (e < L.first + L.count && P.panels[size_t(e)].ctr == t0.ctr) owners += P.panels[size_t(e++)].owner;
-    if (t0.ctr < 0 || t0.ctr >= P.panelCounters || seen[size_t(t0.ctr)]++ || e - q != t0.ntask || owners != 1 ||
-        t0.ntask > P.panelMaxTasks)
+    for (; e < L.first + L.count && P.panels[size_t(e)].ctr == t0.ctr; e++) {
+      const PanelTask& t = P.panels[size_t(e)];
+      owners += t.owner;
+      if (bool(t.flags & kPanelPre) != pre) c.failCol = -2;
+      if ((t.flags & kPanelNext) && (t.ro != t.nb || t.ucols <= 0 || t.nb != E || t.rows < std::min(E, t.ucols)))
+        c.failCol = -2;
+    }
+    if (t0.ctr < 0 || t0.ctr >= P.panelCounters || seen[size_t(t0.ctr)]++ || e - q != t0.ntask ||
+        owners != (pre ? 0 : 1) || t0.ntask > P.panelMaxTasks || (pre && (!ahead.count(t0.w_off) || t0.rows == 0)))
       c.failCol = -2;
     q = e;
   }
   for (int64_t q = L.first; q < L.first + L.count; q++) {
     const PanelTask& t = P.panels[size_t(q)];
-    const int32_t nb = t.nb, E = kTileEdge;
+    if (t.flags & kPanelNext) ahead.insert(t.w_off + int64_t(E) * E);
+  }
+  for (int64_t q = L.first; q < L.first + L.count; q++) {
+    const PanelTask& t = P.panels[size_t(q)];
+    const int32_t nb = t.nb;
     const int64_t ld = t.ld;
     const double* F0 = c.Lx.data() + t.f_off;
     std::vector<double> F(size_t(nb) * nb, 0.0), W(size_t(E) * E, 0.0);
-    for (int32_t j = 0; j < nb; j++)
-      for (int32_t i = j; i < nb; i++) F[i + size_t(j) * nb] = F0[i + j * ld];
-    partialFactor(c, F.data(), nb, nb, nb, t.col0);
-    for (int32_t j = 0; j < nb; j++)
-      for (int32_t i = j; i < nb; i++) {
-        double s = i == j ? 1.0 : 0.0;
-        for (int32_t k = j; k < i; k++) s -= F[i + size_t(k) * nb] * W[k + size_t(j) * E];
-        W[i + size_t(j) * E] = c.ldlt ? s : s / F[i + size_t(i) * nb];
-      }
+    // the factor and inverse of an nb x nb lower block Fb (in place; its pivots recorded at columns col0 ..)
+    auto factorBlock = [&c](std::vector<double>& Fb, std::vector<double>& Wb, int32_t n, int32_t col0) {
+      partialFactor(c, Fb.data(), n, n, n, col0);
+      for (int32_t j = 0; j < n; j++)
+        for (int32_t i = j; i < n; i++) {
+          double s = i == j ? 1.0 : 0.0;
+          for (int32_t k = j; k < i; k++) s -= Fb[i + size_t(k) * n] * Wb[k + size_t(j) * E];
+          Wb[i + size_t(j) * E] = c.ldlt ? s : s / Fb[i + size_t(i) * n];
+        }
+    };
+    if (t.flags & kPanelPre) {
+      std::copy(c.W.begin() + t.w_off, c.W.begin() + t.w_off + int64_t(E) * E, W.begin());
+    } else {
+      for (int32_t j = 0; j < nb; j++)
+        for (int32_t i = j; i < nb; i++) F[i + size_t(j) * nb] = F0[i + j * ld];
+      factorBlock(F, W, nb, t.col0);
+    }
     if (t.owner) {
       for (int32_t j = 0; j < nb; j++)
         for (int32_t i = j; i < nb; i++) lxw.emplace_back(t.f_off + i + j * ld, F[i + size_t(j) * nb]);
@@ -194,6 +221,7 @@ void runPanelLaunch(Ctx& c, const Launch& L) {
     for (int32_t j = 0; j < nb; j++)
       for (int32_t i = 0; i < t.rows; i++) lxw.emplace_back(t.f_off + t.ro + i + j * ld, Lo[i + size_t(j) * t.rows]);
     const int32_t own = (t.ro - nb) / E;
+    std::vector<double> Fn(size_t(E) * E, 0.0);  // (kPanelNext) the updated next block, its lower part
     for (int32_t ct = 0; ct * E < t.ucols && ct <= own; ct++) {
       const int32_t wc = std::min(E, t.ucols - E * ct);
       const std::vector<double> Lc = solveRows(nb + E * ct, wc);
@@ -207,9 +235,19 @@ void runPanelLaunch(Ctx& c, const Launch& L) {
           }
           const int64_t d = t.f_off + t.ro + i + (nb + E * ct + j) * ld;
           lxw.emplace_back(d, c.Lx[size_t(d)] - s);
+          if ((t.flags & kPanelNext) && ct == 0 && i >= j && i < wc) Fn[i + size_t(j) * wc] = c.Lx[size_t(d)] - s;
         }
+      if ((t.flags & kPanelNext) && ct == 0) {
+        // the next block (stored after the updates, so its factor replaces them)
+        std::vector<double> Wn(size_t(E) * E, 0.0);
+        factorBlock(Fn, Wn, wc, t.col0 + E);
+        for (int32_t j = 0; j < wc; j++)
+          for (int32_t i = j; i < wc; i++) nextL.emplace_back(t.f_off + E + i + (E + j) * ld, Fn[i + size_t(j) * wc]);
+        for (int32_t i = 0; i < E * E; i++) ww.emplace_back(t.w_off + int64_t(E) * E + i, Wn[size_t(i)]);
+      }
     }
   }
+  lxw.insert(lxw.end(), nextL.begin(), nextL.end());
   for (const auto& e : lxw) c.Lx[size_t(e.first)] = e.second;
   for (const auto& e : ww) c.W[size_t(e.first)] = e.second;
 }
@@ -484,8 +522,9 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
     const Launch& L = P.launches[l];
     curL = l;
     if (L.kind == kPanel) {
-      // pass 0: every task's reads (the block, its A rows and the column tiles' A rows, its C entries); pass 1: the
-      // writes (owner: the block, W, D; every task: its L rows and C entries)
+      // pass 0: every task's reads (the block, or kPanelPre its W and D; its A rows and the column tiles' A rows, its
+      // C entries); pass 1: the writes (owner: the block, W, D; every task: its L rows and C entries; kPanelNext: the
+      // next block, its W and D, before the task's wait, so not ordered after the launch's reads)
       for (int pass = 0; pass < 2; pass++) {
         readsOrdered = pass == 1;
         for (int64_t q = L.first; q < L.first + L.count; q++) {
@@ -493,13 +532,26 @@ int64_t ps_check_streams(void* p, int32_t dropWaits) {
           const PanelTask& t = P.panels[size_t(q)];
           const int32_t E = kTileEdge, nb = t.nb;
           const int64_t ld = t.ld;
-          const bool wr = pass == 1;
-          if (!wr || t.owner)
+          const bool wr = pass == 1, pre = t.flags & kPanelPre;
+          if (!wr && pre) {
+            for (int64_t i = 0; i < int64_t(E) * E; i++) acc(2, t.w_off + i, false);
+            for (int32_t k = 0; k < nb; k++) acc(3, t.col0 + k, false);
+          }
+          if ((!wr && !pre) || (wr && t.owner))
             for (int32_t j = 0; j < nb; j++)
               for (int32_t i = j; i < nb; i++) acc(0, t.f_off + i + j * ld, wr);
           if (wr && t.owner) {
             for (int64_t i = 0; i < int64_t(E) * E; i++) acc(2, t.w_off + i, true);
             for (int32_t k = 0; k < nb; k++) acc(3, t.col0 + k, true);
+          }
+          if (wr && (t.flags & kPanelNext)) {
+            readsOrdered = false;
+            const int32_t nbn = std::min(E, t.ucols);
+            for (int32_t j = 0; j < nbn; j++)
+              for (int32_t i = j; i < nbn; i++) acc(0, t.f_off + E + i + (E + j) * ld, true);
+            for (int64_t i = 0; i < int64_t(E) * E; i++) acc(2, t.w_off + int64_t(E) * E + i, true);
+            for (int32_t k = 0; k < nbn; k++) acc(3, t.col0 + E + k, true);
+            readsOrdered = true;
           }
           if (t.rows == 0) continue;
           for (int32_t j = 0; j < nb; j++)
