@@ -15,12 +15,14 @@
 // 16 bytes per lane, 1 KB per instruction, with no conversion.
 //
 // Tiles of 128 x 128 (modes 0: C -= A B^T and 1: C = -A B^T of plan.h); eight waves in 4 x 2, each 32 rows x 64
-// columns of 16 x 16 blocks. Depth chunks of 32: a thread loads the heads and tails of 8 consecutive depths of one row
+// columns of 16 x 16 blocks, the matrix cores fed the transposed product so that C is read and written along
+// columns. Depth chunks of 32: a thread loads the heads and tails of 8 consecutive depths of one row
 // of A and of B and writes them to LDS row-major (16-byte groups XOR-swizzled by row so that the 16 rows a
 // quarter-wave reads fall in distinct banks). Two LDS buffers, the chunk after next in registers, one barrier per
 // chunk.
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "factor_kernels.h"
 
@@ -32,11 +34,18 @@ typedef __bf16 bf16x8 __attribute__((__vector_size__(16)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kG = 128, kKC = kSplitDepth;
+constexpr int kFullDepth = kSuperPanel;  // depth of the bulk products of a whole super-panel
 constexpr int kRowBytes = kKC * 2;  // bf16 depth row of a tile row in LDS (64 bytes, four 16-byte groups)
 static_assert(kKC == 32, "four 16-byte groups of depths per tile row");
 
 // byte offset of 16-byte group g (depths 8 g .. 8 g + 7) of tile row r
 __device__ inline int ldsOff(int r, int g) { return r * kRowBytes + 16 * (g ^ ((r >> 2) & 3)); }
+
+// f(integral_constant<int, i>) for i = 0, 1, ... in order
+template <typename F, int... I>
+__device__ inline void forEachIndex(std::integer_sequence<int, I...>, F& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
 
 __device__ inline void split8(const float* x, bf16x8& h, bf16x8& l) {
 #pragma unroll
@@ -90,7 +99,7 @@ __global__ __launch_bounds__(256) void k_split_panel(const SplitTask* __restrict
   }
 }
 
-__global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restrict__ tasks,
+__global__ __launch_bounds__(512, 4) void k_gemm_bf16x3(const GemmTask* __restrict__ tasks,
                                                         const SplitRef* __restrict__ refs, float* Lx, float* U,
                                                         const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
                                                         const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl) {
@@ -116,7 +125,7 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
   for (int bi = 0; bi < NBI; bi++)
 #pragma unroll
     for (int bj = 0; bj < NBJ; bj++) acc[bi][bj] = f4{0, 0, 0, 0};
-  bf16x8 ra[2][2], rb[2][2];  // [register set][head, tail]
+  bf16x8 ra[3][2], rb[3][2];  // [register set][head, tail]
   auto load = [&](auto set, int k0) {
     constexpr int S = decltype(set)::value;
     const int64_t c = int64_t(k0 / kKC) * cp;
@@ -151,14 +160,68 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
       const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&lds[buf][3][off]);
 #pragma unroll
       for (int bi = 0; bi < NBI; bi++) {
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bl, acc[bi][bj], 0, 0, 0);
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bh, acc[bi][bj], 0, 0, 0);
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bh, acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah[bi], acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al[bi], acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah[bi], acc[bi][bj], 0, 0, 0);
       }
     }
   };
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
+  // the matrix cores are fed the transposed product (B's fragment first), so component q of acc[bi][bj] is C row
+  // wr + 16 bi + lane % 16, column wc + 16 bj + 4 (lane / 16) + q: the 16 lanes of a row group cover 64 consecutive
+  // bytes of a C column, and one load or store instruction four such runs
+  const int row0 = wr + (lane & 15), colq = wc + 4 * (lane >> 4);
+  if (t.k == kFullDepth) {
+    // a full super-panel (the common case): its chunks at compile time, each loaded three chunks ahead (three register
+    // sets), and the C entries of mode 0 read while the last chunk runs (unconditional, at clamped addresses, so that
+    // the chunk loads before them are waited for alone; a step earlier the kernel would spill a register)
+    constexpr int NC = kFullDepth / kKC;
+    static_assert(NBI == 2, "two row blocks per wave");
+    float cv[NBJ][4][NBI];
+    load(S0{}, 0);
+    load(S1{}, kKC);
+    load(std::integral_constant<int, 2>{}, 2 * kKC);
+    stash(S0{}, 0);
+    __syncthreads();
+    auto fixedStep = [&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      if constexpr (c + 3 < NC) load(std::integral_constant<int, c % 3>{}, (c + 3) * kKC);
+      if constexpr (c == NC - 1) {
+        const float* Cr = C + min(row0, t.rows - 1);
+        const int d16 = min(row0 + 16, t.rows - 1) - min(row0, t.rows - 1);
+#pragma unroll
+        for (int bj = 0; bj < NBJ; bj++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const float* Cc = Cr + int64_t(min(colq + 16 * bj + q, t.cols - 1)) * t.ldc;
+            cv[bj][q][0] = Cc[0];
+            cv[bj][q][1] = Cc[d16];
+          }
+      }
+      compute(c & 1);
+      if constexpr (c + 1 < NC) {
+        stash(std::integral_constant<int, (c + 1) % 3>{}, (c + 1) & 1);
+        __syncthreads();
+      }
+    };
+    forEachIndex(std::make_integer_sequence<int, NC>{}, fixedStep);
+    if (!active) return;
+#pragma unroll
+    for (int bj = 0; bj < NBJ; bj++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int col = colq + 16 * bj + q;
+        if (col >= t.cols) continue;
+        float* Cc = C + int64_t(col) * t.ldc;
+#pragma unroll
+        for (int bi = 0; bi < NBI; bi++) {
+          const int row = row0 + 16 * bi;
+          if (row < t.rows) Cc[row] = (t.mode == 0 ? cv[bj][q][bi] : 0.0f) - acc[bi][bj][q];
+        }
+      }
+    return;
+  }
   // the split rows hold whole depth chunks (zero past the panel width): every chunk below t.k is in bounds
   load(S0{}, 0);
   if (kKC < t.k) load(S1{}, kKC);
@@ -183,35 +246,26 @@ __global__ __launch_bounds__(512, 2) void k_gemm_bf16x3(const GemmTask* __restri
   step(S1{}, k0, std::false_type{});
   if (k0 + kKC < t.k) step(S0{}, k0 + kKC, std::false_type{});
   if (!active) return;
-  // component q of acc[bi][bj]: C row wr + 16 bi + 4 (lane / 16) + q, column wc + 16 bj + lane % 16. Mode 0 reads a
-  // 16-column block's C entries, then writes them (reading all 32 per lane first measured slower)
-  const int rowq = wr + 4 * (lane >> 4), col0 = wc + (lane & 15);
+  // mode 0 reads a 16-column block's C entries (addresses clamped into the tile), then writes them
 #pragma unroll
   for (int bj = 0; bj < NBJ; bj++) {
-    const int col = col0 + 16 * bj;
-    if (col >= t.cols) continue;
-    float* Cc = C + int64_t(col) * t.ldc;
-    if (t.mode == 0) {
-      float cv[NBI][4];
+    float cv[4][NBI];
 #pragma unroll
-      for (int bi = 0; bi < NBI; bi++)
+    for (int q = 0; q < 4; q++) {
+      const float* Cc = C + int64_t(min(colq + 16 * bj + q, t.cols - 1)) * t.ldc;
 #pragma unroll
-        for (int q = 0; q < 4; q++) cv[bi][q] = Cc[min(rowq + 16 * bi + q, t.rows - 1)];
+      for (int bi = 0; bi < NBI; bi++) cv[q][bi] = t.mode == 0 ? Cc[min(row0 + 16 * bi, t.rows - 1)] : 0.0f;
+    }
 #pragma unroll
-      for (int bi = 0; bi < NBI; bi++)
+    for (int q = 0; q < 4; q++) {
+      const int col = colq + 16 * bj + q;
+      if (col >= t.cols) continue;
+      float* Cc = C + int64_t(col) * t.ldc;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int row = rowq + 16 * bi + q;
-          if (row < t.rows) Cc[row] = cv[bi][q] - acc[bi][bj][q];
-        }
-    } else {
-#pragma unroll
-      for (int bi = 0; bi < NBI; bi++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int row = rowq + 16 * bi + q;
-          if (row < t.rows) Cc[row] = -acc[bi][bj][q];
-        }
+      for (int bi = 0; bi < NBI; bi++) {
+        const int row = row0 + 16 * bi;
+        if (row < t.rows) Cc[row] = cv[q][bi] - acc[bi][bj][q];
+      }
     }
   }
 }
