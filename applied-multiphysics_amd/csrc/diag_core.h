@@ -1,5 +1,7 @@
 // Elimination of one 64 x 64 diagonal block of a tiled front (device helpers of k_diag_block and k_panel_step).
 #pragma once
+#include <type_traits>
+
 #include "pivot.h"
 
 namespace amp {
@@ -8,17 +10,13 @@ constexpr int kDiagWaves = 8;
 constexpr int kDiagStep = 4;  // columns eliminated per barrier
 constexpr int kDiagSlots = kDiagStep * (kTileEdge / kDiagStep / kDiagWaves);  // columns per lane
 
-// The elimination of one block in compute precision TC from storage of precision TS (F: the block, leading dimension
-// t.ld). Leaves the factored columns in rA, the inverse's rows in rW, the owner lanes' pivot records, and in gmax the
-// lane's largest elimination multiplier |l_ik / l_kk| (Cholesky) or |l_ik| (LDL^T). lds: 3 x 2 x B x 64 entries of TC.
-template <typename TC, bool LDLT, typename TS>
-__device__ __forceinline__ void diagCore(const DiagTask& t, const TS* F, TC tau, int w, int lane, TC (&rA)[kDiagSlots],
-                                         TC (&rW)[kDiagSlots], TC* lds, TC& recD, uint8_t& recP, int32_t& recFail,
-                                         TC& gmax) {
-  constexpr int E = kTileEdge, NW = kDiagWaves, B = kDiagStep, NG = E / B, J = kDiagSlots;
-  TC* colL = lds;                // [2][B][E]
-  TC* colD = lds + 2 * B * E;    // [2][B][E]
-  TC* rowW = lds + 4 * B * E;    // [2][B][E]
+// The block's lower part (F: the block, leading dimension t.ld, storage precision TS) into the slots rA in compute
+// precision TC, and the identity into rW: the elimination's inputs.
+template <typename TC, typename TS>
+__device__ __forceinline__ void diagLoad(const DiagTask& t, const TS* F, int w, int lane, TC (&rA)[kDiagSlots],
+                                         TC (&rW)[kDiagSlots]) {
+  constexpr int E = kTileEdge, NW = kDiagWaves, B = kDiagStep, J = kDiagSlots;
+  static_assert(E == B * NW * (J / B), "slots cover the block");
   const int nb = t.nb;
 #pragma unroll
   for (int s = 0; s < J; s++) {
@@ -26,15 +24,33 @@ __device__ __forceinline__ void diagCore(const DiagTask& t, const TS* F, TC tau,
     rA[s] = (j < nb && lane < nb && lane >= j) ? TC(F[lane + int64_t(j) * t.ld]) : TC(0);
     rW[s] = j == lane ? TC(1) : TC(0);
   }
+}
+
+// The elimination of the loaded slots in compute precision TC. Leaves the factored columns in rA, the inverse's rows
+// in rW, the owner lanes' pivot records, and in gmax the lane's largest elimination multiplier |l_ik / l_kk|
+// (Cholesky) or |l_ik| (LDL^T). lds: 3 x 2 x B x 64 entries of TC, aligned to B entries.
+template <typename TC, bool LDLT>
+__device__ __forceinline__ void diagEliminate(const DiagTask& t, TC tau, int w, int lane, TC (&rA)[kDiagSlots],
+                                              TC (&rW)[kDiagSlots], TC* lds, TC& recD, uint8_t& recP,
+                                              int32_t& recFail, TC& gmax) {
+  constexpr int E = kTileEdge, NW = kDiagWaves, B = kDiagStep, NG = E / B, J = kDiagSlots;
+  using TV = typename Vec4<TC>::type;
+  static_assert(B == 4, "a group's published entries of one row are one vector");
+  // published groups, row-major ([2][E] vectors of the group's B entries): one vector access per row and array
+  TV* colL = reinterpret_cast<TV*>(lds);              // multipliers l
+  TV* colD = reinterpret_cast<TV*>(lds + 2 * B * E);  // weights l * mult
+  TV* rowW = reinterpret_cast<TV*>(lds + 4 * B * E);  // W row entries
+  const int nb = t.nb;
   // pivot step of group G (owner wave, slots s0 .. s0 + B - 1): its columns one after another, each pivoted, scaled
   // and applied to the group's later columns and W rows (multipliers read from their own lanes); published as entry
   // c of buffer p. Columns at or beyond nb publish zeros.
-  auto pivotGroup = [&](const int G) {
-    const int s0 = B * (G / NW), p = G & 1;
+  auto pivotGroup = [&](const int G, auto s1) {
+    constexpr int s0 = decltype(s1)::value;  // B (G / NW)
+    const int p = G & 1;
+    TV lv, cdv, wv;
 #pragma unroll
     for (int c = 0; c < B; c++) {
       const int k = B * G + c;
-      const int at = (p * B + c) * E + lane;
       if (k < nb) {
         TC inv, mult;
         const TC dk = laneValue(rA[s0 + c], k);
@@ -46,9 +62,9 @@ __device__ __forceinline__ void diagCore(const DiagTask& t, const TS* F, TC tau,
         const TC wk = LDLT ? rW[s0 + c] : rW[s0 + c] * inv;
         rW[s0 + c] = wk;
         const TC cd = l * mult;
-        colL[at] = l;
-        colD[at] = cd;
-        rowW[at] = wk;
+        lv[c] = l;
+        cdv[c] = cd;
+        wv[c] = wk;
 #pragma unroll
         for (int c2 = c + 1; c2 < B; c2++) {
           const TC lk = laneValue(l, B * G + c2);  // l of the group's row B G + c2
@@ -56,42 +72,42 @@ __device__ __forceinline__ void diagCore(const DiagTask& t, const TS* F, TC tau,
           rW[s0 + c2] -= lk * wk;
         }
       } else {
-        colL[at] = TC(0);
-        colD[at] = TC(0);
-        rowW[at] = TC(0);
+        lv[c] = TC(0);
+        cdv[c] = TC(0);
+        wv[c] = TC(0);
       }
     }
+    colL[p * E + lane] = lv;
+    colD[p * E + lane] = cdv;
+    rowW[p * E + lane] = wv;
   };
-  if (nb > 0 && w == 0) pivotGroup(0);
+  if (nb > 0 && w == 0) pivotGroup(0, std::integral_constant<int, 0>{});
   __syncthreads();
   // rank-B update of slot s from the group published in buffer p (cd, rw: this lane's weights and W row entries)
-  auto update = [&](const int s, const int j, const int p, const TC (&cd)[B], const TC (&rw)[B]) {
+  auto update = [&](const int s, const int j, const int p, const TV& cd, const TV& rw) {
+    const TV lj = colL[p * E + j];
     TC da = TC(0), dw = TC(0);
 #pragma unroll
     for (int c = 0; c < B; c++) {
-      const TC lj = colL[(p * B + c) * E + j];
-      da += cd[c] * lj;
-      dw += lj * rw[c];
+      da += cd[c] * lj[c];
+      dw += lj[c] * rw[c];
     }
     rA[s] -= da;
     rW[s] -= dw;
   };
-#pragma clang loop unroll(full)
-  for (int G = 0; G < NG; G++) {
-    if (B * G >= nb) continue;
+  // group G's step: the owner of group G + 1 updates that group's slots and pivots it, every wave applies group G to
+  // its later slots. A runtime loop (the straight-line form is too large for the instruction cache, and a workgroup
+  // runs it once); `s1`: the first slot of group G + 1 in its owner wave, B (G + 1) / (B NW) (compile-time, so that the
+  // slots stay in registers)
+  auto groupStep = [&](const int G, auto s1) {
     const int p = G & 1, G1 = G + 1;
-    TC cd[B], rw[B];
-#pragma unroll
-    for (int c = 0; c < B; c++) {
-      cd[c] = colD[(p * B + c) * E + lane];
-      rw[c] = rowW[(p * B + c) * E + lane];
-    }
+    const TV cd = colD[p * E + lane], rw = rowW[p * E + lane];
     const bool ahead = B * G1 < nb && w == (G1 % NW);
     if (G1 < NG && ahead) {
-      const int s0 = B * (G1 / NW);
+      constexpr int s0 = decltype(s1)::value;
 #pragma unroll
       for (int c = 0; c < B; c++) update(s0 + c, B * G1 + c, p, cd, rw);
-      pivotGroup(G1);
+      pivotGroup(G1, s1);
     }
 #pragma unroll
     for (int s = 0; s < J; s++) {
@@ -99,7 +115,22 @@ __device__ __forceinline__ void diagCore(const DiagTask& t, const TS* F, TC tau,
       if (NW * (s / B) + NW - 1 > G && Gs > G && !(ahead && Gs == G1)) update(s, j, p, cd, rw);
     }
     __syncthreads();
-  }
+  };
+  static_assert(J == 2 * B, "two groups per wave: groups 0 .. NW - 1 in slots 0 .. B - 1, the rest in B .. 2 B - 1");
+  const int ng = (nb + B - 1) / B;  // groups with columns
+#pragma unroll 1
+  for (int G = 0; G < min(ng, NW - 1); G++) groupStep(G, std::integral_constant<int, 0>{});
+#pragma unroll 1
+  for (int G = NW - 1; G < ng; G++) groupStep(G, std::integral_constant<int, B>{});
+}
+
+// diagLoad, then diagEliminate.
+template <typename TC, bool LDLT, typename TS>
+__device__ __forceinline__ void diagCore(const DiagTask& t, const TS* F, TC tau, int w, int lane, TC (&rA)[kDiagSlots],
+                                         TC (&rW)[kDiagSlots], TC* lds, TC& recD, uint8_t& recP, int32_t& recFail,
+                                         TC& gmax) {
+  diagLoad(t, F, w, lane, rA, rW);
+  diagEliminate<TC, LDLT>(t, tau, w, lane, rA, rW, lds, recD, recP, recFail, gmax);
 }
 
 // Store the block's factor (in place), the inverse W (column-major, edge 64, straight from the registers: each store

@@ -22,7 +22,7 @@ namespace amp {
 template <typename T, bool LDLT, bool GROWTH>
 __global__ __launch_bounds__(64 * kDiagWaves) void k_diag_block(const DiagTask* __restrict__ tasks, FactorArgs<T> a) {
   constexpr int J = kDiagSlots;
-  __shared__ double ldsRaw[3 * 2 * kDiagStep * kTileEdge];
+  __shared__ __attribute__((aligned(32))) double ldsRaw[3 * 2 * kDiagStep * kTileEdge];
   __shared__ double gred[kDiagWaves];
   chainPriority();
   const DiagTask t = tasks[blockIdx.x];

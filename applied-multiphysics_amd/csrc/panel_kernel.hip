@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
   constexpr int B = kDiagStep;
   // the elimination's scratch and, after it, the two operand tiles
   constexpr size_t kOpBytes = size_t(2) * E * P * sizeof(T), kDiagBytes = size_t(3) * 2 * B * E * sizeof(double);
-  __shared__ double ldsRaw[(kOpBytes > kDiagBytes ? kOpBytes : kDiagBytes) / sizeof(double)];
+  __shared__ __attribute__((aligned(32))) double ldsRaw[(kOpBytes > kDiagBytes ? kOpBytes : kDiagBytes) / sizeof(double)];
   __shared__ double gred[NW];
   __shared__ T dsh[E];  // pivots d_k (LDL^T; 1 past nb)
   chainPriority();
@@ -147,13 +147,22 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
     }
     arrive();
   } else {
-  // 2. the diagonal block
+  // 2. the diagonal block. Without the growth check (whose FP64 redo reads the block again) the workgroup counts
+  // itself as soon as the block and its A rows are loaded, so that the owner's store does not wait for the other
+  // workgroups' eliminations
   const DiagTask dt{t.f_off, t.w_off, t.ld, t.nb, t.col0, 0};
   T recD = T(0), gmax = T(0);
   uint8_t recP = 0;
   int32_t recFail = INT32_MAX;
   T rA[J], rW[J];
-  diagCore<T, LDLT>(dt, F, T(a.tau), w, lane, rA, rW, reinterpret_cast<T*>(ldsRaw), recD, recP, recFail, gmax);
+  constexpr bool kEarlyArrive = kEarly && !GROWTH;
+  diagLoad(dt, F, w, lane, rA, rW);
+  if constexpr (kEarlyArrive) {
+#pragma unroll
+    for (int s = 0; s < J; s++) asm volatile("" ::"v"(rA[s]));
+    arrive();
+  }
+  diagEliminate<T, LDLT>(dt, T(a.tau), w, lane, rA, rW, reinterpret_cast<T*>(ldsRaw), recD, recP, recFail, gmax);
   bool redo = false;
   if constexpr (GROWTH && std::is_same<T, float>::value) {
     float g = gmax;
@@ -192,8 +201,8 @@ __global__ __launch_bounds__(64 * kDiagWaves, sizeof(T) == 8 ? 4 : 1) void k_pan
       for (int c = 0; c < NCT; c++)
         if (c < nred) loadTile(pre[1 + c], c);
     }
-    // 3. count, then the owner stores
-    arrive();
+    // 3. count (unless counted already), then the owner stores
+    if constexpr (!kEarlyArrive) arrive();
     if (t.owner) {
       waitLaunchCount(a.panelCtr + t.ctr, t.ntask, a.panelErr);
       diagStore<T, LDLT>(dt, a, w, lane, rA, rW, recD, recP, recFail);
