@@ -27,7 +27,7 @@ class SymInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("nsuper", C.c_int32), ("maxFront", C.c_int32), ("nlevels", C.c_int32),
                 ("nnzL", C.c_int64), ("nnzA", C.c_int64), ("nnzLStored", C.c_int64), ("nnzPattern", C.c_int64),
                 ("flops", C.c_double), ("fill", C.c_double), ("gemmFlops", C.c_double),
-                ("patternKey", C.c_char * 64)]
+                ("bulkBytes", C.c_double), ("bulkLaunches", C.c_int32), ("patternKey", C.c_char * 64)]
 
 
 class SymArrays(C.Structure):

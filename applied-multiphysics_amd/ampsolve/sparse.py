@@ -92,6 +92,7 @@ class Symbolic:
         self.nnzL, self.nnzA, self.nnzLStored = int(info.nnzL), int(info.nnzA), int(info.nnzLStored)
         self.nnzPattern = int(info.nnzPattern)
         self.flops, self.fill, self.gemmFlops = info.flops, info.fill, info.gemmFlops
+        self.bulkBytes, self.bulkLaunches = info.bulkBytes, int(info.bulkLaunches)
         self.patternKey = info.patternKey.decode()
         arr = A.SymArrays()
         A.check(A.lib().amp_sym_get_arrays(self._h, C.byref(arr)))
@@ -132,6 +133,7 @@ class Symbolic:
         info = A.SymInfo()
         A.check(A.lib().amp_sym_get_info(self._h, C.byref(info)))
         self.gemmFlops = info.gemmFlops
+        self.bulkBytes, self.bulkLaunches = info.bulkBytes, int(info.bulkLaunches)
 
     def __del__(self):
         h = getattr(self, "_h", None)

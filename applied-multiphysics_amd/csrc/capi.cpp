@@ -241,8 +241,8 @@ int runFactor(amp_num* num, const double* d_values, double pivotTol, amp_factor_
   for (auto it = num->kernelMs.begin(); it != num->kernelMs.end();)
     it = it->first.rfind("solve_", 0) == 0 ? std::next(it) : num->kernelMs.erase(it);
   if (num->profiling) {
-    static const char* names[] = {"zero",       "assemble",  "small_front", "extend_add",
-                                  "diag_block", "gemm_tile", "split_panel", "panel_step"};
+    static const char* names[] = {"zero",       "assemble",  "small_front", "extend_add", "diag_block",
+                                  "gemm_tile", "split_panel", "panel_step", "gemm_bf16x3"};
     for (size_t i = 0; i < num->prof.used; i++) {
       float k = 0;
       HIP_TRY(hipEventElapsedTime(&k, num->prof.events[2 * i], num->prof.events[2 * i + 1]));
@@ -555,6 +555,8 @@ int amp_sym_get_info(const amp_sym* h, amp_sym_info* info) {
   info->flops = S.flops;
   info->fill = S.fill;
   info->gemmFlops = h->F.gemmFlops;
+  info->bulkBytes = h->F.bulkBytes;
+  info->bulkLaunches = h->F.bulkLaunches;
   std::snprintf(info->patternKey, sizeof info->patternKey, "%s", S.patternKey.c_str());
   return AMP_OK;
 }

@@ -15,7 +15,7 @@ namespace amp {
 // step G: every wave applies the rank-B update to its active columns (A[i][j] -= (l_ik d_k) l_jk, l_jk read as LDS
 // broadcasts; rows above the diagonal of column j are never read or stored) and W rows (W[j][:] -= l_jk W[k][:]).
 // Lookahead: the owner of group G + 1 updates its columns first and runs its pivot step while the other waves finish
-// step G. The step loop is unrolled, so every register index is a constant.
+// step G. The step loop runs at run time; within a step every register index is a constant.
 // GROWTH (FP32 factor with a growth limit, a.growthLimit > 0): the block is eliminated in FP32; when its largest
 // elimination multiplier exceeds the limit, the block (still unmodified in memory) is eliminated again in FP64
 // arithmetic, with FP64 pivots, and stored rounded to FP32; a.fallbacks counts such blocks.

@@ -25,8 +25,9 @@ struct DeviceFactorPlan {
   std::vector<Launch> launches;
 };
 
-// Optional per-launch timing: events[2 i], events[2 i + 1] bracket launch i; kinds[i] is its launch kind and
-// levels[i] the elimination-tree level it serves.
+// Optional per-launch timing: events[2 i], events[2 i + 1] bracket launch i; kinds[i] is its launch kind (or
+// kProfGemmBf16x3 for a bulk launch run as bf16 products) and levels[i] the elimination-tree level it serves.
+constexpr int32_t kProfGemmBf16x3 = 8;
 struct ProfileSink {
   std::vector<hipEvent_t> events;
   std::vector<int32_t> kinds, levels;

@@ -684,6 +684,7 @@ void launchFactorPlan(const DeviceFactorPlan& dp, const FactorArgs<T>& a, const 
           // bulk launches (side stream: modes 0 / 1 only) of a factor with bf16 bulk updates, on their split panel
           if (a.bulkBf16 && L.edge == kGemmEdge && (L.sync & kSyncSide)) {
             launchGemmBf16x3(gt, dp.srefs + L.first, grid, a, st);
+            if (prof && 2 * prof->used + 1 < prof->events.size()) prof->kinds[prof->used] = kProfGemmBf16x3;
             break;
           }
         }

@@ -425,6 +425,23 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm) {
     }
     P.gemmFlops += 2.0 * entries * t.k;
   }
+  for (size_t l = 1; l < P.launches.size(); l++) {
+    const Launch& L = P.launches[l];
+    const Launch& Sl = P.launches[l - 1];
+    if (L.kind != kGemm || Sl.kind != kSplit) continue;
+    P.bulkLaunches++;
+    for (int64_t i = Sl.first; i < Sl.first + Sl.count; i++)
+      P.bulkBytes += 4.0 * double(P.splits[size_t(i)].rows) * P.splits[size_t(i)].w;
+    for (int64_t i = L.first; i < L.first + L.count; i++) {
+      const GemmTask& t = P.gemm[size_t(i)];
+      double entries = double(t.rows) * t.cols;
+      if (t.a_off == t.b_off) {  // a tile on the diagonal: its lower part
+        entries = 0;
+        for (int32_t j = 0; j < t.cols; j++) entries += std::max(0, t.rows - j);
+      }
+      P.bulkBytes += entries * (t.mode == 0 ? 8.0 : 4.0);
+    }
+  }
   P.srefs.resize(P.gemm.size(), SplitRef{0, 0, 0, 0});
   return P;
 }

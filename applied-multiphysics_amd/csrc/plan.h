@@ -132,6 +132,11 @@ struct FactorPlan {
   int64_t nnzLStored = 0, uEntries = 0, wEntries = 0, nnzA = 0, nnzPattern = 0;
   double flops = 0;
   double gemmFlops = 0;  // algorithmic flops of the gemm tasks: lower-triangle outputs, triangular solves as such
+  // algorithmic HBM bytes of the wide bulk product launches (those with a panel split in front): every lower entry of
+  // their outputs read and written (mode 0) or written (mode 1), 4 bytes each way, and every entry of the split
+  // panels read once (4 bytes: an FP32 value, or its bf16 head and tail)
+  double bulkBytes = 0;
+  int32_t bulkLaunches = 0;
   // per supernode
   std::vector<int32_t> level, isBig;
   std::vector<int64_t> Uoff;  // offset of the mu x mu update matrix (leading dimension mu), -1 when mu == 0

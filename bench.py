@@ -21,8 +21,9 @@ sys.path.insert(0, os.path.join(ROOT, "applied-multiphysics_amd"))
 METRIC = "factor+solve s & TFLOP/s vs peak, 3D P1 heat 61³ nodes, FP64 vs mixed @1e-10 resid"
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (TFLOP/s): FP64, FP32
 PEAK_HBM = 8000.0  # MI355X HBM3E (GB/s)
-KERNEL_CLASSES = ("zero", "assemble", "small_front", "extend_add", "diag_block", "panel_step", "gemm_tile", "solve_small_fwd",
-                  "solve_gather", "solve_fwd_flow", "solve_small_bwd", "solve_bwd_gather", "solve_bwd_init", "solve_bwd_flow", "solve_total")
+KERNEL_CLASSES = ("zero", "assemble", "small_front", "extend_add", "diag_block", "panel_step", "gemm_tile", "split_panel",
+                  "gemm_bf16x3", "solve_small_fwd", "solve_gather", "solve_fwd_flow", "solve_small_bwd", "solve_bwd_gather",
+                  "solve_bwd_init", "solve_bwd_flow", "solve_total")
 
 
 def log(*a):
@@ -80,10 +81,10 @@ def cpu_baseline(pat, values, b, target_flops):
     return rec, x
 
 
-def gemm_traffic(path):
-    """HBM bytes per k_gemm_tile launch of the given path, from the committed counter summary (rocprofv3 --pmc
-    FETCH_SIZE / WRITE_SIZE runs of this benchmark, scaled by the same counters on a plain copy), or None."""
-    f = os.path.join(ROOT, "profiles", "gemm_tile_heat3d61_hbm_traffic.txt")
+def gemm_traffic(path, summary="gemm_tile_heat3d61_hbm_traffic.txt"):
+    """HBM bytes per launch of the product kernel of the given path, from a committed counter summary (rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE runs of this benchmark, scaled by the same counters on a plain copy), or None."""
+    f = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(f):
         return None
     for line in open(f):
@@ -167,9 +168,11 @@ def run_path(name, sym, dv, db, dx, steps, warmup, tol, log_prefix, method="ir",
         kinds = {k: num.kernel_ms(k) for k in KERNEL_CLASSES}
         num.set_profiling(False)
         gm, gn = kinds["gemm_tile"]
+        bm, bn = kinds["gemm_bf16x3"]
         st_ms, st_n = kinds["solve_total"]
         res["solve_ms_per_pass"] = st_ms / st_n if st_n else 0.0
         res.update({"factor_ms_profiled": float(num.info["ms"]), "gemm_ms_per_step": gm, "gemm_launches_per_step": gn,
+                    "bf16x3_ms_per_step": bm, "bf16x3_launches_per_step": bn,
                     "kernel_ms_profiled_step": {k: v[0] for k, v in kinds.items()}})
         log(f"{log_prefix}: {1e3 * dt:.2f} ms/step; profiled step by class: " +
             ", ".join(f"{k} {v[0]:.2f} ms/{v[1]}" for k, v in kinds.items()))
@@ -268,10 +271,12 @@ def main():
     ok = {k: v[0] for k, v in paths.items() if v[0]["relres"] <= args.tol}
     best = min(ok, key=lambda k: ok[k]["ms_per_step"]) if ok else next(iter(paths))
     r = paths[best][0]
-    # roofline of k_gemm_tile from the profiled step of the same precision's eager path
+    # roofline of the dominant kernel, from the profiled step of the same precision's eager path: k_gemm_tile (matrix
+    # cores) for the FP64 and FP32 paths; for the bf16 path its bulk products k_gemm_bf16x3, which are bound by their
+    # HBM traffic (the trailing matrix read and written once per super-panel)
     eager_of = best[:-len("-graph")] if best.endswith("-graph") else best
     prof = paths.get(eager_of, (r, None))[0]
-    gms = prof.get("gemm_ms_per_step", 0.0)
+    gms = prof.get("gemm_ms_per_step", 0.0) + prof.get("bf16x3_ms_per_step", 0.0)
     ach = sym.gemmFlops / (gms / 1e3) / 1e12 if gms > 0 else 0.0
     prec = "fp64" if best.startswith("fp64") else "mixed"
     result = {
@@ -299,6 +304,17 @@ def main():
                      "launches_per_step": prof.get("gemm_launches_per_step"), "flops_per_step": sym.gemmFlops,
                      "measured_on": f"profiled step of the {prec} path (HIP events around each launch)"},
     }
+    bms, bn = prof.get("bf16x3_ms_per_step", 0.0), prof.get("bf16x3_launches_per_step", 0)
+    if best.startswith("bf16x3") and bms > 0 and bn > 0:
+        per = sym.bulkBytes / sym.bulkLaunches  # algorithmic bytes per launch
+        gbs = per / (bms / bn / 1e3) / 1e9
+        result["roofline"] = {
+            "bound": "hbm", "kernel": "k_gemm_bf16x3", "achieved": gbs, "peak": PEAK_HBM, "unit": "GB/s",
+            "frac": gbs / PEAK_HBM, "traffic": gemm_traffic("bf16x3", "gemm_bf16x3_heat3d61_hbm_traffic.txt"),
+            "bytes_per_launch": per, "launches_per_step": bn, "avg_launch_ms": bms / bn,
+            "measured_on": "profiled step of the bf16x3 path (HIP events around each launch)",
+            "products_fp32_equivalent": {"flops_per_step": sym.gemmFlops, "ms": gms, "tflops": ach,
+                                         "note": "all product launches of the step (bf16 bulk and FP32 lookahead)"}}
     # achieved HBM rates of the scatter (extend-add into tiled fronts) and of the solve pass, same profiled step
     w = 8 if prec == "fp64" else 4
     ks = prof.get("kernel_ms_profiled_step", {})

@@ -45,6 +45,10 @@ typedef struct {
   double flops, fill;
   double gemmFlops;   /* algorithmic flops of the matrix-core tile kernel per FP32 factorization (lower parts only;
                        * FP64 factors also run the panel steps on it) */
+  double bulkBytes;   /* algorithmic HBM bytes per FP32 factorization of its wide bulk trailing-update launches (the
+                       * bf16 products of AMP_F32_BF16X3): lower output entries read and written, 4 bytes each way
+                       * (written only by a front's first such update), and each split panel entry read once */
+  int32_t bulkLaunches; /* those launches per factorization */
   char patternKey[64];
 } amp_sym_info;
 
@@ -177,8 +181,9 @@ int amp_sym_set_pattern(amp_sym *sym, const int32_t *rowptr, const int32_t *coli
 int amp_download_factor(amp_num *num, double *Lx, double *D);
 
 /* Device time (ms) of the last factor and of the last solve ("solve_total", always recorded), and the total of each
- * kernel class of the last factor and solve when profiling is enabled (amp_num_set_profiling). `kernel` is a class name ("gemm_tile", "solve_fwd_flow",
- * ...) or a class name and an elimination-tree level ("gemm_tile@23") for that level's launches only. */
+ * kernel class of the last factor and solve when profiling is enabled (amp_num_set_profiling). `kernel` is a class
+ * name ("gemm_tile", "gemm_bf16x3" (the bf16 bulk products of AMP_F32_BF16X3), "split_panel", "panel_step",
+ * "solve_fwd_flow", ...) or a class name and an elimination-tree level ("gemm_tile@23") for that level's launches. */
 int amp_num_set_profiling(amp_num *num, int32_t enable);
 int amp_num_get_kernel_ms(amp_num *num, const char *kernel, double *total_ms, int32_t *launches);
 
