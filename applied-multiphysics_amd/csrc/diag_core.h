@@ -146,7 +146,16 @@ __device__ __forceinline__ void diagStore(const DiagTask& t, const FactorArgs<TS
   TS* F = a.Lx + t.f_off;
   // the pivot lane of every column belongs to the column's owner wave
   const bool owner = lane < nb && w == (lane / B) % NW;
-  if (owner && recFail != INT32_MAX) atomicMin(a.failCol, recFail);
+  if constexpr (LDLT) {
+    if (owner && recFail != INT32_MAX) atomicMin(a.failCol, recFail);
+  } else {
+    // Cholesky: the factored diagonal of column `lane` (its pivot lane's slot) is NaN where the pivot was not positive
+    TC piv = TC(1);
+#pragma unroll
+    for (int s = 0; s < J; s++)
+      if (B * (NW * (s / B) + w) + s % B == lane) piv = rA[s];
+    if (owner && !(piv > TC(0))) atomicMin(a.failCol, t.col0 + lane);
+  }
   if (LDLT && owner) {
     a.D[t.col0 + lane] = TS(recD);
     a.pert[t.col0 + lane] = recP;

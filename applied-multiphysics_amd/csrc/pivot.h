@@ -86,12 +86,12 @@ __device__ inline T pivotRule(T d, int32_t col, const FactorArgs<T>& a, T* inv, 
 // Branch-free form of pivotRule for a kernel built for one factorization kind (LDLT). Every lane evaluates the rule on
 // the broadcast pivot d; the lane with `record` keeps the column's results in its own registers (rec_d: D, rec_p:
 // perturbation flag, rec_fail: the column if the pivot fails, else unchanged) for a single store at the end of the
-// kernel, so the elimination step has no branch and no memory access.
+// kernel, so the elimination step has no branch and no memory access. Cholesky records no failure here: a pivot
+// d <= 0 (or NaN) gives the factored diagonal d * d^-1/2 = NaN, which the store checks once per column.
 template <typename T, bool LDLT>
 __device__ inline T pivotRuleNB(T d, int32_t col, T tau, T* inv, T* mult, bool record, T& rec_d, uint8_t& rec_p,
                                 int32_t& rec_fail) {
   if constexpr (!LDLT) {
-    rec_fail = (record && !(d > T(0))) ? min(rec_fail, col) : rec_fail;
     const T r = invSqrt(d);
     *inv = r;
     *mult = T(1);
