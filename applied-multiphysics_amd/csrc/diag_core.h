@@ -55,10 +55,11 @@ __device__ __forceinline__ void diagEliminate(const DiagTask& t, TC tau, int w, 
         TC inv, mult;
         const TC dk = laneValue(rA[s0 + c], k);
         const TC piv = pivotRuleNB<TC, LDLT>(dk, t.col0 + k, tau, &inv, &mult, lane == k, recD, recP, recFail);
-        const TC l = lane > k ? rA[s0 + c] * inv : TC(0);
-        gmax = fmax(gmax, fabs(LDLT ? l : l * inv));
-        if (lane == k) rA[s0 + c] = piv;
-        if (lane > k) rA[s0 + c] = l;
+        // the rows above the diagonal (lanes < k) hold values that are never stored or read again (pivots read the
+        // diagonal, updates read the rows of later columns), so the multipliers are not masked there
+        const TC l = rA[s0 + c] * inv;
+        gmax = fmax(gmax, lane > k ? fabs(LDLT ? l : l * inv) : TC(0));
+        rA[s0 + c] = lane > k ? l : piv;
         const TC wk = LDLT ? rW[s0 + c] : rW[s0 + c] * inv;
         rW[s0 + c] = wk;
         const TC cd = l * mult;
