@@ -62,7 +62,7 @@ class FactorInfo(C.Structure):
 class TuneInfo(C.Structure):
     _fields_ = [("smallMax", C.c_int32), ("smallNarrow", C.c_int32), ("wideLaunch", C.c_int32),
                 ("superPanel", C.c_int32), ("flowWideMin", C.c_int32), ("packMin", C.c_int32),
-                ("fusedPanel", C.c_int32),
+                ("fusedPanel", C.c_int32), ("fusedSolve", C.c_int32),
                 ("candidates", C.c_int32),
                 ("defaultMs", C.c_double), ("bestMs", C.c_double)]
 

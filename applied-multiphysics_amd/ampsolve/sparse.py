@@ -118,14 +118,15 @@ class Symbolic:
         return {f: getattr(t, f) for f, _ in A.TuneInfo._fields_}
 
     def set_params(self, params):
-        """Apply shape thresholds (a dict as autotune returns; flowWideMin defaults to 160, packMin to 512, fusedPanel to 1) and rebuild
-        the plans (amp_sym_set_params)."""
+        """Apply shape thresholds (a dict as autotune returns; flowWideMin defaults to 160, packMin to 512, fusedPanel to 1,
+        fusedSolve to 3) and rebuild the plans (amp_sym_set_params)."""
         t = A.TuneInfo()
         for f in ("smallMax", "smallNarrow", "wideLaunch", "superPanel"):
             setattr(t, f, int(params[f]))
         t.flowWideMin = int(params.get("flowWideMin", 160))
         t.packMin = int(params.get("packMin", 512))
         t.fusedPanel = int(params.get("fusedPanel", 1))
+        t.fusedSolve = int(params.get("fusedSolve", 3))
         A.check(A.lib().amp_sym_set_params(self._h, C.byref(t)))
         self._refresh()
 

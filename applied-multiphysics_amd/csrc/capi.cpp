@@ -339,7 +339,8 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
   total.second = 1;
   if (prof) {
     static const char* names[] = {"solve_small_fwd", "solve_gather", "solve_fwd_flow", "solve_small_bwd",
-                                  "solve_bwd_flow", "solve_bwd_gather", "solve_bwd_init"};
+                                  "solve_bwd_flow", "solve_bwd_gather", "solve_bwd_init", "solve_fwd_fused",
+                                  "solve_bwd_fused"};
     for (size_t i = 0; i < prof->used; i++) {
       float k = 0;
       HIP_TRY(hipEventElapsedTime(&k, prof->events[2 * i], prof->events[2 * i + 1]));
@@ -624,9 +625,10 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
   {
     const amp::SolvePlan& SP = h->SP;
     int32_t nflow = 0;
-    for (const amp::SolveLaunch& L : SP.launches) nflow += L.kind == amp::kSolveFwdFlow || L.kind == amp::kSolveBwdFlow;
-    // flags of both directions, error flag, pass number, task counters of the flow launches
-    const size_t bytes = size_t(2 * SP.flowFlags + 2 + nflow) * 4;
+    for (const amp::SolveLaunch& L : SP.launches) nflow += amp::solveKindIsFlow(L.kind);
+    // flags of both directions, error flag, pass number, task counters of the flow and fused launches, counters of
+    // the fused launches
+    const size_t bytes = size_t(2 * SP.flowFlags + 2 + nflow + SP.doneSlots) * 4;
     HIP_TRY(num->flow.alloc(bytes));
     HIP_TRY(hipMemset(num->flow.p, 0, bytes));
     int32_t* w = num->flow.as<int32_t>();
@@ -636,6 +638,8 @@ int amp_num_create(amp_sym* h, int32_t mode, int32_t precision, amp_num** out) {
     num->fc.epoch = w + 2 * SP.flowFlags + 1;
     num->fc.ctr = w + 2 * SP.flowFlags + 2;
     num->fc.nctr = nflow;
+    num->fc.done = num->fc.ctr + nflow;
+    num->fc.ndone = int32_t(SP.doneSlots);
     // solved-value mailboxes of both directions (pass number 0 in every word: never current)
     const size_t boxBytes = size_t(std::max<int64_t>(SP.flowFlags, 1)) * amp::kBoxWords * 8;
     HIP_TRY(num->boxes.alloc(2 * boxBytes));
@@ -896,7 +900,8 @@ int rebuildPlans(amp_sym* h, const amp::PlanParams& p) {
 bool validParams(const amp::PlanParams& p) {
   return p.smallMax >= 1 && p.smallMax <= amp::kSmallMax && p.smallNarrow >= 1 && p.smallNarrow <= p.smallMax &&
          p.wideLaunch >= 1 && p.superPanel >= amp::kTileEdge && p.superPanel % amp::kTileEdge == 0 &&
-         p.flowWideMin >= 0 && p.packMin >= 0 && (p.fusedPanel == 0 || p.fusedPanel == 1);
+         p.flowWideMin >= 0 && p.packMin >= 0 && (p.fusedPanel == 0 || p.fusedPanel == 1) &&
+         p.fusedSolve >= 0 && p.fusedSolve <= 3;
 }
 
 // factor + one solve (zero right-hand side), device milliseconds of the second of two runs
@@ -934,6 +939,7 @@ int amp_sym_set_params(amp_sym* h, const amp_tune_info* t) {
   p.flowWideMin = t->flowWideMin;
   p.packMin = t->packMin;
   p.fusedPanel = t->fusedPanel;
+  p.fusedSolve = t->fusedSolve;
   if (!validParams(p)) return fail(AMP_ERR_ARG, "amp_sym_set_params: thresholds out of range");
   return rebuildPlans(h, p);
 }
@@ -1007,6 +1013,13 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     p.fusedPanel = 0;
     if ((rc = attempt(p))) return rc;
   }
+  {  // the solve's runs of tiled levels as two launches per level and direction instead of one per run
+    amp::PlanParams p = best;
+    for (int32_t v = 0; v < 4; v++) {
+      p.fusedSolve = v;
+      if (v != best.fusedSolve && (rc = attempt(p))) return rc;
+    }
+  }
   rc = rebuildPlans(h, best);
   if (rc) return rc;
   if (info) {
@@ -1017,6 +1030,7 @@ int amp_sym_autotune(amp_sym* h, const double* d_values, int32_t mode, int32_t p
     info->flowWideMin = best.flowWideMin;
     info->packMin = best.packMin;
     info->fusedPanel = best.fusedPanel;
+    info->fusedSolve = best.fusedSolve;
     info->candidates = tried;
     info->defaultMs = defaultMs;
     info->bestMs = bestMs;

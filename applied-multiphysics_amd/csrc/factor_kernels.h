@@ -87,6 +87,9 @@ struct FlowCtl {
   uint64_t* bwdBox;
   int32_t* ctr;  // task counters of the flow launches of a pass (cleared by the kernel that raises the pass number)
   int32_t nctr;
+  int32_t* done;  // counters of the fused launches (SolvePlan::doneSlots, stored after the task counters, cleared
+                  // with them)
+  int32_t ndone;
 };
 constexpr int kBoxWords = 2 * kFlowRhs * 64;  // mailbox words per flag (FP64 values)
 

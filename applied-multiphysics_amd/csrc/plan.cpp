@@ -512,13 +512,102 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F, const PlanParam
     list(kFlowQuadsF64, sl.first2, sl.count2, sl.nq2);
     if (sl.count > 0) P.launches.push_back(sl);
   };
+  std::vector<std::vector<int32_t>> bigOf(F.nlevels);
+  for (int32_t L = 0; L < F.nlevels; L++)
+    for (int32_t s : byLevel[L])
+      if (F.isBig[s]) bigOf[L].push_back(s);
+  // fused runs: a level joins one when its flow launches would take one block per task in both precisions (fewer than
+  // flowWideMin tasks of kFlowQuadsF64 blocks, hence of kFlowQuadsF32) and its fronts have at most
+  // kFusedSolveMaxBlocks blocks; every level of a run after the first holds
+  // tiled supernodes only, so no whole-front launch has to run between two of its levels (the first level's whole
+  // fronts run before the forward launch and after the backward one). runEnd[L]: last level of the run starting at L
+  std::vector<int32_t> runEnd(F.nlevels, -1), runStart(F.nlevels, -1);
+  if (prm.fusedSolve & 3) {
+    std::vector<int8_t> narrow(F.nlevels, 0), pure(F.nlevels, 0);
+    for (int32_t L = 0; L < F.nlevels; L++) {
+      int64_t wide = 0;
+      int32_t maxBlocks = 0;
+      for (int32_t s : bigOf[L]) {
+        maxBlocks = std::max(maxBlocks, nblocks(s));
+        const int32_t mu = frontRows(S, s) - frontCols(S, s);
+        wide += (nblocks(s) + kFlowQuadsF64 - 1) / kFlowQuadsF64 + (mu + kFlowQuadsF64 * kTileEdge - 1) / (kFlowQuadsF64 * kTileEdge);
+      }
+      narrow[L] = !bigOf[L].empty() && wide < prm.flowWideMin && maxBlocks <= kFusedSolveMaxBlocks;
+      pure[L] = bigOf[L].size() == byLevel[L].size();
+    }
+    for (int32_t L = 0; L < F.nlevels;) {
+      int32_t e = L;
+      if (narrow[L])
+        while (e + 1 < F.nlevels && narrow[e + 1] && pure[e + 1]) e++;
+      if (e > L) {
+        runEnd[L] = e;
+        runStart[e] = L;
+      }
+      L = e + 1;
+    }
+  }
+  auto parentOf = [&S](int32_t s) { return S.superParent[s]; };
+  // a fused launch over `tasks` (the same list serves both precisions: one block per task)
+  auto pushFused = [&](SolveKind kind, int32_t level, int64_t first) {
+    SolveLaunch sl{kind, level, first, int64_t(P.blocks.size()) - first, 0};
+    sl.first2 = sl.first;
+    sl.count2 = sl.count;
+    if (sl.count > 0) P.launches.push_back(sl);
+  };
   // forward, leaves first
   for (int32_t L = 0; L < F.nlevels; L++) {
-    std::vector<int32_t> big;
-    for (int32_t s : byLevel[L])
-      if (F.isBig[s]) big.push_back(s);
+    const std::vector<int32_t>& big = bigOf[L];
     pushSmall(kSolveSmallFwd, L);
     if (big.empty()) continue;
+    if (runEnd[L] >= 0 && (prm.fusedSolve & 1)) {
+      // levels L .. runEnd[L] in one launch; per supernode two counters: below tasks of its children done (slot) and
+      // its gather tasks done (slot + 1)
+      const int32_t Le = runEnd[L];
+      std::vector<int32_t> slot(ns, -1);
+      for (int32_t l = L; l <= Le; l++)
+        for (int32_t s : bigOf[l]) {
+          slot[s] = int32_t(P.doneSlots);
+          P.doneSlots += 2;
+        }
+      auto belowTasks = [&](int32_t s) { return (frontRows(S, s) - frontCols(S, s) + kTileEdge - 1) / kTileEdge; };
+      const int64_t first = int64_t(P.blocks.size());
+      for (int32_t l = L; l <= Le; l++) {
+        for (int32_t s : bigOf[l]) {
+          int32_t fromChildren = 0;
+          for (int32_t c = S.childHead[s]; c != -1; c = S.childNext[c])
+            if (slot[c] >= 0) fromChildren += belowTasks(c);
+          const int32_t m = frontRows(S, s);
+          for (int32_t r = 0; r < m; r += kGatherRows) {
+            BlockTask t{s, r / kGatherRows, r, std::min(m, r + kGatherRows), -1, flag0[s], 0, 0};
+            t.wslot = fromChildren > 0 ? slot[s] : -1;
+            t.wcount = fromChildren;
+            t.dslot = slot[s] + 1;
+            P.blocks.push_back(t);
+          }
+        }
+        int32_t maxBlk = 0;
+        for (int32_t s : bigOf[l]) maxBlk = std::max(maxBlk, nblocks(s));
+        auto flowTask = [&](BlockTask t) {
+          t.wslot = slot[t.s] + 1;
+          t.wcount = (frontRows(S, t.s) + kGatherRows - 1) / kGatherRows;
+          P.blocks.push_back(t);
+          return &P.blocks.back();
+        };
+        for (int32_t t = 0; t < maxBlk; t++)
+          for (int32_t s : bigOf[l])
+            if (t < nblocks(s))
+              flowTask(BlockTask{s, t, t * kTileEdge, std::min(frontCols(S, s), (t + 1) * kTileEdge), 1, flag0[s], 0, 0});
+        for (int32_t s : bigOf[l]) {
+          const int32_t nc = frontCols(S, s), m = frontRows(S, s), par = parentOf(s);
+          for (int32_t r = nc; r < m; r += kTileEdge)
+            flowTask(BlockTask{s, nblocks(s), r, std::min(m, r + kTileEdge), 0, flag0[s], 0, 0})->dslot =
+                par >= 0 && slot[par] >= 0 ? slot[par] : -1;
+        }
+      }
+      pushFused(kSolveFwdFused, L, first);
+      L = Le;
+      continue;
+    }
     int64_t first = int64_t(P.fronts.size());
     P.fronts.insert(P.fronts.end(), big.begin(), big.end());
     int32_t maxM = 0;
@@ -544,25 +633,65 @@ SolvePlan buildSolvePlan(const Symbolic& S, const FactorPlan& F, const PlanParam
     });
   }
   // backward, root first; blocks of each supernode from the last one down
+  // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk); y[below] is read through
+  // the row structure, so no gather precedes them
+  std::vector<int32_t> p0(ns, 0), np(ns, 0);
+  auto initTasks = [&](int32_t s, int32_t wslot, int32_t wcount, int32_t dslot) {
+    const int32_t nc = frontCols(S, s), mu = frontRows(S, s) - nc;
+    np[s] = (mu + kInitRows - 1) / kInitRows;
+    p0[s] = int32_t(P.initSlots);
+    for (int32_t b = 0; b < nblocks(s); b++)
+      for (int32_t c = 0; c < np[s]; c++) {
+        BlockTask t{s, b, c * kInitRows, std::min(mu, (c + 1) * kInitRows), 0, flag0[s], int32_t(P.initSlots++), 1};
+        t.wslot = wslot;
+        t.wcount = wcount;
+        t.dslot = dslot;
+        P.blocks.push_back(t);
+      }
+  };
   for (int32_t L = F.nlevels - 1; L >= 0; L--) {
-    std::vector<int32_t> big;
-    for (int32_t s : byLevel[L])
-      if (F.isBig[s]) big.push_back(s);
+    const std::vector<int32_t>& big = bigOf[L];
+    if (runStart[L] >= 0 && (prm.fusedSolve & 2)) {
+      // levels L down to runStart[L] in one launch; per supernode two counters: its initialisation tasks done (slot)
+      // and its chain tasks done (slot + 1)
+      const int32_t Ls = runStart[L];
+      std::vector<int32_t> slot(ns, -1);
+      for (int32_t l = L; l >= Ls; l--)
+        for (int32_t s : bigOf[l]) {
+          slot[s] = int32_t(P.doneSlots);
+          P.doneSlots += 2;
+        }
+      const int64_t first = int64_t(P.blocks.size());
+      for (int32_t l = L; l >= Ls; l--) {
+        for (int32_t s : bigOf[l]) {
+          const int32_t par = parentOf(s);
+          const bool waits = par >= 0 && slot[par] >= 0;
+          initTasks(s, waits ? slot[par] + 1 : -1, waits ? nblocks(par) : 0, slot[s]);
+        }
+        int32_t maxBlk = 0;
+        for (int32_t s : bigOf[l]) maxBlk = std::max(maxBlk, nblocks(s));
+        for (int32_t i = 0; i < maxBlk; i++)
+          for (int32_t s : bigOf[l]) {
+            const int32_t nc = frontCols(S, s), nb = nblocks(s);
+            if (i >= nb) continue;
+            const int32_t lo = nb - 1 - i;
+            BlockTask t{s, lo, lo * kTileEdge, std::min(nc, (lo + 1) * kTileEdge), 1, flag0[s], p0[s] + lo * np[s],
+                        np[s]};
+            t.wslot = np[s] > 0 ? slot[s] : -1;
+            t.wcount = nb * np[s];
+            t.dslot = slot[s] + 1;
+            P.blocks.push_back(t);
+          }
+      }
+      pushFused(kSolveBwdFused, L, first);
+      pushSmall(kSolveSmallBwd, Ls);  // they read the solutions of the run's supernodes
+      L = Ls;
+      continue;
+    }
     pushSmall(kSolveSmallBwd, L);
     if (big.empty()) continue;
-    // partial products -L21[chunk, block]^T y[below chunk], one slot per (block, row chunk); y[below] is read through
-    // the row structure, so no gather launch precedes them
-    std::vector<int32_t> p0(ns, 0), np(ns, 0);
     const int64_t bf = int64_t(P.blocks.size());
-    for (int32_t s : big) {
-      const int32_t nc = frontCols(S, s), mu = frontRows(S, s) - nc;
-      np[s] = (mu + kInitRows - 1) / kInitRows;
-      p0[s] = int32_t(P.initSlots);
-      for (int32_t b = 0; b < nblocks(s); b++)
-        for (int32_t c = 0; c < np[s]; c++)
-          P.blocks.push_back(BlockTask{s, b, c * kInitRows, std::min(mu, (c + 1) * kInitRows), 0, flag0[s],
-                                       int32_t(P.initSlots++), 1});
-    }
+    for (int32_t s : big) initTasks(s, -1, 0, -1);
     push(kSolveBwdInit, L, bf, int64_t(P.blocks.size()) - bf);
     int32_t maxBlk = 0;
     for (int32_t s : big) maxBlk = std::max(maxBlk, nblocks(s));

@@ -182,6 +182,9 @@ struct PlanParams {
   int32_t fusedPanel = 1;  // a panel step of tiled fronts is one kPanel launch (else diagonal blocks, panel products
                            // and the super-panel's update as three launches); super-panels of at most kSuperPanel
                            // columns only
+  int32_t fusedSolve = 3;  // runs of consecutive levels of tiled fronts are solved by one launch per direction (bit 0:
+                           // forward, bit 1: backward; else a gather or initialisation launch and a flow launch per
+                           // level)
 };
 
 FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams());
@@ -196,6 +199,12 @@ FactorPlan buildFactorPlan(const Symbolic& S, const PlanParams& prm = PlanParams
 // inside one launch.
 constexpr int kFlowRhs = 8;       // right-hand sides per pass of the tiled-supernode solve kernels
 constexpr int kSolveWaveMax = 64;  // whole-front supernodes with m <= this are solved by one wave each
+constexpr int kGatherRows = 512;
+// Largest front (64-column blocks) of a level that a fused solve launch takes: a fused launch lets a level's gather or
+// initialisation tasks stream beside the chains of the level above, which slows the longest chains; with fronts of
+// up to 16 blocks (3D heat, 30^3 nodes) one launch per direction still wins (0.99 against 1.03 ms per pass), with
+// the 59-block root of 61^3 nodes it loses (3.67 against 3.03 ms).
+constexpr int kFusedSolveMaxBlocks = 16;  // front-vector rows per gather workgroup or task of the forward solve
 constexpr int kInitRows = 256;    // below rows per task of the backward z = -L21^T y[below] partial products
 
 // Blocks per task of the tiled-supernode solve launches: a task has one 256-thread quarter per 64-row (forward) or
@@ -215,11 +224,19 @@ constexpr int kFlowQuadsF32 = 4, kFlowQuadsF64 = 2;
 // blk + w .. nb - 1 and, for its block blk + i, the np partial products at slots p0 + i np .. p0 + i np + np - 1,
 // then solves its blocks from the last one down. Backward initialisation: partial product of column block blk over
 // below rows [r0, r1) into slot p0.
+// Fused launches (kSolveFwdFused, kSolveBwdFused) hold the tasks of several levels, one block per chain task, and
+// also the level's gather tasks (forward, writer = -1: rows [r0, r1) of the front vector of s) or initialisation tasks
+// (backward, writer = 0). A task that reads what other tasks of the launch stored first waits until counter `wslot`
+// of the pass reaches `wcount`, and a task whose stores others read raises counter `dslot` when it is done (-1: none):
+// the gather tasks of s wait for the below tasks of its children, the chain and below tasks of s for the gather tasks
+// of s; backward, the initialisation tasks of s wait for the chain tasks of its parent, the chain tasks of s for the
+// initialisation tasks of s.
 struct BlockTask {
   int32_t s, blk;
   int32_t r0, r1;
   int32_t writer, flag0;
   int32_t p0, np;
+  int32_t wslot = -1, wcount = 0, dslot = -1;
 };
 
 enum SolveKind : int32_t {
@@ -231,7 +248,16 @@ enum SolveKind : int32_t {
   kSolveBwdGather = 5,  // not emitted (the backward partial products read y[below] through the row structure); the
                         // number stays reserved so that the kinds' numbers in profiles are unchanged
   kSolveBwdInit = 6,
+  kSolveFwdFused = 7,  // gather and forward flow tasks of a run of levels (task lists of BlockTask, one block each)
+  kSolveBwdFused = 8,  // initialisation and backward flow tasks of a run of levels
 };
+
+inline bool solveKindIsForward(SolveKind k) {
+  return k == kSolveSmallFwd || k == kSolveGather || k == kSolveFwdFlow || k == kSolveFwdFused;
+}
+inline bool solveKindIsFlow(SolveKind k) {
+  return k == kSolveFwdFlow || k == kSolveBwdFlow || k == kSolveFwdFused || k == kSolveBwdFused;
+}
 
 struct SolveLaunch {
   SolveKind kind;
@@ -249,6 +275,7 @@ struct SolvePlan {
   int64_t fvEntries = 0;  // Sp[nsuper]
   int64_t flowFlags = 0;  // 64-column blocks of all tiled supernodes (one flag each per direction)
   int64_t initSlots = 0;  // 64-entry partial products of the backward initialisation
+  int64_t doneSlots = 0;  // counters of the fused launches (BlockTask::wslot / dslot)
   std::vector<int32_t> fronts;
   std::vector<BlockTask> blocks;
   std::vector<SolveLaunch> launches;

@@ -16,7 +16,7 @@ __global__ void k_clear_flag(int32_t* flag) {
 }
 
 // Raise the pass number of the tiled-supernode solve flags (wrapping to 1; flags start at 0) and clear the task
-// counters of the pass's flow launches.
+// counters of the pass's flow launches and the counters of its fused launches (stored after them).
 __global__ void k_epoch_bump(int32_t* epoch, int32_t* ctr, int32_t nctr) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
   for (int32_t i = threadIdx.x; i < nctr; i += blockDim.x) ctr[i] = 0;
@@ -271,22 +271,45 @@ __global__ __launch_bounds__(256) void k_solve_seg_fwd(const int32_t* __restrict
   }
 }
 
+// Values that tasks of one fused launch hand to each other (front-vector rows, partial products, backward solutions)
+// are stored and loaded with device-coherent accesses (written through to the point of coherence, never served from
+// a non-coherent cache), so a hand-over needs no cache-wide write-back or invalidation, only the completion of the
+// stores before the producer's count (CO = true); CO = false: ordinary accesses, for values handed over between
+// launches.
+template <bool CO, typename S>
+__device__ inline S ldv(const S* p) {
+  if constexpr (!CO) {
+    return *p;
+  } else if constexpr (sizeof(S) == 4) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<uint32_t*>(const_cast<S*>(p)), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT));
+  } else {
+    return __longlong_as_double(int64_t(__hip_atomic_load(reinterpret_cast<uint64_t*>(const_cast<S*>(p)),
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  }
+}
+template <bool CO, typename S>
+__device__ inline void stv(S* p, S v) {
+  if constexpr (!CO) {
+    *p = v;
+  } else if constexpr (sizeof(S) == 4) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), uint64_t(__double_as_longlong(v)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Front vectors of the tiled supernodes of a level, rows [kGatherRows y, kGatherRows (y + 1)) of front fronts[x] per
 // workgroup (x, y), so that the large fronts near the root are gathered by many workgroups. Every workgroup walks each
 // child's whole relative map once for all right-hand sides and keeps the rows that land in its slice (the map
 // increases, so a lane stops at the first row past the slice); per entry the order is the single-workgroup one: own
-// value or zero, then the children in order.
-constexpr int kGatherRows = 512;
-
-template <typename T, typename S, int Q>
-__global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y, S* fv,
-                                                      int32_t q, int64_t fvN) {
-  const int32_t s = fronts[blockIdx.x];
-  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f, m = a.Sp[s + 1] - a.Sp[s];
-  const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(m, i0 + kGatherRows);
-  if (i0 >= m) return;  // whole workgroup
+// value or zero, then the children in order. gatherSlice: rows [i0, i1) of supernode s (256 threads).
+template <typename T, typename S, int Q, bool CO = false>
+__device__ __forceinline__ void gatherSlice(int32_t s, int32_t i0, int32_t i1, const SolveArgs<T>& a, const S* y,
+                                            S* fv, int32_t q, int64_t fvN, S (&vs)[Q][kGatherRows]) {
+  const int32_t f = a.superStart[s], nc = a.superStart[s + 1] - f;
   // the slice of v is accumulated in LDS (children in order, one barrier each) and stored once
-  __shared__ S vs[Q][kGatherRows];
   for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
 #pragma unroll
     for (int k = 0; k < Q; k++)
@@ -306,7 +329,7 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
         const int32_t r = r0 + u * int32_t(blockDim.x);
         iv[u] = r < muc ? a.rel[o + r] : INT32_MAX;
 #pragma unroll
-        for (int k = 0; k < Q; k++) vv[u][k] = (r < muc && k < q) ? fv[k * fvN + o + r] : S(0);
+        for (int k = 0; k < Q; k++) vv[u][k] = (r < muc && k < q) ? ldv<CO>(fv + k * fvN + o + r) : S(0);
       }
       bool past = false;
 #pragma unroll
@@ -324,7 +347,18 @@ __global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict_
   for (int32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
 #pragma unroll
     for (int k = 0; k < Q; k++)
-      if (k < q) fv[k * fvN + a.Sp[s] + i] = vs[k][i - i0];
+      if (k < q) stv<CO>(fv + k * fvN + a.Sp[s] + i, vs[k][i - i0]);
+}
+
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_gather(const int32_t* __restrict__ fronts, SolveArgs<T> a, S* y, S* fv,
+                                                      int32_t q, int64_t fvN) {
+  const int32_t s = fronts[blockIdx.x];
+  const int32_t m = a.Sp[s + 1] - a.Sp[s];
+  const int32_t i0 = int32_t(blockIdx.y) * kGatherRows, i1 = min(m, i0 + kGatherRows);
+  if (i0 >= m) return;  // whole workgroup
+  __shared__ S vs[Q][kGatherRows];
+  gatherSlice<T, S, Q>(s, i0, i1, a, y, fv, q, fvN, vs);
 }
 
 namespace {
@@ -381,6 +415,40 @@ __device__ inline uint64_t boxWord(const uint64_t* p, int32_t want, int32_t* err
   }
   return w;
 }
+// Counters of the fused launches (cleared by the kernel that raises the pass number). The values a counter guards are
+// stored and loaded device-coherently (ldv / stv with CO), so the counters need no fences: raiseCount adds one to
+// counter `slot` once every wave has waited for its stores to complete and the workgroup has met at a barrier;
+// waitCount: thread 0 polls counter `slot` until it reaches `count` (bounded like boxWord), and the workgroup barrier
+// holds every wave's loads until then. A slot below 0: nothing.
+__device__ inline void waitCount(const FlowCtl& c, int32_t slot, int32_t count) {
+  if (slot < 0) return;
+  if (threadIdx.x == 0) {
+    // the waits of a launch's later levels last long and are many: their polls back off, so that they do not crowd
+    // the device-coherent traffic of the running chains
+    for (uint32_t spins = 0; __hip_atomic_load(c.done + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < count;) {
+      if (++spins >= kFlowSpinLimit ||
+          ((spins & 255) == 0 && __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+        atomicOr(c.err, 1);
+        break;
+      }
+      if (spins < 16)
+        __builtin_amdgcn_s_sleep(1);
+      else if (spins < 64)
+        __builtin_amdgcn_s_sleep(4);
+      else
+        __builtin_amdgcn_s_sleep(16);
+    }
+  }
+  __syncthreads();
+}
+
+__device__ inline void raiseCount(const FlowCtl& c, int32_t slot) {
+  if (slot < 0) return;
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed (the barrier alone waits for LDS only)
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(c.done + slot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename S>
 __device__ inline S boxLoad(const uint64_t* box, int64_t slot, int32_t want, int32_t* err);
 template <>
@@ -405,7 +473,7 @@ __device__ inline double boxLoad<double>(const uint64_t* box, int64_t slot, int3
 // diagonal block (x = W v), publishes x (y, mailbox, flag) and hands it to the later quarters through LDS, which apply
 // it to their rows before the next quarter's turn: inside a task, a block passes to the next without a round trip
 // through device memory. Q is the number of right-hand sides the kernel is built for (q <= Q).
-template <typename T, typename S, int Q, int NQ>
+template <typename T, typename S, int Q, int NQ, bool CO = false>
 __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, S* fv, int32_t q,
                                         int64_t fvN, const FlowCtl& c, int32_t want, S (&xs)[Q][kFlowRun * kTileEdge],
                                         S (&vh)[Q][kTileEdge]) {
@@ -430,7 +498,7 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
   for (int jj = 0; jj < Q; jj++) {
     acc[jj] = S(0);
-    vown[jj] = (cp == 0 && live && jj < q) ? fv[jj * fvN + a.Sp[s] + r] : S(0);
+    vown[jj] = (cp == 0 && live && jj < q) ? ldv<CO>(fv + jj * fvN + a.Sp[s] + r) : S(0);
   }
   // blocks the row consumes: those before blk (all of them for a below task), then in a chain task the task's own
   // blocks before the quarter's
@@ -463,7 +531,7 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
   if (total > 0) loadL(cur, 0);
   if (local0) {
     if (tid < E)
-      for (int jj = 0; jj < q; jj++) vh[jj][tid] = tid < nc ? fv[jj * fvN + a.Sp[s] + tid] : S(0);
+      for (int jj = 0; jj < q; jj++) vh[jj][tid] = tid < nc ? ldv<CO>(fv + jj * fvN + a.Sp[s] + tid) : S(0);
     __syncthreads();
     if (qd == 0) {
 #pragma unroll
@@ -504,7 +572,7 @@ __device__ __forceinline__ void fwdTask(const BlockTask t, const SolveArgs<T>& a
 #pragma unroll
     for (int jj = 0; jj < Q; jj++) {
       const S v = rowTotal(acc[jj]);
-      if (jj < q && cp == 0 && live) fv[jj * fvN + a.Sp[s] + r] = vown[jj] - v;
+      if (jj < q && cp == 0 && live) stv<CO>(fv + jj * fvN + a.Sp[s] + r, vown[jj] - v);
     }
     return;
   }
@@ -784,14 +852,12 @@ __device__ inline S waveSumSpread(S (&a)[V]) {
 // reduced across the wave together (waveSumSpread): lane l accumulates value v = c Q + jj (column c, right-hand side
 // jj) in slot v ^ spreadMask(l), which it arranges by loading column c ^ (mask / Q) into slot c and right-hand side
 // jj ^ (mask % Q) into slot jj.
-template <typename T, typename S, int Q>
-__global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a,
-                                                        const S* y, S* pz, int32_t q) {
+template <typename T, typename S, int Q, bool CO = false>
+__device__ __forceinline__ void initTask(const BlockTask t, const SolveArgs<T>& a, const S* y, S* pz, int32_t q) {
   constexpr int E = kTileEdge, NR = kInitRows / E;
   constexpr int CG = (Q > 1 && sizeof(S) == 8) ? 2 : 4;  // columns per group (registers: CG Q sums per lane)
   constexpr int V = CG * Q;
   static_assert((Q & (Q - 1)) == 0, "right-hand sides per pass: a power of two");
-  const BlockTask t = tasks[blockIdx.x];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int mask = spreadMask<V>(lane & (V - 1)), mc = mask / Q, mj = mask % Q;
   const int32_t s = t.s, nc = a.superStart[s + 1] - a.superStart[s], m = a.Sp[s + 1] - a.Sp[s];
@@ -805,7 +871,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restr
 #pragma unroll
     for (int js = 0; js < Q; js++) {
       const int jj = js ^ mj;
-      yv[u][js] = (i < t.r1 && jj < q) ? y[int64_t(jj) * a.n + yi] : S(0);
+      yv[u][js] = (i < t.r1 && jj < q) ? ldv<CO>(y + int64_t(jj) * a.n + yi) : S(0);
     }
   }
 #pragma unroll 1
@@ -826,8 +892,14 @@ __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restr
       }
     }
     const S sum = waveSumSpread<V>(acc);  // lane l < V: value `mask` (column mask / Q, right-hand side mask % Q)
-    if (lane < V && mj < q) pz[(int64_t(t.p0) * q + mj) * E + cj0 + mc] = sum;
+    if (lane < V && mj < q) stv<CO>(pz + (int64_t(t.p0) * q + mj) * E + cj0 + mc, sum);
   }
+}
+
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restrict__ tasks, SolveArgs<T> a,
+                                                        const S* y, S* pz, int32_t q) {
+  initTask<T, S, Q>(tasks[blockIdx.x], a, y, pz, q);
 }
 
 // Backward solve of the tiled supernodes of one level. A task has NQ quarters of 256 threads, quarter qd owning column
@@ -838,7 +910,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_init(const BlockTask* __restr
 // time, L loaded one block ahead). The task's blocks are then solved from the last one down: quarter i forms z,
 // applies its block's transposed inverted diagonal block (x = W^T z), publishes x and hands it to the earlier quarters
 // through LDS, which apply it to their columns before the next quarter's turn.
-template <typename T, typename S, int Q, int NQ>
+template <typename T, typename S, int Q, int NQ, bool CO = false>
 __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a, S* y, const S* pz, int32_t q,
                                         const FlowCtl& c, int32_t want, S (&xs)[Q][kFlowRun * kTileEdge],
                                         S (&zs)[Q][kTileEdge]) {
@@ -862,7 +934,7 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
   for (int jj = 0; jj < Q; jj++) {
     S below = S(0);
     if (own && jj < q && rs == 0)
-      for (int32_t p = 0; p < t.np; p++) below += pz[(int64_t(t.p0 + qd * t.np + p) * q + jj) * E + cj];
+      for (int32_t p = 0; p < t.np; p++) below += ldv<CO>(pz + (int64_t(t.p0 + qd * t.np + p) * q + jj) * E + cj);
     acc[jj] = below;
     yown[jj] = (jj < q && rs == 0 && live) ? y[int64_t(jj) * a.n + f + col] : S(0);
   }
@@ -937,7 +1009,7 @@ __device__ __forceinline__ void bwdTask(const BlockTask t, const SolveArgs<T>& a
         if (rs == 0) {
           xs[jj][cj] = (jj < q && live) ? sum : S(0);  // hand-over to the earlier quarters
           if (jj < q && live) {
-            y[int64_t(jj) * a.n + f + col] = sum;
+            stv<CO>(y + int64_t(jj) * a.n + f + col, sum);
             if (kb > 0) boxStore(c.bwdBox, boxSlot(t.flag0 + kb, jj, cj), want, sum);
           }
         }
@@ -966,6 +1038,47 @@ __global__ __launch_bounds__(256 * NQ) void k_solve_bwd_flow(const BlockTask* __
   const int32_t want = *c.epoch;
   for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr))
     bwdTask<T, S, Q, NQ>(tasks[i], a, y, pz, q, c, want, xs, zs);
+}
+
+// Fused launches: the tasks of a run of levels in one launch (one block per chain task), with the gather tasks
+// (forward, writer < 0) or initialisation tasks (backward, writer = 0) of each level among them. A task first waits for
+// the counter of the tasks whose stores it reads (BlockTask::wslot) and raises its own when done (dslot); every task
+// waits only on tasks listed before it, so the launch drains as the flow launches do.
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_fwd_fused(const BlockTask* __restrict__ tasks, int32_t ntasks,
+                                                         int32_t* ctr, SolveArgs<T> a, S* y, S* fv, int32_t q,
+                                                         int64_t fvN, FlowCtl c) {
+  __shared__ S xs[Q][kFlowRun * kTileEdge];
+  __shared__ S vh[Q][kTileEdge];
+  __shared__ S vs[Q][kGatherRows];
+  const int32_t want = *c.epoch;
+  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr)) {
+    const BlockTask t = tasks[i];
+    waitCount(c, t.wslot, t.wcount);
+    if (t.writer < 0)
+      gatherSlice<T, S, Q, true>(t.s, t.r0, t.r1, a, y, fv, q, fvN, vs);
+    else
+      fwdTask<T, S, Q, 1, true>(t, a, y, fv, q, fvN, c, want, xs, vh);
+    raiseCount(c, t.dslot);
+  }
+}
+
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(256) void k_solve_bwd_fused(const BlockTask* __restrict__ tasks, int32_t ntasks,
+                                                         int32_t* ctr, SolveArgs<T> a, S* y, S* pz, int32_t q,
+                                                         FlowCtl c) {
+  __shared__ S xs[Q][kFlowRun * kTileEdge];
+  __shared__ S zs[Q][kTileEdge];
+  const int32_t want = *c.epoch;
+  for (int32_t i = nextTask(ctr); i < ntasks; i = nextTask(ctr)) {
+    const BlockTask t = tasks[i];
+    waitCount(c, t.wslot, t.wcount);
+    if (t.writer == 0)
+      initTask<T, S, Q, true>(t, a, y, pz, q);
+    else
+      bwdTask<T, S, Q, 1, true>(t, a, y, pz, q, c, want, xs, zs);
+    raiseCount(c, t.dslot);
+  }
 }
 
 namespace {
@@ -1009,7 +1122,7 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
   hipError_t err = hipSuccess;
   int32_t nflow = 0;  // flow launches so far: index of the next one's task counter
   for (const SolveLaunch& L : dp.launches) {
-    if (a.ldlt && !scaled && L.kind >= kSolveSmallBwd) {
+    if (a.ldlt && !scaled && !solveKindIsForward(L.kind)) {
       hipLaunchKernelGGL((k_diag_scale<T, S>), dim3(g), dim3(256), 0, st, y, a.D, n, q);
       scaled = true;
     }
@@ -1109,6 +1222,21 @@ hipError_t solvePass(const DeviceSolvePlan& dp, const SolveArgs<T>& a, const dou
           hipLaunchKernelGGL((k_solve_bwd_init<T, S, kFlowRhs>), dim3(grid), dim3(256), 0, st, dp.blocks + L.first, a,
                              y, pz, q);
         break;
+      case kSolveFwdFused: {
+        int32_t* ctr = fc.ctr + nflow++;
+        const BlockTask* tk = dp.blocks + L.first;
+        err = q == 1 ? launchFlow<1>(k_solve_fwd_fused<T, S, 1>, L.count, st, tk, ctr, a, y, fv, q, dp.fvEntries, fc)
+                     : launchFlow<1>(k_solve_fwd_fused<T, S, kFlowRhs>, L.count, st, tk, ctr, a, y, fv, q,
+                                     dp.fvEntries, fc);
+        break;
+      }
+      case kSolveBwdFused: {
+        int32_t* ctr = fc.ctr + nflow++;
+        const BlockTask* tk = dp.blocks + L.first;
+        err = q == 1 ? launchFlow<1>(k_solve_bwd_fused<T, S, 1>, L.count, st, tk, ctr, a, y, pz, q, fc)
+                     : launchFlow<1>(k_solve_bwd_fused<T, S, kFlowRhs>, L.count, st, tk, ctr, a, y, pz, q, fc);
+        break;
+      }
       case kSolveBwdFlow: {
         // backward launches keep one block per task (their column loads make a wide task's panel slow to stream)
         const BlockTask* tk = dp.blocks + (sizeof(T) == 4 ? L.first : L.first2);
@@ -1139,7 +1267,7 @@ hipError_t launchSolvePlan(const DeviceSolvePlan& dp, const SolveArgs<T>& a, con
   if (n == 0 || q == 0) return hipSuccess;
   for (int32_t k0 = 0; k0 < q; k0 += kFlowRhs) {
     const int32_t qc = std::min(kFlowRhs, q - k0);
-    hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(256), 0, st, fc.epoch, fc.ctr, fc.nctr);
+    hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(256), 0, st, fc.epoch, fc.ctr, fc.nctr + fc.ndone);
     const hipError_t e = solvePass<T, S>(dp, a, b + int64_t(k0) * n, x + int64_t(k0) * n, qc, y, fv, pz, fc, st, prof);
     if (e != hipSuccess) return e;
   }

@@ -23,7 +23,7 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (TFLOP/s): FP
 PEAK_HBM = 8000.0  # MI355X HBM3E (GB/s)
 KERNEL_CLASSES = ("zero", "assemble", "small_front", "extend_add", "diag_block", "panel_step", "gemm_tile", "split_panel",
                   "gemm_bf16x3", "solve_small_fwd", "solve_gather", "solve_fwd_flow", "solve_small_bwd", "solve_bwd_gather",
-                  "solve_bwd_init", "solve_bwd_flow", "solve_total")
+                  "solve_bwd_init", "solve_bwd_flow", "solve_fwd_fused", "solve_bwd_fused", "solve_total")
 
 
 def log(*a):

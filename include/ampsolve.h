@@ -207,13 +207,14 @@ int amp_num_release_workspace(amp_num *num);
  * fronts of a level gives each workgroup several 64-row blocks (passed on inside the workgroup) when it has at least
  * flowWideMin such tasks, else one block each; whole fronts of at most 8 (16) rows are packed 8 (4) to a wave when a
  * level has at least packMin of them; with fusedPanel each 64-column step of a tiled front (diagonal block, panel,
- * the super-panel's later columns) is one launch, for super-panels of at most 256 columns. amp_sym_autotune times the
- * factor and one
- * solve of d_values (device, pattern order) on the device for a set of candidate thresholds, starting from the
+ * the super-panel's later columns) is one launch, for super-panels of at most 256 columns; with fusedSolve (bit 0: forward, bit 1:
+ * backward; default 3) each run of consecutive levels of tiled fronts of at most 16 blocks of 64 columns whose solve
+ * tasks are one block wide is solved by one launch per direction (its gather or initialisation tasks among the flow
+ * tasks, handed over through counters) instead of two launches per level and direction. amp_sym_autotune times the factor and one solve of d_values (device, pattern order) on the device for a set of candidate thresholds, starting from the
  * defaults and changing one threshold at a time, and keeps the fastest; amp_sym_set_params applies given thresholds.
  * Both rebuild the plans, so they refuse (AMP_ERR_STATE) while numeric handles of the analysis exist. */
 typedef struct {
-  int32_t smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin, fusedPanel;
+  int32_t smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin, fusedPanel, fusedSolve;
   int32_t candidates;        /* out: plans timed */
   double defaultMs, bestMs;  /* out: factor + solve device time with the default and with the chosen thresholds */
 } amp_tune_info;

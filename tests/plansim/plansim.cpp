@@ -305,6 +305,7 @@ void* ps_create_params(int32_t n, const int32_t* rowptr, const int32_t* colidx, 
     pp.flowWideMin = prm[4];
     pp.packMin = prm[5];
     pp.fusedPanel = prm[6];
+    pp.fusedSolve = prm[7];
   }
   h->F = buildFactorPlan(h->S, pp);
   h->SP = buildSolvePlan(h->S, h->F, pp);
@@ -678,27 +679,37 @@ static int32_t flowRange(const SolveLaunch& L, int32_t list, int64_t& first, int
 
 // Ordering contract of the tiled-supernode solve launches, for the task lists of both precisions (their workgroups wait on flags
 // raised by earlier tasks of the same launch, so an order violation would be a deadlock on the device): returns the
-// number of violations.
+// number of violations. Fused launches also wait on counters: every wait must name exactly the tasks whose stores the
+// waiting task reads (the below tasks of a supernode's children for its gather tasks, its gather tasks for its flow
+// tasks; backward, the chain tasks of its parent for its initialisation tasks, those for its chain tasks), all listed
+// before it, and a counter is raised by the tasks of one launch only.
 int64_t ps_check_flow(void* p) {
   Handle* h = static_cast<Handle*>(p);
   const Symbolic& S = h->S;
   const SolvePlan& P = h->SP;
   int64_t bad = 0;
+  auto front = [&S](int32_t s, int32_t& nc, int32_t& m, int32_t& nb) {
+    nc = S.superStart[s + 1] - S.superStart[s];
+    m = S.Sp[s + 1] - S.Sp[s];
+    nb = (nc + kTileEdge - 1) / kTileEdge;
+  };
   for (const int32_t nq : {kFlowQuadsF32, kFlowQuadsF64}) {
     std::vector<int32_t> flag0(S.nsuper, -1);
     std::vector<int8_t> seenFwd(size_t(P.flowFlags), 0), seenBwd(size_t(P.flowFlags), 0);
     for (const SolveLaunch& L : P.launches) {
-      if (L.kind != kSolveFwdFlow && L.kind != kSolveBwdFlow) continue;
-      const bool fwd = L.kind == kSolveFwdFlow;
+      if (!solveKindIsFlow(L.kind)) continue;
+      const bool fwd = solveKindIsForward(L.kind), fused = L.kind == kSolveFwdFused || L.kind == kSolveBwdFused;
       std::vector<int8_t>& seen = fwd ? seenFwd : seenBwd;
       std::map<int32_t, std::vector<int8_t>> done;  // per supernode: blocks solved so far in this launch
       int64_t first, count;
       const int32_t width = flowRange(L, nq, first, count);
-      bad += width != 1 && (width != nq || !fwd);
+      bad += width != 1 && (width != nq || !fwd || fused);
+      bad += fused && (L.first2 != L.first || L.count2 != L.count);
       for (int64_t t = first; t < first + count; t++) {
         const BlockTask& b = P.blocks[t];
-        const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], nb = (nc + kTileEdge - 1) / kTileEdge;
-        const int32_t m = S.Sp[b.s + 1] - S.Sp[b.s];
+        int32_t nc, m, nb;
+        front(b.s, nc, m, nb);
+        if (fused && ((fwd && b.writer < 0) || (!fwd && b.writer == 0))) continue;  // gather / initialisation task
         if (flag0[b.s] == -1) flag0[b.s] = b.flag0;
         if (flag0[b.s] != b.flag0 || b.flag0 < 0 || b.flag0 + nb > P.flowFlags) bad++;
         auto& d = done[b.s];
@@ -729,36 +740,104 @@ int64_t ps_check_flow(void* p) {
       }
     }
     for (int64_t i = 0; i < P.flowFlags; i++) bad += (seenFwd[i] != 1) + (seenBwd[i] != 1);
-    // every initialisation slot is written by exactly one task of an earlier launch than the flow task that reads it
-    std::vector<int32_t> written(size_t(P.initSlots), -1);
+    // every initialisation slot is written by exactly one task listed before the flow task that reads it (in an
+    // earlier launch, or earlier in the same fused launch)
+    std::vector<int64_t> written(size_t(P.initSlots), -1);
+    int64_t seq = 0;  // position of a task in the order of all launches
     for (size_t li = 0; li < P.launches.size(); li++) {
       const SolveLaunch& L = P.launches[li];
-      if (L.kind == kSolveBwdInit) {
-        for (int64_t t = L.first; t < L.first + L.count; t++) {
-          const BlockTask& b = P.blocks[t];
+      if (L.kind != kSolveBwdInit && L.kind != kSolveBwdFlow && L.kind != kSolveBwdFused) continue;
+      int64_t first = L.first, count = L.count;
+      if (L.kind != kSolveBwdInit) flowRange(L, nq, first, count);
+      for (int64_t t = first; t < first + count; t++, seq++) {
+        const BlockTask& b = P.blocks[t];
+        if (L.kind == kSolveBwdInit || (L.kind == kSolveBwdFused && b.writer == 0)) {
           if (b.p0 < 0 || b.p0 >= P.initSlots || written[b.p0] != -1)
             bad++;
           else
-            written[b.p0] = int32_t(li);
+            written[b.p0] = seq;
+          continue;
         }
-      } else if (L.kind == kSolveBwdFlow) {
-        int64_t first, count;
-        flowRange(L, nq, first, count);
-        for (int64_t t = first; t < first + count; t++) {
-          const BlockTask& b = P.blocks[t];
-          const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], mu = S.Sp[b.s + 1] - S.Sp[b.s] - nc;
-          if (b.np != (mu + kInitRows - 1) / kInitRows) bad++;
-          for (int32_t p = b.p0; p < b.p0 + b.writer * b.np; p++)
-            bad += p < 0 || p >= P.initSlots || written[p] < 0 || written[p] >= int32_t(li);
-        }
+        const int32_t nc = S.superStart[b.s + 1] - S.superStart[b.s], mu = S.Sp[b.s + 1] - S.Sp[b.s] - nc;
+        if (b.np != (mu + kInitRows - 1) / kInitRows) bad++;
+        for (int32_t p = b.p0; p < b.p0 + b.writer * b.np; p++)
+          bad += p < 0 || p >= P.initSlots || written[p] < 0 || written[p] >= seq;
       }
     }
-    for (int32_t w : written) bad += w < 0;
+    for (int64_t w : written) bad += w < 0;
+  }
+  // counters of the fused launches
+  std::vector<int32_t> slotLaunch(size_t(P.doneSlots), -1);
+  for (size_t li = 0; li < P.launches.size(); li++) {
+    const SolveLaunch& L = P.launches[li];
+    if (L.kind != kSolveFwdFused && L.kind != kSolveBwdFused) continue;
+    const bool fwd = L.kind == kSolveFwdFused;
+    // role of a task: 0 gather (forward) / initialisation (backward), 1 chain, 2 below (forward)
+    auto role = [&](const BlockTask& b) { return fwd ? (b.writer < 0 ? 0 : b.writer > 0 ? 1 : 2) : (b.writer == 0 ? 0 : 1); };
+    std::map<int32_t, std::vector<int64_t>> bySlot;     // counter -> tasks raising it
+    std::map<std::pair<int32_t, int>, std::vector<int64_t>> byRole;  // (supernode, role) -> tasks
+    for (int64_t t = L.first; t < L.first + L.count; t++) {
+      const BlockTask& b = P.blocks[t];
+      byRole[{b.s, role(b)}].push_back(t);
+      if (b.dslot >= 0) {
+        if (b.dslot >= P.doneSlots || (slotLaunch[b.dslot] != -1 && slotLaunch[b.dslot] != int32_t(li))) bad++;
+        else slotLaunch[b.dslot] = int32_t(li);
+        bySlot[b.dslot].push_back(t);
+      }
+    }
+    // forward gather tasks of every supernode in the launch cover its rows in slices of kGatherRows, in order
+    if (fwd)
+      for (auto& kv : byRole) {
+        if (kv.first.second != 0) continue;
+        int32_t nc, m, nb;
+        front(kv.first.first, nc, m, nb);
+        bad += int64_t(kv.second.size()) != (m + kGatherRows - 1) / kGatherRows;
+        for (size_t i = 0; i < kv.second.size(); i++) {
+          const BlockTask& b = P.blocks[kv.second[i]];
+          bad += b.r0 != int32_t(i) * kGatherRows || b.r1 != std::min(m, b.r0 + kGatherRows);
+        }
+      }
+    for (int64_t t = L.first; t < L.first + L.count; t++) {
+      const BlockTask& b = P.blocks[t];
+      const int r = role(b);
+      std::vector<int64_t> need;  // the tasks whose stores this task reads
+      auto add = [&](int32_t s, int rl) {
+        auto it = byRole.find({s, rl});
+        if (it != byRole.end()) need.insert(need.end(), it->second.begin(), it->second.end());
+      };
+      if (fwd && r == 0) {
+        for (int32_t c = S.childHead[b.s]; c != -1; c = S.childNext[c]) add(c, 2);
+      } else if (fwd) {
+        add(b.s, 0);
+        bad += need.empty();  // every supernode of a forward fused launch is gathered in it
+      } else if (r == 0) {
+        if (S.superParent[b.s] >= 0) add(S.superParent[b.s], 1);
+      } else {
+        add(b.s, 0);
+      }
+      std::sort(need.begin(), need.end());
+      std::vector<int64_t> got;
+      if (b.wslot >= 0) {
+        auto it = bySlot.find(b.wslot);
+        if (it != bySlot.end()) got = it->second;
+      }
+      std::sort(got.begin(), got.end());
+      bad += got != need;
+      bad += b.wslot >= 0 ? b.wcount != int32_t(got.size()) : !need.empty();
+      for (int64_t u : got) bad += u >= t;
+    }
   }
   return bad;
 }
 
-// Task lists of the flow launches that ps_solve simulates: those of FP32 factors (kFlowQuadsF32) or FP64 factors.
+// Solve launches of each kind (out[0 .. 8], SolveKind) and the counters of the fused launches (out[9]).
+void ps_solve_launches(void* p, int64_t* out) {
+  Handle* h = static_cast<Handle*>(p);
+  for (int i = 0; i < 10; i++) out[i] = 0;
+  for (const SolveLaunch& L : h->SP.launches) out[L.kind]++;
+  out[9] = h->SP.doneSlots;
+}
+
 void ps_set_flow_width(void* p, int32_t nq) { static_cast<Handle*>(p)->flowWidth = nq; }
 
 // Whole fronts in launches that pack several to a wave: out[0] / out[1] factor fronts packed by 8 / 16 lanes,
@@ -798,19 +877,23 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
     for (int32_t i = 0; i < n; i++) y[size_t(k) * n + i] = b[size_t(k) * n + S.perm[i]];
   bool scaled = false;
   for (const SolveLaunch& L : P.launches) {
-    if (c.ldlt && !scaled &&
-        (L.kind == kSolveSmallBwd || L.kind == kSolveBwdGather || L.kind == kSolveBwdInit || L.kind == kSolveBwdFlow)) {
+    if (c.ldlt && !scaled && !solveKindIsForward(L.kind)) {
       for (int32_t k = 0; k < q; k++)
         for (int32_t j = 0; j < n; j++) y[size_t(k) * n + j] /= c.D[j];
       scaled = true;
     }
     int64_t first = L.first, count = L.count;
-    if (L.kind == kSolveFwdFlow || L.kind == kSolveBwdFlow) flowRange(L, h->flowWidth, first, count);
+    if (solveKindIsFlow(L.kind)) flowRange(L, h->flowWidth, first, count);
     for (int64_t t = first; t < first + count; t++) {
+      // a fused launch's task runs as the task of its kind (gather rows of one slice, initialisation, flow)
+      SolveKind kind = L.kind;
+      if (kind == kSolveFwdFused) kind = P.blocks[t].writer < 0 ? kSolveGather : kSolveFwdFlow;
+      if (kind == kSolveBwdFused) kind = P.blocks[t].writer == 0 ? kSolveBwdInit : kSolveBwdFlow;
+      const bool slice = L.kind == kSolveFwdFused && kind == kSolveGather;
       for (int32_t k = 0; k < q; k++) {
         double* yk = y.data() + size_t(k) * n;
         double* fk = fv.data() + size_t(k) * P.fvEntries;
-        if (L.kind == kSolveBwdInit) {
+        if (kind == kSolveBwdInit) {
           const BlockTask& bt = P.blocks[t];
           const int32_t s = bt.s, nc = S.superStart[s + 1] - S.superStart[s], m = S.Sp[s + 1] - S.Sp[s];
           const double* Ls = Lx + S.Lxp[s];
@@ -821,12 +904,12 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
               for (int32_t r = bt.r0; r < bt.r1; r++) sum += Ls[nc + r + int64_t(col) * m] * yk[S.Si[S.Sp[s] + nc + r]];
             pz[(size_t(bt.p0) * q + k) * kTileEdge + j] = sum;
           }
-        } else if (L.kind == kSolveBwdGather) {
+        } else if (kind == kSolveBwdGather) {
           const int32_t s = P.fronts[t];
           const int32_t nc = S.superStart[s + 1] - S.superStart[s], m = S.Sp[s + 1] - S.Sp[s];
           for (int32_t r = nc; r < m; r++) fk[S.Sp[s] + r] = yk[S.Si[S.Sp[s] + r]];
-        } else if (L.kind == kSolveSmallFwd || L.kind == kSolveGather) {
-          const int32_t s = P.fronts[t];
+        } else if (kind == kSolveSmallFwd || kind == kSolveGather) {
+          const int32_t s = slice ? P.blocks[t].s : P.fronts[t];
           const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
           std::vector<double> v(m);
           for (int32_t i = 0; i < m; i++) v[i] = i < nc ? yk[f + i] : 0.0;
@@ -834,7 +917,7 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
             const int32_t ncc = S.superStart[ch + 1] - S.superStart[ch], muc = S.Sp[ch + 1] - S.Sp[ch] - ncc;
             for (int32_t r = 0; r < muc; r++) v[F.rel[S.Sp[ch] + ncc + r]] += fk[S.Sp[ch] + ncc + r];
           }
-          if (L.kind == kSolveSmallFwd) {
+          if (kind == kSolveSmallFwd) {
             const double* Ls = Lx + S.Lxp[s];
             for (int32_t col = 0; col < nc; col++) {
               const double xc = c.ldlt ? v[col] : v[col] / Ls[col + int64_t(col) * m];
@@ -843,8 +926,9 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
             }
             for (int32_t i = 0; i < nc; i++) yk[f + i] = v[i];
           }
-          for (int32_t i = 0; i < m; i++) fk[S.Sp[s] + i] = v[i];
-        } else if (L.kind == kSolveFwdFlow) {
+          const int32_t i0 = slice ? P.blocks[t].r0 : 0, i1 = slice ? P.blocks[t].r1 : m;
+          for (int32_t i = i0; i < i1; i++) fk[S.Sp[s] + i] = v[i];
+        } else if (kind == kSolveFwdFlow) {
           // tasks run in list order: every block a task consumes was solved by an earlier task or, in a chain task,
           // by the task itself before
           const BlockTask& bt = P.blocks[t];
@@ -873,7 +957,7 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
               yk[f + r] = sum;
             }
           }
-        } else if (L.kind == kSolveSmallBwd) {
+        } else if (kind == kSolveSmallBwd) {
           const int32_t s = P.fronts[t];
           const int32_t f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
           const double* Ls = Lx + S.Lxp[s];
@@ -889,7 +973,7 @@ void ps_solve(void* p, const double* b, double* x, int32_t q) {
             for (int32_t i = 0; i < col; i++) z[i] -= Ls[col + int64_t(i) * m] * xc;
           }
           for (int32_t i = 0; i < nc; i++) yk[f + i] = z[i];
-        } else if (L.kind == kSolveBwdFlow) {
+        } else if (kind == kSolveBwdFlow) {
           const BlockTask& bt = P.blocks[t];
           const int32_t s = bt.s, f = S.superStart[s], nc = S.superStart[s + 1] - f, m = S.Sp[s + 1] - S.Sp[s];
           const double* Ls = Lx + S.Lxp[s];

@@ -39,6 +39,7 @@ def sim():
     L.ps_check_streams.restype = C.c_int64
     L.ps_packed.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.ps_packed.restype = C.c_int64
+    L.ps_solve_launches.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     return L
 
 
@@ -62,19 +63,20 @@ CASES = {
 }
 
 
-DEFAULT_PRM = (96, 32, 512, 256, 160, 512, 1)  # smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin, packMin,
-                                               # fusedPanel
+DEFAULT_PRM = (96, 32, 512, 256, 160, 512, 1, 3)  # smallMax, smallNarrow, wideLaunch, superPanel, flowWideMin,
+                                                  # packMin, fusedPanel, fusedSolve (bits: forward, backward)
 
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("fused", [1, 0])
 def test_plan_cholesky_and_ldlt(sim, name, fused):
-    """The plan replayed on the CPU, panel steps as one fused launch each (fused = 1) or as three launches, against
-    the oracle's Cholesky and LDL^T factors and solves."""
+    """The plan replayed on the CPU, panel steps as one fused launch each and runs of tiled levels solved by one
+    launch per direction (fused = 1), or as three launches per step and two per level and direction, against the
+    oracle's Cholesky and LDL^T factors and solves."""
     pat, v = CASES[name]()
     n = pat["n"]
     S = O.Sym(pat)
-    pa = np.array(DEFAULT_PRM[:6] + (fused,), np.int32)
+    pa = np.array(DEFAULT_PRM[:6] + (fused, 3 * fused), np.int32)
     h = sim.ps_create_params(n, pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
     try:
         assert sim.ps_check_flow(h) == 0  # every solve task waits only on tasks listed before it
@@ -108,6 +110,32 @@ def test_plan_cholesky_and_ldlt(sim, name, fused):
             assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
     finally:
         sim.ps_free(h)
+
+
+@pytest.mark.parametrize("nx,fwd,init,bwd", [(20, 9, 8, 9), (30, 12, 12, 12), (61, 0, 0, 0)])
+def test_fused_solve_launches(sim, nx, fwd, init, bwd):
+    """Runs of levels of tiled fronts take one launch per direction: on the 3D heat problem the top levels (whose flow
+    tasks are one block wide and whose fronts have at most 16 blocks) become one forward and one backward launch in
+    place of a gather (initialisation) and a flow launch per level, and their counters obey the contract of
+    ps_check_flow. The root level has no initialisation launch (20^3: the root is in the run); at 61^3 the fronts near
+    the root are too large and nothing is fused."""
+    pat = W.heat3d(nx)[0]
+    kinds = []
+    for fs in (3, 0):
+        pa = np.array(DEFAULT_PRM[:7] + (fs,), np.int32)
+        h = sim.ps_create_params(pat["n"], pat["rowptr"], pat["colidx"], None, 1, pa.ctypes.data)
+        try:
+            out = (C.c_int64 * 10)()
+            sim.ps_solve_launches(h, out)
+            kinds.append(list(out))
+            assert sim.ps_check_flow(h) == 0
+        finally:
+            sim.ps_free(h)
+    on, off = kinds
+    assert on[7] == on[8] == (1 if fwd else 0) and (on[9] > 0) == (fwd > 0)  # one fused launch per direction
+    assert off[7] == off[8] == off[9] == 0
+    assert off[1] - on[1] == fwd and off[2] - on[2] == fwd  # gather and forward flow launches folded in
+    assert off[6] - on[6] == init and off[4] - on[4] == bwd
 
 
 def test_plan_indefinite_ldlt_inertia(sim):
