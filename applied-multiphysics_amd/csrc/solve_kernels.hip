@@ -653,11 +653,16 @@ __global__ __launch_bounds__(256) void k_solve_small_bwd(const int32_t* __restri
 #pragma unroll
   for (int k = 0; k < Q; k++) zr[k] = (own && k < q) ? y[int64_t(k) * a.n + f + i] : S(0);
   if (own) {
-    for (int r = 0; r < mu; r++) {
-      const S l = S(Li[nc + r]);
+    // eight entries of the column's below part loaded before any is applied (one round trip per eight rows)
+    for (int r0 = 0; r0 < mu; r0 += 8) {
+      S lv[8];
 #pragma unroll
-      for (int k = 0; k < Q; k++)
-        if (k < q) zr[k] -= l * xb[k][r];
+      for (int u = 0; u < 8; u++) lv[u] = r0 + u < mu ? S(Li[nc + r0 + u]) : S(0);
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+          if (k < q && r0 + u < mu) zr[k] -= lv[u] * xb[k][r0 + u];
     }
   }
   T lcur = (own && nc > 0) ? Li[nc - 1] : T(0);  // L[col][i] for col = nc - 1
