@@ -360,6 +360,29 @@ int runSolve(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
 namespace amp {
 int solveOnDevice(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) { return runSolve(num, d_b, d_x, nrhs); }
 
+int solveOnDeviceNoSync(amp_num* num, const double* d_b, double* d_x, int32_t nrhs) {
+  if (!num->ok) return fail(AMP_ERR_STATE, "solve: no successful numeric factorization");
+  int rc = ensureVecs(num, nrhs);
+  if (rc) return rc;
+  if (num->prec == AMP_F64)
+    HIP_TRY((amp::launchSolvePlan<double, double>(num->sym->dsp, solveArgs<double>(num), d_b, d_x, nrhs,
+                                                  num->y.as<double>(), num->fv.as<double>(), num->pz.as<double>(),
+                                                  num->fc, num->stream, nullptr)));
+  else
+    HIP_TRY((amp::launchSolvePlan<float, float>(num->sym->dsp, solveArgs<float>(num), d_b, d_x, nrhs,
+                                                num->y.as<float>(), num->fv.as<float>(), num->pz.as<float>(), num->fc,
+                                                num->stream, nullptr)));
+  HIP_TRY(hipGetLastError());
+  return AMP_OK;
+}
+
+int solveWaitFailed(amp_num* num) {
+  int32_t waitErr = 0;
+  HIP_TRY(hipMemcpyAsync(&waitErr, num->fc.err, 4, hipMemcpyDeviceToHost, num->stream));
+  HIP_TRY(hipStreamSynchronize(num->stream));
+  return waitErr ? fail(AMP_ERR_INTERNAL, "solve: a block dependency wait timed out") : AMP_OK;
+}
+
 SolveScratch solveScratchNeed(const amp_num* num, int32_t nrhs) {
   const amp_sym* h = num->sym;
   const size_t es = num->prec == AMP_F64 ? 8 : 4;

@@ -66,6 +66,11 @@ hipError_t upload(DevBuf& d, const std::vector<T>& v) {
 }
 
 int solveOnDevice(amp_num* num, const double* d_b, double* d_x, int32_t nrhs);
+// The solve's launches on the handle's stream, with no synchronisation: the caller clears the wait-failure flag
+// (launchClearFlag) before its first such solve and checks it (solveWaitFailed) after its last, for drivers that solve
+// once per iteration and synchronise for their own scalars anyway.
+int solveOnDeviceNoSync(amp_num* num, const double* d_b, double* d_x, int32_t nrhs);
+int solveWaitFailed(amp_num* num);  // reads the flag (synchronises the handle's stream): AMP_OK or AMP_ERR_INTERNAL
 
 // Bytes of the solve scratch (working-precision y, front vectors fv, backward partial products pz, each rounded up to
 // 256 bytes) that nrhs right-hand sides need, and the solve with scratch supplied by the caller (no allocation).

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <vector>
 
+#include "factor_kernels.h"
 #include "handles.h"
 #include "refine_kernels.h"
 
@@ -244,7 +245,8 @@ int runPcg(amp_num* num, const double* d_values, const double* d_b, double* d_x,
   double rel = rn / bn, rz = 0, pq = 0;
   int32_t it = 0;
   if (rel > o->tol && o->maxIter > 0) {
-    if ((rc = amp::solveOnDevice(num, r, z, 1))) return rc;
+    amp::launchClearFlag(num->fc.err, num->stream);  // the solves' wait-failure flag, read once at the end
+    if ((rc = amp::solveOnDeviceNoSync(num, r, z, 1))) return rc;
     HIP_TRY(hipMemcpyAsync(p, z, vb, hipMemcpyDeviceToDevice, num->stream));
     if ((rc = c.dots(r, 0, 1, z, &rz))) return rc;
     for (it = 1; it <= o->maxIter; it++) {
@@ -257,7 +259,7 @@ int runPcg(amp_num* num, const double* d_values, const double* d_b, double* d_x,
       if ((rc = c.norm(r, &rn))) return rc;
       rel = rn / bn;
       if (rel <= o->tol) break;
-      if ((rc = amp::solveOnDevice(num, r, z, 1))) return rc;
+      if ((rc = amp::solveOnDeviceNoSync(num, r, z, 1))) return rc;
       double rzNew = 0;
       if ((rc = c.dots(r, 0, 1, z, &rzNew))) return rc;
       amp::launchAxpby(1.0, z, rzNew / rz, p, n, num->stream);  // p = z + beta p
@@ -266,6 +268,7 @@ int runPcg(amp_num* num, const double* d_values, const double* d_b, double* d_x,
     it = std::min(it, o->maxIter);
     if ((rc = c.residual(d_b, d_x, r, &rn))) return rc;  // true residual at exit
     rel = rn / bn;
+    if ((rc = amp::solveWaitFailed(num))) return rc;
   }
   HIP_TRY(hipEventRecord(num->evR1, num->stream));
   HIP_TRY(hipStreamSynchronize(num->stream));
@@ -358,7 +361,8 @@ int runPcgBatch(amp_num* num, const double* d_values, const double* d_b, int64_t
     return true;
   };
   if (!allDone()) {
-    if ((rc = amp::solveOnDevice(num, r, z, nb))) return rc;
+    amp::launchClearFlag(num->fc.err, st);  // the solves' wait-failure flag, read once after the iterations
+    if ((rc = amp::solveOnDeviceNoSync(num, r, z, nb))) return rc;
     HIP_TRY(hipMemcpyAsync(p, z, vb, hipMemcpyDeviceToDevice, st));
     if ((rc = dots(r, z, rz))) return rc;
     std::vector<double> alpha(nb), negAlpha(nb), beta(nb), pq(nb), rzNew(nb);
@@ -379,7 +383,7 @@ int runPcgBatch(amp_num* num, const double* d_values, const double* d_b, int64_t
       for (int32_t j = 0; j < nb; j++)
         if (!done[j] && !(std::sqrt(rn[j]) / bn[j] > o->tol)) done[j] = 1;
       if (allDone()) break;
-      if ((rc = amp::solveOnDevice(num, r, z, nb))) return rc;
+      if ((rc = amp::solveOnDeviceNoSync(num, r, z, nb))) return rc;
       if ((rc = dots(r, z, rzNew))) return rc;
       std::vector<double> one(nb, 1.0);
       for (int32_t j = 0; j < nb; j++) {
@@ -392,6 +396,7 @@ int runPcgBatch(amp_num* num, const double* d_values, const double* d_b, int64_t
       amp::launchBatchAxpby(dd + nb, z, dd + 2 * nb, p, n, nb, st);
     }
   }
+  if (it > 0 && (rc = amp::solveWaitFailed(num))) return rc;
   // true residuals at exit
   amp::launchBatchResidual(rp, ci, d_values, nnz, d_x, d_b, bStride, r, n, nb, st);
   if ((rc = dots(r, r, rn))) return rc;
